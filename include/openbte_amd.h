@@ -1,0 +1,235 @@
+/*
+ * libopenbte_amd — C ABI of the MI355X-native steady-state BTE path.
+ *
+ * Every function returns 0 on success and a negative OBTE_E* code on
+ * failure; obte_last_error() returns the calling thread's last message.
+ * Device pointers are plain HIP device addresses (e.g. torch
+ * tensor.data_ptr()); every launch takes the caller's hipStream_t.  No entry
+ * point allocates device memory: workspaces come in through the plan.
+ * All floating-point data is FP64, all indices int32 unless noted.
+ *
+ * The reference path is pure Python; its "FFI" for this path is the pair of
+ * Python entries BTE_RTA (openbte/bte.py:19-27) and Fourier
+ * (openbte/fourier.py:15-24).  Each entry below replaces one numerical stage
+ * inside those functions; the ctypes binding lives in openbte_amd/_lib.py.
+ */
+#ifndef OPENBTE_AMD_H
+#define OPENBTE_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void* obte_stream_t; /* hipStream_t */
+
+enum {
+  OBTE_OK = 0,
+  OBTE_EINVAL = -1,
+  OBTE_ENOMEM = -2,
+  OBTE_EHIP = -3,
+  OBTE_EINTERNAL = -4,
+  OBTE_ELIMIT = -5
+};
+
+const char* obte_last_error(void);
+
+/* ---- host scheduling (replaces the implicit ordering inside SuperLU,
+ *      openbte/bte.py:183-185) ------------------------------------------- */
+
+/* Topological sweep order of every direction's upwind graph with the
+ * periodic inflow faces (cut_candidate[f] != 0) cut.  i_mat/j_mat/k_mat are
+ * the directed internal faces of openbte/bte.py:48-54 (k = n A / V_row). */
+int obte_sched_create(int32_t n, int64_t n_faces, int32_t dim, const int32_t* i_mat,
+                      const int32_t* j_mat, const double* k_mat, const uint8_t* cut_candidate,
+                      int32_t n_dirs, const double* dirs, void** handle);
+/* sizes[3*l + {0,1,2}] = levels, inflow entries, cut faces of direction l. */
+int obte_sched_sizes(void* handle, int64_t* sizes);
+int obte_sched_export(void* handle, int32_t* order, int32_t* pos, int32_t* lvl_ptr, int32_t* in_ptr,
+                      int32_t* in_src, int32_t* in_face, int32_t* cut_exit, int32_t* cut_face);
+int obte_sched_destroy(void* handle);
+
+/* ---- device plan of one BTE solve ----------------------------------- */
+
+typedef struct obte_bte_plan {
+  /* sizes */
+  int32_t n;          /* cells */
+  int32_t n_dirs;     /* L */
+  int32_t n_mfp;      /* M */
+  int32_t dim;        /* 2 or 3 */
+  int32_t mc;         /* MFP bins per chunk: 8, 16, 32 or 64 */
+  int32_t n_chunks;   /* ceil(M / mc) */
+  int32_t block;      /* threads per sweep workgroup (multiple of 64, <= 1024) */
+  int32_t n_cut;      /* total cut faces over all directions */
+  int32_t n_wall;     /* diffuse wall sides */
+  int32_t n_contact;  /* sides of the kappa contact */
+  int32_t n_faces;    /* directed internal faces */
+  int32_t contact_kind; /* 0 none, 1 periodic (e1,e2,alpha), 2 mixed (e1, alpha) */
+  double normalization; /* alpha-hat of EffectiveThermalConductivity */
+  double contact_jump;  /* alpha of openbte/bte.py:151,156 */
+
+  /* schedule (per direction, concatenated; see obte_sched_export) */
+  const int32_t* order;    /* [L][n] cell at position */
+  const int32_t* pos;      /* [L][n] position of cell */
+  const int32_t* lvl_off;  /* [L+1] offsets into lvl_ptr */
+  const int32_t* lvl_ptr;  /* level start positions */
+  const int32_t* in_ptr;   /* [L][n+1] local entry pointers */
+  const int64_t* in_off;   /* [L+1] entry offsets */
+  const int32_t* in_src;   /* upstream position or ~cut */
+  const int32_t* in_face;  /* directed face of each entry */
+  const int32_t* cut_off;  /* [L+1] cut offsets */
+  const int32_t* cut_exit; /* position of the upstream (exit) cell of each cut */
+
+  /* geometry / material */
+  const double* dirs;      /* [L][dim] s_l */
+  const double* k_face;    /* [n_faces][dim] n A / V_row */
+  const double* jump_face; /* [n_faces] periodic jump with the sign of openbte/bte.py:106,110 */
+  const int32_t* cf_ptr;   /* [n+1] directed faces whose row is cell c */
+  const int32_t* cf_face;
+  const int32_t* wall_ptr; /* [n+1] wall entries of cell c (diffuse and mixed) */
+  const int32_t* wall_side;/* diffuse index (>= 0) or -1 for a mixed wall */
+  const double* wall_db;   /* [n_wallent][dim] n A / V */
+  const double* wall_temp; /* [n_wallent] T_wall of mixed walls (0 for diffuse) */
+  const int32_t* wall_cell;/* [n_wall] cell of each diffuse side */
+  const double* wall_rinv; /* [n_wall] 1 / sum_ml min(sigma.db, 0) (0 if the sum is 0) */
+  const double* wall_dbs;  /* [n_wall][dim] db of each diffuse side */
+  const int32_t* ce1;      /* [n_contact] */
+  const int32_t* ce2;      /* [n_contact] (periodic contact only) */
+  const double* c_na;      /* [n_contact][dim] normal areas of the contact */
+  const double* mfps;      /* [M] Lambda_m (nm) */
+  const double* t_coeff;   /* [M][L] */
+  const double* sigma;     /* [M][L][dim] */
+  const double* heat;      /* [n] heat_source * heat_source_coeff */
+  double nano;             /* 1 for the nanowire perturbation -s_z (3D), else 0 */
+
+  /* assembled operator (written by obte_bte_assemble) */
+  double* in_coef;         /* entries: -min(s . k, 0) */
+  double* diag;            /* [L][n] D_l in position order */
+  double* pert;            /* [L][n] perturbation in position order */
+  double* q;               /* [L][n] pert + diffuse-wall source (per outer iteration) */
+
+  /* state */
+  const double* delta_t;   /* [n] lattice temperature of the previous iteration */
+  const double* tb;        /* [n_wall] wall temperatures of the previous iteration */
+  double* t;               /* [L][n_chunks][n][mc] solution workspace */
+  double* part_t;          /* [L][n_chunks][n] */
+  double* part_j;          /* [L][n_chunks][n][dim] */
+  double* part_tb;         /* [L][n_chunks][n_wall] */
+  double* kappa;           /* [M][L] */
+  double* supp;            /* [M][L] */
+  double* scratch;         /* >= 4096 doubles of reduction scratch */
+} obte_bte_plan_t;
+
+/* Face-flux assembly of D_l, the inflow coefficients and the perturbation
+ * (openbte/bte.py:47-124). */
+int obte_bte_assemble(const obte_bte_plan_t* plan, obte_stream_t stream);
+
+/* Per-iteration source q = pert + Lambda-free diffuse-wall term -TB Gbm2
+ * (openbte/bte.py:199-200,211-214). */
+int obte_bte_source(const obte_bte_plan_t* plan, obte_stream_t stream);
+
+/* Level-scheduled upwind sweeps of all (l, m) systems (openbte/bte.py:217-218).
+ * mode 0: source + interface inflow y -> exit values x
+ * mode 1: zero source, interface inflow y -> exit values x (transfer operator)
+ * mode 2: as 0, plus the mode sums of openbte/bte.py:222-243 into the
+ *         partial buffers and kappa / supp.
+ * y, x: [n_chunks][n_cut][mc].  active: [L][n_chunks] (or NULL = all). */
+int obte_bte_sweep(const obte_bte_plan_t* plan, int32_t mode, const double* y, double* x,
+                   const int32_t* active, obte_stream_t stream);
+
+/* Interface Krylov layer: GCR with a recycled basis on the cut-face
+ * equations y = R S(B, y).  The transfer operator does not depend on the
+ * outer iteration, so the basis (u_k, c_k = (I - T_op) u_k, c orthonormal)
+ * is kept across iterations. */
+typedef struct obte_gcr {
+  int32_t kmax;        /* basis capacity */
+  double rtol;         /* convergence: ||r||_2 <= rtol * max(1, ||y||_inf) * sqrt(q) */
+  double* y;           /* [n_chunks][n_cut][mc] interface values */
+  double* r;           /* residual */
+  double* x;           /* exit values of the last sweep */
+  double* u;           /* [kmax][n_chunks][n_cut][mc] */
+  double* c;           /* [kmax][n_chunks][n_cut][mc] */
+  int32_t* k;          /* [L][n_chunks] basis size */
+  int32_t* active;     /* [L][n_chunks] 1 while some lane is unconverged */
+  int32_t* lane_active;/* [L][n_chunks][mc] per-system flag */
+  double* w;           /* [n_chunks][n_cut][mc] inflow of the extension sweep */
+  int32_t* n_active;   /* [1] number of active (l, chunk) groups */
+  double* rnorm;       /* [L][M] residual norms (diagnostics) */
+} obte_gcr_t;
+
+/* r = x - y, project r on the recycled basis, update y; set active flags. */
+int obte_gcr_project(const obte_bte_plan_t* plan, const obte_gcr_t* g, obte_stream_t stream);
+/* Copy r of active groups into basis slot k (the next sweep's inflow). */
+int obte_gcr_begin_extend(const obte_bte_plan_t* plan, const obte_gcr_t* g, obte_stream_t stream);
+/* c = u - x, orthogonalise against the basis, normalise, update y and r. */
+int obte_gcr_end_extend(const obte_bte_plan_t* plan, const obte_gcr_t* g, obte_stream_t stream);
+/* Clear the basis (k = 0) and y. */
+int obte_gcr_reset(const obte_bte_plan_t* plan, const obte_gcr_t* g, obte_stream_t stream);
+
+/* Fixed-order reduction of the partial buffers into DeltaT, J, TB
+ * (openbte/bte.py:245-249 -> openbte/objects.py:86-101); norm2 = ||DeltaT||^2. */
+int obte_bte_reduce(const obte_bte_plan_t* plan, double* delta_t_new, double* j_new, double* tb_new,
+                    double* norm2, obte_stream_t stream);
+
+/* ---- Fourier (openbte/fourier.py:15-185) ------------------------------ */
+
+typedef struct obte_fourier_plan {
+  int32_t n;          /* cells */
+  int32_t dim;
+  int32_t nv;         /* sides per element */
+  int32_t n_int;      /* internal sides */
+  int32_t pin;        /* pinned cell of openbte/fourier.py:43,115-118, or -1 */
+  /* symmetric operator A = V F (+ boundary / modified terms):
+   * (A x)_i = diag_i x_i - sum_k w_k x_col[k] over row i (no diagonal in CSR) */
+  const int32_t* row_ptr;
+  const int32_t* col;
+  const double* w;
+  const double* diag;
+  const double* vol;       /* [n] element volumes */
+  /* LSQ gradient (openbte/objects.py:322-405) */
+  const int32_t* slot_cell;  /* [n][nv] neighbour cell or -1 */
+  const int32_t* slot_mode;  /* [n][nv] 0: zero, 1: T[nb] - T + val, 2: val - T */
+  const double* slot_val;    /* [n][nv] periodic jump (mode 1) or wall value (mode 2) */
+  const double* gweights;    /* [n][dim][nv] pinv of the distance rows */
+  /* non-orthogonal correction (openbte/fourier.py:161-165) */
+  const int32_t* side_e;     /* [n_int][2] */
+  const double* side_w;      /* [n_int] interpolation weights */
+  const double* side_vn;     /* [n_int][dim] v_nonorth */
+  const int32_t* slot_side;  /* [n][nv] internal side of the slot or -1 */
+  const double* slot_sgn;    /* [n][nv] +1/V (cell is side_e[.][0]) or -1/V */
+  /* diffusive kappa through the contact (openbte/objects.py:139-172) */
+  int32_t n_contact;
+  int32_t contact_kind;      /* 1 periodic, 2 mixed */
+  const int32_t* ce1;
+  const int32_t* ce2;
+  const double* c_coef;      /* v_orth (periodic) or conductance-limited v_orth (mixed) */
+  double c_value;            /* periodic jump or wall temperature */
+  double normalization;
+} obte_fourier_plan_t;
+
+/* Solve F T = B + C (openbte/fourier.py:152-156) as the symmetric system
+ * A T = V (B + C) by Jacobi-preconditioned CG, T holding the initial guess;
+ * with a pinned cell the pinned row is replaced by the compatibility
+ * condition and T is centred by (max + min) / 2.  ws: 6n + 64 doubles; scal: 16
+ * doubles.  iters_out / relres_out are host pointers (may be NULL). */
+int obte_fourier_solve(const obte_fourier_plan_t* plan, const double* b, const double* c, double* t, double* ws,
+                       double* scal, double rtol, int32_t maxit, int32_t check_every, int32_t* iters_out,
+                       double* relres_out, obte_stream_t stream);
+
+/* LSQ cell gradient of T: grad [n][dim]. */
+int obte_fourier_gradient(const obte_fourier_plan_t* plan, const double* t, double* grad, obte_stream_t stream);
+
+/* Correction C [n] from the gradient; side_tmp: n_int doubles of workspace;
+ * norm2 (device, 1 double) receives ||C||^2; ws: >= n/256+1 doubles. */
+int obte_fourier_correction(const obte_fourier_plan_t* plan, const double* grad, double* c, double* side_tmp,
+                            double* ws, double* norm2, obte_stream_t stream);
+
+/* Diffusive kappa of T through the contact into out (device, 1 double). */
+int obte_fourier_kappa(const obte_fourier_plan_t* plan, const double* t, double* out, obte_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OPENBTE_AMD_H */

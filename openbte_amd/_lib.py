@@ -1,0 +1,120 @@
+"""ctypes binding of libopenbte_amd.so (the C ABI declared in include/openbte_amd.h).
+
+The library is built in-tree by ``openbte_amd.build``; loading never builds.
+A missing library raises ``RuntimeError`` -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, 'libopenbte_amd.so')
+
+_i32 = C.c_int32
+_i64 = C.c_int64
+_f64 = C.c_double
+_p = C.c_void_p
+
+
+class BtePlan(C.Structure):
+    """Mirror of obte_bte_plan_t (field order matters)."""
+    _fields_ = [
+        ('n', _i32), ('n_dirs', _i32), ('n_mfp', _i32), ('dim', _i32), ('mc', _i32), ('n_chunks', _i32),
+        ('block', _i32), ('n_cut', _i32), ('n_wall', _i32), ('n_contact', _i32), ('n_faces', _i32),
+        ('contact_kind', _i32), ('normalization', _f64), ('contact_jump', _f64),
+        ('order', _p), ('pos', _p), ('lvl_off', _p), ('lvl_ptr', _p), ('in_ptr', _p), ('in_off', _p),
+        ('in_src', _p), ('in_face', _p), ('cut_off', _p), ('cut_exit', _p),
+        ('dirs', _p), ('k_face', _p), ('jump_face', _p), ('cf_ptr', _p), ('cf_face', _p),
+        ('wall_ptr', _p), ('wall_side', _p), ('wall_db', _p), ('wall_temp', _p), ('wall_cell', _p),
+        ('wall_rinv', _p), ('wall_dbs', _p), ('ce1', _p), ('ce2', _p), ('c_na', _p), ('mfps', _p),
+        ('t_coeff', _p), ('sigma', _p), ('heat', _p), ('nano', _f64),
+        ('in_coef', _p), ('diag', _p), ('pert', _p), ('q', _p),
+        ('delta_t', _p), ('tb', _p), ('t', _p), ('part_t', _p), ('part_j', _p), ('part_tb', _p),
+        ('kappa', _p), ('supp', _p), ('scratch', _p),
+    ]
+
+
+class Gcr(C.Structure):
+    """Mirror of obte_gcr_t."""
+    _fields_ = [
+        ('kmax', _i32), ('rtol', _f64), ('y', _p), ('r', _p), ('x', _p), ('u', _p), ('c', _p),
+        ('k', _p), ('active', _p), ('lane_active', _p), ('w', _p), ('n_active', _p), ('rnorm', _p),
+    ]
+
+
+class FourierPlan(C.Structure):
+    """Mirror of obte_fourier_plan_t."""
+    _fields_ = [
+        ('n', _i32), ('dim', _i32), ('nv', _i32), ('n_int', _i32), ('pin', _i32),
+        ('row_ptr', _p), ('col', _p), ('w', _p), ('diag', _p), ('vol', _p),
+        ('slot_cell', _p), ('slot_mode', _p), ('slot_val', _p), ('gweights', _p),
+        ('side_e', _p), ('side_w', _p), ('side_vn', _p), ('slot_side', _p), ('slot_sgn', _p),
+        ('n_contact', _i32), ('contact_kind', _i32), ('ce1', _p), ('ce2', _p), ('c_coef', _p),
+        ('c_value', _f64), ('normalization', _f64),
+    ]
+
+
+# name -> (argtypes)
+_SIGNATURES = {
+    'obte_last_error': [],
+    'obte_sched_create': [_i32, _i64, _i32, _p, _p, _p, _p, _i32, _p, C.POINTER(_p)],
+    'obte_sched_sizes': [_p, _p],
+    'obte_sched_export': [_p, _p, _p, _p, _p, _p, _p, _p, _p],
+    'obte_sched_destroy': [_p],
+    'obte_bte_assemble': [C.POINTER(BtePlan), _p],
+    'obte_bte_source': [C.POINTER(BtePlan), _p],
+    'obte_bte_sweep': [C.POINTER(BtePlan), _i32, _p, _p, _p, _p],
+    'obte_gcr_project': [C.POINTER(BtePlan), C.POINTER(Gcr), _p],
+    'obte_gcr_begin_extend': [C.POINTER(BtePlan), C.POINTER(Gcr), _p],
+    'obte_gcr_end_extend': [C.POINTER(BtePlan), C.POINTER(Gcr), _p],
+    'obte_gcr_reset': [C.POINTER(BtePlan), C.POINTER(Gcr), _p],
+    'obte_bte_reduce': [C.POINTER(BtePlan), _p, _p, _p, _p, _p],
+    'obte_fourier_solve': [C.POINTER(FourierPlan), _p, _p, _p, _p, _p, _f64, _i32, _i32,
+                           C.POINTER(_i32), C.POINTER(_f64), _p],
+    'obte_fourier_gradient': [C.POINTER(FourierPlan), _p, _p, _p],
+    'obte_fourier_correction': [C.POINTER(FourierPlan), _p, _p, _p, _p, _p, _p],
+    'obte_fourier_kappa': [C.POINTER(FourierPlan), _p, _p, _p],
+}
+
+EXPORTED = tuple(_SIGNATURES)
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load the in-tree library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f'{LIB_PATH} is missing: run `python -m openbte_amd.build` (no CPU fallback exists)')
+    lib = C.CDLL(LIB_PATH)
+    for name, args in _SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = C.c_char_p if name == 'obte_last_error' else C.c_int
+    _lib = lib
+    return lib
+
+
+class ObteError(RuntimeError):
+    pass
+
+
+def call(name: str, *args) -> None:
+    """Call a C-ABI function and raise ObteError on a negative return code."""
+    lib = load()
+    rc = getattr(lib, name)(*args)
+    if rc != 0:
+        msg = lib.obte_last_error().decode(errors='replace')
+        raise ObteError(f'{name} failed ({rc}): {msg}')
+
+
+def ptr(t) -> int:
+    """Device/host address of a torch tensor or numpy array (None -> 0)."""
+    if t is None:
+        return 0
+    if hasattr(t, 'data_ptr'):
+        return t.data_ptr()
+    return t.ctypes.data

@@ -1,0 +1,404 @@
+// FP64 HIP kernels of the Fourier initial guess (openbte/fourier.py:15-185).
+//
+// The reference factorises F = V^-1 Lap (one row pinned) with SuperLU and
+// back-solves up to 40 times.  Here F T = b is solved in its symmetric form
+// A T = V b with A = Lap (+ isothermal / modified diagonal), by
+// Jacobi-preconditioned CG: every iteration is one CSR pass (~4 nnz per row)
+// plus vector streams, i.e. HBM-bound sparse work.  The pinned row is
+// replaced by the compatibility condition of the singular Laplacian; since the
+// reference centres T by (max + min) / 2 right after the solve, the pin's
+// constant drops out.
+//
+// Reductions are deterministic: per-block partial sums in a fixed tree,
+// then one block sums the partials in order.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "capi_common.h"
+
+namespace {
+
+constexpr int kB = 256;
+
+enum { S_RZ = 0, S_PQ, S_ALPHA, S_BETA, S_RR, S_BB, S_TOT, S_MAX, S_MIN };
+
+__device__ __forceinline__ double block_sum(double v, double* red) {
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  const double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+__device__ __forceinline__ double apply_row(const obte_fourier_plan_t& P, const double* x, int i) {
+  double v = P.diag[i] * x[i];
+  for (int k = P.row_ptr[i]; k < P.row_ptr[i + 1]; ++k) v -= P.w[k] * x[P.col[k]];
+  return v;
+}
+
+// bh = V (b + c); the pinned entry is set later to minus the sum of the rest.
+__global__ void k_rhs(obte_fourier_plan_t P, const double* b, const double* c, double* bh, double* part) {
+  __shared__ double red[kB];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double v = 0.0;
+  if (i < P.n) {
+    v = P.vol[i] * (b[i] + (c ? c[i] : 0.0));
+    if (i == P.pin) v = 0.0;
+    bh[i] = v;
+  }
+  const double s = block_sum(v, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ void k_rhs_fix(obte_fourier_plan_t P, double* bh, const double* part, int nb) {
+  __shared__ double red[1024];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) s += part[i];
+  s = block_sum(s, red);
+  if (threadIdx.x == 0 && P.pin >= 0) bh[P.pin] = -s;
+}
+
+// r = bh - A x, z = r / diag, p = z; partials of r.z, r.r, bh.bh
+__global__ void k_cg_init(obte_fourier_plan_t P, const double* bh, const double* x, double* r, double* z, double* p,
+                          double* part, int nb) {
+  __shared__ double red[kB];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double rz = 0.0, rr = 0.0, bb = 0.0;
+  if (i < P.n) {
+    const double ri = bh[i] - apply_row(P, x, i);
+    const double zi = ri / P.diag[i];
+    r[i] = ri;
+    z[i] = zi;
+    p[i] = zi;
+    rz = ri * zi;
+    rr = ri * ri;
+    bb = bh[i] * bh[i];
+  }
+  rz = block_sum(rz, red);
+  rr = block_sum(rr, red);
+  bb = block_sum(bb, red);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = rz;
+    part[nb + blockIdx.x] = rr;
+    part[2 * nb + blockIdx.x] = bb;
+  }
+}
+
+__global__ void k_cg_init_fin(const double* part, int nb, double* scal) {
+  __shared__ double red[1024];
+  double a = 0.0, b = 0.0, c = 0.0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+    a += part[i];
+    b += part[nb + i];
+    c += part[2 * nb + i];
+  }
+  a = block_sum(a, red);
+  b = block_sum(b, red);
+  c = block_sum(c, red);
+  if (threadIdx.x == 0) {
+    scal[S_RZ] = a;
+    scal[S_RR] = b;
+    scal[S_BB] = c;
+  }
+}
+
+// q = A p, partial p.q
+__global__ void k_cg_spmv(obte_fourier_plan_t P, const double* p, double* q, double* part) {
+  __shared__ double red[kB];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double pq = 0.0;
+  if (i < P.n) {
+    const double qi = apply_row(P, p, i);
+    q[i] = qi;
+    pq = p[i] * qi;
+  }
+  pq = block_sum(pq, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = pq;
+}
+
+__global__ void k_cg_spmv_fin(const double* part, int nb, double* scal) {
+  __shared__ double red[1024];
+  double a = 0.0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) a += part[i];
+  a = block_sum(a, red);
+  if (threadIdx.x == 0) {
+    scal[S_PQ] = a;
+    scal[S_ALPHA] = a != 0.0 ? scal[S_RZ] / a : 0.0;
+  }
+}
+
+// x += alpha p, r -= alpha q, z = r / diag; partials r.z, r.r
+__global__ void k_cg_update(obte_fourier_plan_t P, double* x, double* r, double* z, const double* p, const double* q,
+                            const double* scal, double* part, int nb) {
+  __shared__ double red[kB];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const double alpha = scal[S_ALPHA];
+  double rz = 0.0, rr = 0.0;
+  if (i < P.n) {
+    x[i] += alpha * p[i];
+    const double ri = r[i] - alpha * q[i];
+    const double zi = ri / P.diag[i];
+    r[i] = ri;
+    z[i] = zi;
+    rz = ri * zi;
+    rr = ri * ri;
+  }
+  rz = block_sum(rz, red);
+  rr = block_sum(rr, red);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = rz;
+    part[nb + blockIdx.x] = rr;
+  }
+}
+
+__global__ void k_cg_update_fin(const double* part, int nb, double* scal) {
+  __shared__ double red[1024];
+  double a = 0.0, b = 0.0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+    a += part[i];
+    b += part[nb + i];
+  }
+  a = block_sum(a, red);
+  b = block_sum(b, red);
+  if (threadIdx.x == 0) {
+    scal[S_BETA] = scal[S_RZ] != 0.0 ? a / scal[S_RZ] : 0.0;
+    scal[S_RZ] = a;
+    scal[S_RR] = b;
+  }
+}
+
+__global__ void k_cg_dir(int n, const double* z, double* p, const double* scal) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = z[i] + scal[S_BETA] * p[i];
+}
+
+// (max + min) / 2 centring (openbte/fourier.py:155-156)
+__global__ void k_minmax(int n, const double* t, double* part, int nb) {
+  __shared__ double mx[kB], mn[kB];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  mx[threadIdx.x] = i < n ? t[i] : -INFINITY;
+  mn[threadIdx.x] = i < n ? t[i] : INFINITY;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      mx[threadIdx.x] = fmax(mx[threadIdx.x], mx[threadIdx.x + o]);
+      mn[threadIdx.x] = fmin(mn[threadIdx.x], mn[threadIdx.x + o]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = mx[0];
+    part[nb + blockIdx.x] = mn[0];
+  }
+}
+
+__global__ void k_minmax_fin(const double* part, int nb, double* scal) {
+  __shared__ double mx[1024], mn[1024];
+  double a = -INFINITY, b = INFINITY;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+    a = fmax(a, part[i]);
+    b = fmin(b, part[nb + i]);
+  }
+  mx[threadIdx.x] = a;
+  mn[threadIdx.x] = b;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      mx[threadIdx.x] = fmax(mx[threadIdx.x], mx[threadIdx.x + o]);
+      mn[threadIdx.x] = fmin(mn[threadIdx.x], mn[threadIdx.x + o]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    scal[S_MAX] = mx[0];
+    scal[S_MIN] = mn[0];
+  }
+}
+
+__global__ void k_shift(int n, double* t, const double* scal) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) t[i] -= (scal[S_MAX] + scal[S_MIN]) * 0.5;
+}
+
+__global__ void k_gradient(obte_fourier_plan_t P, const double* t, double* grad) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= P.n) return;
+  const int nv = P.nv;
+  const int dim = P.dim;
+  double dd[4];
+  const double tc = t[c];
+  for (int s = 0; s < nv; ++s) {
+    const int m = P.slot_mode[c * nv + s];
+    double v = 0.0;
+    if (m == 1) v = t[P.slot_cell[c * nv + s]] - tc + P.slot_val[c * nv + s];
+    else if (m == 2) v = P.slot_val[c * nv + s] - tc;
+    dd[s] = v;
+  }
+  for (int d = 0; d < dim; ++d) {
+    const double* w = P.gweights + ((int64_t)c * dim + d) * nv;
+    double g = 0.0;
+    for (int s = 0; s < nv; ++s) g += w[s] * dd[s];
+    grad[(int64_t)c * dim + d] = g;
+  }
+}
+
+__global__ void k_side_flux(obte_fourier_plan_t P, const double* grad, double* tmp) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= P.n_int) return;
+  const int dim = P.dim;
+  const int a = P.side_e[2 * s], b = P.side_e[2 * s + 1];
+  const double w = P.side_w[s];
+  double v = 0.0;
+  for (int d = 0; d < dim; ++d)
+    v += (w * grad[(int64_t)a * dim + d] + (1.0 - w) * grad[(int64_t)b * dim + d]) * P.side_vn[(int64_t)s * dim + d];
+  tmp[s] = v;
+}
+
+__global__ void k_correction(obte_fourier_plan_t P, const double* tmp, double* c, double* part) {
+  __shared__ double red[kB];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double v = 0.0;
+  if (i < P.n) {
+    for (int s = 0; s < P.nv; ++s) {
+      const int sd = P.slot_side[i * P.nv + s];
+      if (sd >= 0) v += P.slot_sgn[i * P.nv + s] * tmp[sd];
+    }
+    c[i] = v;
+  }
+  const double s2 = block_sum(v * v, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = s2;
+}
+
+__global__ void k_sum_fin(const double* part, int nb, double* out) {
+  __shared__ double red[1024];
+  double a = 0.0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) a += part[i];
+  a = block_sum(a, red);
+  if (threadIdx.x == 0) *out = a;
+}
+
+__global__ void k_kappa(obte_fourier_plan_t P, const double* t, double* out) {
+  __shared__ double red[1024];
+  double a = 0.0;
+  for (int s = threadIdx.x; s < P.n_contact; s += blockDim.x) {
+    const double t1 = t[P.ce1[s]];
+    const double t2 = P.contact_kind == 1 ? t[P.ce2[s]] + P.c_value : P.c_value;
+    a += (t1 - t2) * P.c_coef[s];
+  }
+  a = block_sum(a, red);
+  if (threadIdx.x == 0) *out = a * P.normalization;
+}
+
+inline unsigned nblocks(int64_t n) { return (unsigned)((n + kB - 1) / kB); }
+
+int status(const char* who) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return obte_fail(OBTE_EHIP, who);
+  return OBTE_OK;
+}
+
+int check(const obte_fourier_plan_t* P) {
+  if (!P || P->n <= 0 || (P->dim != 2 && P->dim != 3) || P->nv <= 0 || P->nv > 4)
+    return obte_fail(OBTE_EINVAL, "fourier plan: invalid sizes");
+  if (P->pin >= P->n) return obte_fail(OBTE_EINVAL, "fourier plan: pin out of range");
+  return OBTE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int obte_fourier_solve(const obte_fourier_plan_t* plan, const double* b, const double* c, double* t, double* ws,
+                       double* scal, double rtol, int32_t maxit, int32_t check_every, int32_t* iters_out,
+                       double* relres_out, obte_stream_t stream) {
+  int rc = check(plan);
+  if (rc) return rc;
+  if (!b || !t || !ws || !scal) return obte_fail(OBTE_EINVAL, "obte_fourier_solve: null pointer");
+  const obte_fourier_plan_t& P = *plan;
+  hipStream_t st = (hipStream_t)stream;
+  const int n = P.n;
+  const unsigned nb = nblocks(n);
+  double* bh = ws;
+  double* r = ws + n;
+  double* z = ws + 2 * (int64_t)n;
+  double* p = ws + 3 * (int64_t)n;
+  double* q = ws + 4 * (int64_t)n;
+  double* part = ws + 5 * (int64_t)n;  // n + 64 doubles >= 3 * nb
+  if (check_every <= 0) check_every = 16;
+
+  hipLaunchKernelGGL(k_rhs, dim3(nb), dim3(kB), 0, st, P, b, c, bh, part);
+  if (P.pin >= 0) hipLaunchKernelGGL(k_rhs_fix, dim3(1), dim3(1024), 0, st, P, bh, part, (int)nb);
+  hipLaunchKernelGGL(k_cg_init, dim3(nb), dim3(kB), 0, st, P, bh, t, r, z, p, part, (int)nb);
+  hipLaunchKernelGGL(k_cg_init_fin, dim3(1), dim3(1024), 0, st, part, (int)nb, scal);
+  rc = status("obte_fourier_solve: init launch failed");
+  if (rc) return rc;
+  double host[8];
+  if (hipMemcpyAsync(host, scal, 8 * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return obte_fail(OBTE_EHIP, "obte_fourier_solve: readback failed");
+  const double bnorm = sqrt(host[S_BB]);
+  double rnorm = sqrt(host[S_RR]);
+  int it = 0;
+  const double target = rtol * (bnorm > 0.0 ? bnorm : 1.0);
+  while (rnorm > target && it < maxit) {
+    for (int k = 0; k < check_every && it < maxit; ++k, ++it) {
+      hipLaunchKernelGGL(k_cg_spmv, dim3(nb), dim3(kB), 0, st, P, p, q, part);
+      hipLaunchKernelGGL(k_cg_spmv_fin, dim3(1), dim3(1024), 0, st, part, (int)nb, scal);
+      hipLaunchKernelGGL(k_cg_update, dim3(nb), dim3(kB), 0, st, P, t, r, z, p, q, scal, part, (int)nb);
+      hipLaunchKernelGGL(k_cg_update_fin, dim3(1), dim3(1024), 0, st, part, (int)nb, scal);
+      hipLaunchKernelGGL(k_cg_dir, dim3(nb), dim3(kB), 0, st, n, z, p, scal);
+    }
+    rc = status("obte_fourier_solve: iteration launch failed");
+    if (rc) return rc;
+    if (hipMemcpyAsync(host, scal, 8 * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      return obte_fail(OBTE_EHIP, "obte_fourier_solve: readback failed");
+    rnorm = sqrt(host[S_RR]);
+  }
+  if (P.pin >= 0) {
+    hipLaunchKernelGGL(k_minmax, dim3(nb), dim3(kB), 0, st, n, t, part, (int)nb);
+    hipLaunchKernelGGL(k_minmax_fin, dim3(1), dim3(1024), 0, st, part, (int)nb, scal);
+    hipLaunchKernelGGL(k_shift, dim3(nb), dim3(kB), 0, st, n, t, scal);
+  }
+  if (iters_out) *iters_out = it;
+  if (relres_out) *relres_out = rnorm / (bnorm > 0.0 ? bnorm : 1.0);
+  return status("obte_fourier_solve: finish launch failed");
+}
+
+int obte_fourier_gradient(const obte_fourier_plan_t* plan, const double* t, double* grad, obte_stream_t stream) {
+  int rc = check(plan);
+  if (rc) return rc;
+  if (!t || !grad) return obte_fail(OBTE_EINVAL, "obte_fourier_gradient: null pointer");
+  hipLaunchKernelGGL(k_gradient, dim3(nblocks(plan->n)), dim3(kB), 0, (hipStream_t)stream, *plan, t, grad);
+  return status("obte_fourier_gradient: launch failed");
+}
+
+int obte_fourier_correction(const obte_fourier_plan_t* plan, const double* grad, double* c, double* side_tmp,
+                            double* ws, double* norm2, obte_stream_t stream) {
+  int rc = check(plan);
+  if (rc) return rc;
+  if (!grad || !c || !side_tmp || !ws || !norm2) return obte_fail(OBTE_EINVAL, "obte_fourier_correction: null pointer");
+  hipStream_t st = (hipStream_t)stream;
+  if (plan->n_int > 0)
+    hipLaunchKernelGGL(k_side_flux, dim3(nblocks(plan->n_int)), dim3(kB), 0, st, *plan, grad, side_tmp);
+  const unsigned nb = nblocks(plan->n);
+  hipLaunchKernelGGL(k_correction, dim3(nb), dim3(kB), 0, st, *plan, side_tmp, c, ws);
+  hipLaunchKernelGGL(k_sum_fin, dim3(1), dim3(1024), 0, st, ws, (int)nb, norm2);
+  return status("obte_fourier_correction: launch failed");
+}
+
+int obte_fourier_kappa(const obte_fourier_plan_t* plan, const double* t, double* out, obte_stream_t stream) {
+  int rc = check(plan);
+  if (rc) return rc;
+  if (!t || !out) return obte_fail(OBTE_EINVAL, "obte_fourier_kappa: null pointer");
+  hipLaunchKernelGGL(k_kappa, dim3(1), dim3(1024), 0, (hipStream_t)stream, *plan, t, out);
+  return status("obte_fourier_kappa: launch failed");
+}
+
+}  // extern "C"

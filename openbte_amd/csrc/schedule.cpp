@@ -1,0 +1,242 @@
+// Host-side sweep scheduling for the upwind BTE operator.
+//
+// For every direction s_l the upwind operator A_ml = I + Lambda_m (D_l + G^-_l)
+// (openbte/bte.py:181-185) couples a cell only to the neighbours across its
+// inflow faces (s_l . n_f < 0).  Without the periodic faces that dependency
+// graph is acyclic for convex 2D cells, so cutting the periodic inflow faces
+// (openbte/mesh.py:268-296 keeps them inside `internal`) leaves a DAG that a
+// level-ordered sweep inverts exactly.  The cut couplings become interface
+// unknowns solved by the Krylov layer.  Any cycle left after that cut (possible
+// on non-Delaunay tetrahedra) is broken greedily and its faces join the
+// interface as well.
+//
+// Output per direction, in sweep ("position") order:
+//   order[p]        cell at position p (cells grouped by level)
+//   pos[c]          inverse permutation
+//   lvl_ptr[k]      first position of level k (nlev+1 entries)
+//   in_ptr[p]       CSR row pointer into the inflow entries of position p
+//   in_src[e]       upstream position (>= 0) or ~cut_index (< 0)
+//   in_face[e]      directed face id of the entry (for coefficient assembly)
+//   cut_exit[f]     position of the upstream cell of cut face f
+//   cut_face[f]     directed face id of cut face f
+#include <algorithm>
+#include <cstdint>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "capi_common.h"
+
+namespace {
+
+struct DirSchedule {
+  std::vector<int32_t> order, pos, lvl_ptr, in_ptr, in_src, in_face, cut_exit, cut_face;
+};
+
+struct Schedule {
+  int32_t n = 0, n_dirs = 0;
+  std::vector<DirSchedule> dirs;
+};
+
+// Directed-face inflow test: row cell i_mat[f] receives from j_mat[f] when
+// s . k_f < 0 (G^- of openbte/bte.py:58-60).
+inline double face_dot(const double* s, const double* k, int dim) {
+  double g = 0.0;
+  for (int d = 0; d < dim; ++d) g += s[d] * k[d];
+  return g;
+}
+
+void build_direction(int32_t n, int64_t n_faces, int dim, const int32_t* i_mat, const int32_t* j_mat,
+                     const double* k_mat, const uint8_t* cut_candidate, const double* s,
+                     DirSchedule& out) {
+  // inflow faces grouped by row cell
+  std::vector<int32_t> cnt(n + 1, 0);
+  std::vector<uint8_t> inflow(n_faces, 0);
+  for (int64_t f = 0; f < n_faces; ++f) {
+    if (face_dot(s, k_mat + f * dim, dim) < 0.0) {
+      inflow[f] = 1;
+      cnt[i_mat[f] + 1]++;
+    }
+  }
+  for (int32_t c = 0; c < n; ++c) cnt[c + 1] += cnt[c];
+  std::vector<int32_t> row_faces(cnt[n]);
+  {
+    std::vector<int32_t> fill(cnt.begin(), cnt.end() - 1);
+    for (int64_t f = 0; f < n_faces; ++f)
+      if (inflow[f]) row_faces[fill[i_mat[f]]++] = (int32_t)f;
+  }
+  std::vector<uint8_t> is_cut(n_faces, 0);
+  for (int64_t f = 0; f < n_faces; ++f)
+    if (inflow[f] && cut_candidate[f]) is_cut[f] = 1;
+
+  // downstream adjacency (upstream cell -> faces feeding others)
+  std::vector<int32_t> dcnt(n + 1, 0);
+  for (int64_t f = 0; f < n_faces; ++f)
+    if (inflow[f] && !is_cut[f]) dcnt[j_mat[f] + 1]++;
+  for (int32_t c = 0; c < n; ++c) dcnt[c + 1] += dcnt[c];
+  std::vector<int32_t> down_face(dcnt[n]);
+  {
+    std::vector<int32_t> fill(dcnt.begin(), dcnt.end() - 1);
+    for (int64_t f = 0; f < n_faces; ++f)
+      if (inflow[f] && !is_cut[f]) down_face[fill[j_mat[f]]++] = (int32_t)f;
+  }
+
+  std::vector<int32_t> indeg(n, 0);
+  for (int32_t c = 0; c < n; ++c)
+    for (int32_t e = cnt[c]; e < cnt[c + 1]; ++e)
+      if (!is_cut[row_faces[e]]) indeg[c]++;
+
+  // Kahn's algorithm by levels (longest-path level of each cell)
+  std::vector<int32_t> level(n, -1);
+  std::vector<int32_t> front, next;
+  for (int32_t c = 0; c < n; ++c)
+    if (indeg[c] == 0) front.push_back(c);
+  int32_t done = 0, lev = 0;
+  out.order.clear();
+  out.order.reserve(n);
+  out.lvl_ptr.clear();
+  while (done < n) {
+    if (front.empty()) {
+      // cycle: make the pending cell with the fewest remaining inflows a
+      // source by cutting its remaining inflow faces
+      int32_t best = -1, bestdeg = 1 << 30;
+      for (int32_t c = 0; c < n; ++c)
+        if (level[c] < 0 && indeg[c] < bestdeg) { best = c; bestdeg = indeg[c]; }
+      for (int32_t e = cnt[best]; e < cnt[best + 1]; ++e) {
+        int32_t f = row_faces[e];
+        if (!is_cut[f] && level[j_mat[f]] < 0) {
+          is_cut[f] = 1;
+          // drop the face from its upstream cell's downstream list
+          int32_t u = j_mat[f];
+          for (int32_t q = dcnt[u]; q < dcnt[u + 1]; ++q)
+            if (down_face[q] == f) down_face[q] = -1;
+        }
+      }
+      indeg[best] = 0;
+      front.push_back(best);
+    }
+    std::sort(front.begin(), front.end());
+    out.lvl_ptr.push_back(done);
+    next.clear();
+    for (int32_t c : front) {
+      level[c] = lev;
+      out.order.push_back(c);
+    }
+    done += (int32_t)front.size();
+    for (int32_t c : front) {
+      for (int32_t q = dcnt[c]; q < dcnt[c + 1]; ++q) {
+        int32_t f = down_face[q];
+        if (f < 0) continue;
+        int32_t d = i_mat[f];
+        if (--indeg[d] == 0) next.push_back(d);
+      }
+    }
+    front.swap(next);
+    ++lev;
+  }
+  out.lvl_ptr.push_back(n);
+
+  out.pos.assign(n, 0);
+  for (int32_t p = 0; p < n; ++p) out.pos[out.order[p]] = p;
+
+  // inflow CSR in position order; cut faces numbered in position order
+  out.in_ptr.assign(n + 1, 0);
+  out.in_src.clear();
+  out.in_face.clear();
+  out.cut_exit.clear();
+  out.cut_face.clear();
+  for (int32_t p = 0; p < n; ++p) {
+    int32_t c = out.order[p];
+    for (int32_t e = cnt[c]; e < cnt[c + 1]; ++e) {
+      int32_t f = row_faces[e];
+      if (is_cut[f]) {
+        int32_t q = (int32_t)out.cut_face.size();
+        out.cut_face.push_back(f);
+        out.cut_exit.push_back(out.pos[j_mat[f]]);
+        out.in_src.push_back(~q);
+      } else {
+        out.in_src.push_back(out.pos[j_mat[f]]);
+      }
+      out.in_face.push_back(f);
+    }
+    out.in_ptr[p + 1] = (int32_t)out.in_src.size();
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int obte_sched_create(int32_t n, int64_t n_faces, int32_t dim, const int32_t* i_mat, const int32_t* j_mat,
+                      const double* k_mat, const uint8_t* cut_candidate, int32_t n_dirs,
+                      const double* dirs, void** handle) {
+  try {
+    if (!handle || n <= 0 || n_faces < 0 || (dim != 2 && dim != 3) || n_dirs <= 0)
+      return obte_fail(OBTE_EINVAL, "obte_sched_create: invalid arguments");
+    for (int64_t f = 0; f < n_faces; ++f)
+      if (i_mat[f] < 0 || i_mat[f] >= n || j_mat[f] < 0 || j_mat[f] >= n)
+        return obte_fail(OBTE_EINVAL, "obte_sched_create: face cell index out of range");
+    auto* s = new Schedule();
+    s->n = n;
+    s->n_dirs = n_dirs;
+    s->dirs.resize(n_dirs);
+#pragma omp parallel for schedule(dynamic)
+    for (int32_t l = 0; l < n_dirs; ++l)
+      build_direction(n, n_faces, dim, i_mat, j_mat, k_mat, cut_candidate, dirs + (int64_t)l * dim, s->dirs[l]);
+    *handle = s;
+    return OBTE_OK;
+  } catch (const std::bad_alloc&) {
+    return obte_fail(OBTE_ENOMEM, "obte_sched_create: out of host memory");
+  } catch (...) {
+    return obte_fail(OBTE_EINTERNAL, "obte_sched_create: internal error");
+  }
+}
+
+// sizes[l*3 + 0] = number of levels, [1] = inflow entries, [2] = cut faces
+int obte_sched_sizes(void* handle, int64_t* sizes) {
+  if (!handle || !sizes) return obte_fail(OBTE_EINVAL, "obte_sched_sizes: null argument");
+  auto* s = static_cast<Schedule*>(handle);
+  for (int32_t l = 0; l < s->n_dirs; ++l) {
+    const DirSchedule& d = s->dirs[l];
+    sizes[3 * l + 0] = (int64_t)d.lvl_ptr.size() - 1;
+    sizes[3 * l + 1] = (int64_t)d.in_src.size();
+    sizes[3 * l + 2] = (int64_t)d.cut_face.size();
+  }
+  return OBTE_OK;
+}
+
+// Concatenate all directions.  Array lengths: order/pos n*L; lvl_ptr
+// sum(nlev_l+1); in_ptr L*(n+1) (local entry indices); in_src/in_face
+// sum(entries_l); cut_exit/cut_face sum(cuts_l).
+int obte_sched_export(void* handle, int32_t* order, int32_t* pos, int32_t* lvl_ptr, int32_t* in_ptr,
+                      int32_t* in_src, int32_t* in_face, int32_t* cut_exit, int32_t* cut_face) {
+  if (!handle) return obte_fail(OBTE_EINVAL, "obte_sched_export: null handle");
+  auto* s = static_cast<Schedule*>(handle);
+  const int64_t n = s->n;
+  int64_t lo = 0, eo = 0, co = 0;
+  for (int32_t l = 0; l < s->n_dirs; ++l) {
+    const DirSchedule& d = s->dirs[l];
+    std::memcpy(order + l * n, d.order.data(), n * sizeof(int32_t));
+    std::memcpy(pos + l * n, d.pos.data(), n * sizeof(int32_t));
+    std::memcpy(lvl_ptr + lo, d.lvl_ptr.data(), d.lvl_ptr.size() * sizeof(int32_t));
+    lo += (int64_t)d.lvl_ptr.size();
+    std::memcpy(in_ptr + l * (n + 1), d.in_ptr.data(), (n + 1) * sizeof(int32_t));
+    std::memcpy(in_src + eo, d.in_src.data(), d.in_src.size() * sizeof(int32_t));
+    std::memcpy(in_face + eo, d.in_face.data(), d.in_face.size() * sizeof(int32_t));
+    eo += (int64_t)d.in_src.size();
+    if (!d.cut_face.empty()) {
+      std::memcpy(cut_exit + co, d.cut_exit.data(), d.cut_exit.size() * sizeof(int32_t));
+      std::memcpy(cut_face + co, d.cut_face.data(), d.cut_face.size() * sizeof(int32_t));
+    }
+    co += (int64_t)d.cut_face.size();
+  }
+  return OBTE_OK;
+}
+
+int obte_sched_destroy(void* handle) {
+  delete static_cast<Schedule*>(handle);
+  return OBTE_OK;
+}
+
+}  // extern "C"

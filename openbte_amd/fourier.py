@@ -1,0 +1,256 @@
+"""``Fourier`` on MI355X: the diffusive initial guess (drop-in for
+openbte/fourier.py:15-185).
+
+The two-point-flux Laplacian with its orthogonal / non-orthogonal split
+(openbte/objects.py:312-319) is assembled on the host into a symmetric CSR
+operator (structure plus weights, once); every solve, gradient and
+correction of the correction loop (openbte/fourier.py:149-178) runs as HIP
+kernels (obte_fourier_*), with T, C and the gradient resident on the device.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import NamedTuple
+
+import numpy as np
+import torch
+
+from . import _lib
+from .objects import SolverResults
+
+PINNED_CELL = 10  # openbte/fourier.py:43
+
+
+def decomposed_directions(geo, kappa):
+    """v_orth, v_nonorth of openbte/objects.py:312-319."""
+    na = np.asarray(geo.normal_areas)
+    v_orth = np.einsum('ki,ij,kj->k', na, kappa, na) / np.einsum('ki,ki->k', na, geo.dists)
+    v_non = na @ kappa.T - geo.dists * v_orth[:, None]
+    return v_orth, v_non
+
+
+class FourierHost(NamedTuple):
+    n: int
+    dim: int
+    nv: int
+    pin: int
+    row_ptr: np.ndarray
+    col: np.ndarray
+    w: np.ndarray
+    diag: np.ndarray
+    vol: np.ndarray
+    B: np.ndarray
+    slot_cell: np.ndarray
+    slot_mode: np.ndarray
+    slot_val: np.ndarray
+    gweights: np.ndarray
+    side_e: np.ndarray
+    side_w: np.ndarray
+    side_vn: np.ndarray
+    slot_side: np.ndarray
+    slot_sgn: np.ndarray
+    contact_kind: int
+    ce1: np.ndarray
+    ce2: np.ndarray
+    c_coef: np.ndarray
+    c_value: float
+    normalization: float
+
+
+def build_fourier_host(geo, kappa, bcs, effective_thermal_conductivity=None, heat_source=(),
+                       modified_fourier=False) -> FourierHost:
+    """Symmetric operator A = V F and right-hand side B of openbte/fourier.py:46-131."""
+    n, dim = int(geo.n_elems), int(geo.dim)
+    esm = np.asarray(geo.elem_side_map)
+    nv = esm.shape[1]
+    sem = np.asarray(geo.side_elem_map)
+    vol = np.asarray(geo.elem_volumes, np.float64)
+    v_orth, v_non = decomposed_directions(geo, kappa)
+    inn = np.asarray(geo.internal, np.int64)
+    a, b = sem[inn, 0], sem[inn, 1]
+    vo = v_orth[inn]
+    diag = np.zeros(n)
+    np.add.at(diag, a, vo)
+    np.add.at(diag, b, vo)
+    rows = np.concatenate([a, b])
+    cols = np.concatenate([b, a])
+    ws = np.concatenate([vo, vo])
+    o = np.lexsort((cols, rows))
+    rows, cols, ws = rows[o], cols[o], ws[o]
+    row_ptr = np.zeros(n + 1, np.int64)
+    np.add.at(row_ptr, rows + 1, 1)
+    row_ptr = np.cumsum(row_ptr)
+    B = np.zeros(n)
+    for key, contact in bcs.mixed.items():
+        sides = np.asarray(geo.side_physical_regions[key], np.int64)
+        ii = sem[sides, 0]
+        h = contact['boundary_conductance']
+        tmp = v_orth[sides] * h / (h + v_orth[sides] / geo.side_areas[sides])
+        np.add.at(diag, ii, tmp)
+        B[ii] = contact['value'] * tmp / vol[ii]  # assignment (openbte/fourier.py:86)
+    for region, jump in bcs.periodic.items():
+        sides = np.asarray(geo.periodic_sides[region], np.int64)
+        ii, jj = sem[sides, 0], sem[sides, 1]
+        np.add.at(B, ii, jump * v_orth[sides] / vol[ii])
+        np.add.at(B, jj, -jump * v_orth[sides] / vol[jj])
+    if len(heat_source) != 0:
+        B += (np.asarray(heat_source) * 1e-18) if not modified_fourier else np.asarray(heat_source)
+    pin = -1
+    if modified_fourier:
+        diag = diag + vol
+    elif len(bcs.mixed) == 0:
+        pin = PINNED_CELL
+        B[pin] = 0.0
+
+    # gradient slots (openbte/objects.py:322-405)
+    is_int = np.zeros(len(geo.sides), bool)
+    is_int[inn] = True
+    int_index = -np.ones(len(geo.sides), np.int64)
+    int_index[inn] = np.arange(len(inn))
+    side_jump = np.zeros(len(geo.sides))
+    for region, sides in geo.periodic_sides.items():
+        side_jump[np.asarray(sides, np.int64)] = float(bcs.periodic[region])
+    mixed_val = np.full(len(geo.sides), np.nan)
+    for key, value in bcs.mixed.items():
+        mixed_val[np.asarray(geo.side_physical_regions[key], np.int64)] = float(value['value'])
+    cells = np.repeat(np.arange(n), nv)
+    sd = esm.reshape(-1)
+    first = sem[sd, 0] == cells
+    other = np.where(first, sem[sd, 1], sem[sd, 0])
+    mode = np.zeros(n * nv, np.int32)
+    val = np.zeros(n * nv)
+    nb = -np.ones(n * nv, np.int64)
+    intr = is_int[sd]
+    mode[intr] = 1
+    nb[intr] = other[intr]
+    val[intr] = np.where(first[intr], side_jump[sd[intr]], -side_jump[sd[intr]])
+    mix = ~intr & ~np.isnan(mixed_val[sd])
+    mode[mix] = 2
+    val[mix] = mixed_val[sd[mix]]
+    slot_side = np.where(intr, int_index[sd], -1)
+    slot_sgn = np.where(intr, np.where(first, 1.0, -1.0) / vol[cells], 0.0)
+
+    contact_kind, ce1, ce2, c_coef, c_value, normalization = 0, np.zeros(0, np.int64), np.zeros(0, np.int64), \
+        np.zeros(0), 0.0, 0.0
+    eff = effective_thermal_conductivity
+    if eff is not None:
+        normalization = float(eff.normalization)
+        if bcs.get_bc_by_name(eff.contact) == 'Mixed':
+            sides = np.asarray(geo.side_physical_regions[eff.contact], np.int64)
+            h = bcs.mixed[eff.contact]['boundary_conductance']
+            contact_kind, ce1 = 2, sem[sides, 0]
+            c_coef = v_orth[sides] * h / (h + v_orth[sides] / geo.side_areas[sides])
+            c_value = float(bcs.mixed[eff.contact]['value'])
+        else:
+            sides = np.asarray(geo.periodic_sides[eff.contact], np.int64)
+            contact_kind, ce1, ce2 = 1, sem[sides, 0], sem[sides, 1]
+            c_coef = v_orth[sides]
+            c_value = float(bcs.periodic[eff.contact])
+
+    return FourierHost(
+        n=n, dim=dim, nv=nv, pin=pin, row_ptr=row_ptr.astype(np.int32), col=cols.astype(np.int32), w=ws,
+        diag=diag, vol=vol, B=B, slot_cell=nb.astype(np.int32), slot_mode=mode, slot_val=val,
+        gweights=np.ascontiguousarray(np.asarray(geo.gradient_weights, np.float64)),
+        side_e=np.ascontiguousarray(np.stack([a, b], 1).astype(np.int32)),
+        side_w=np.asarray(geo.interp_weights, np.float64)[inn], side_vn=np.ascontiguousarray(v_non[inn]),
+        slot_side=slot_side.astype(np.int32), slot_sgn=slot_sgn, contact_kind=contact_kind,
+        ce1=np.asarray(ce1, np.int32), ce2=np.asarray(ce2, np.int32), c_coef=np.asarray(c_coef, np.float64),
+        c_value=c_value, normalization=normalization)
+
+
+class FourierDevice:
+    """Device-resident Fourier operator and correction loop state."""
+
+    def __init__(self, host: FourierHost, device=None):
+        if not torch.cuda.is_available():
+            raise RuntimeError('openbte_amd.Fourier needs a ROCm GPU; there is no CPU fallback')
+        self.h = host
+        self.dev = dev = device or torch.device('cuda', torch.cuda.current_device())
+        f64, i32 = torch.float64, torch.int32
+
+        def T(a, dt):
+            a = np.ascontiguousarray(a)
+            if a.size == 0:
+                a = np.zeros(1, dtype=a.dtype)
+            return torch.as_tensor(a, dtype=dt).to(dev)
+        h = host
+        self.b = b = {}
+        for name, dt in (('row_ptr', i32), ('col', i32), ('w', f64), ('diag', f64), ('vol', f64), ('B', f64),
+                         ('slot_cell', i32), ('slot_mode', i32), ('slot_val', f64), ('gweights', f64),
+                         ('side_e', i32), ('side_w', f64), ('side_vn', f64), ('slot_side', i32), ('slot_sgn', f64),
+                         ('ce1', i32), ('ce2', i32), ('c_coef', f64)):
+            b[name] = T(getattr(h, name), dt)
+        n = h.n
+        b['T'] = torch.zeros(n, dtype=f64, device=dev)
+        b['C'] = torch.zeros(n, dtype=f64, device=dev)
+        b['grad'] = torch.zeros(n * h.dim, dtype=f64, device=dev)
+        b['ws'] = torch.zeros(6 * n + 64, dtype=f64, device=dev)
+        b['scal'] = torch.zeros(16, dtype=f64, device=dev)
+        b['side_tmp'] = torch.zeros(max(1, len(h.side_w)), dtype=f64, device=dev)
+        b['red'] = torch.zeros(n // 256 + 2, dtype=f64, device=dev)
+        b['out'] = torch.zeros(2, dtype=f64, device=dev)
+        P = self.plan = _lib.FourierPlan()
+        P.n, P.dim, P.nv, P.n_int, P.pin = n, h.dim, h.nv, len(h.side_w), h.pin
+        for name in ('row_ptr', 'col', 'w', 'diag', 'vol', 'slot_cell', 'slot_mode', 'slot_val', 'gweights',
+                     'side_e', 'side_w', 'side_vn', 'slot_side', 'slot_sgn', 'ce1', 'ce2', 'c_coef'):
+            setattr(P, name, b[name].data_ptr())
+        P.n_contact, P.contact_kind = len(h.ce1), h.contact_kind
+        P.c_value, P.normalization = h.c_value, h.normalization
+        self.stream = torch.cuda.current_stream(dev).cuda_stream
+        self.cg_iters = []
+
+    def solve(self, rtol=1e-13, maxit=20000):
+        b = self.b
+        it = C.c_int32(0)
+        rr = C.c_double(0.0)
+        _lib.call('obte_fourier_solve', self.plan, b['B'].data_ptr(), b['C'].data_ptr(), b['T'].data_ptr(),
+                  b['ws'].data_ptr(), b['scal'].data_ptr(), rtol, maxit, 16, C.byref(it), C.byref(rr), self.stream)
+        self.cg_iters.append((it.value, rr.value))
+
+    def gradient(self):
+        _lib.call('obte_fourier_gradient', self.plan, self.b['T'].data_ptr(), self.b['grad'].data_ptr(), self.stream)
+
+    def correction(self) -> float:
+        b = self.b
+        _lib.call('obte_fourier_correction', self.plan, b['grad'].data_ptr(), b['C'].data_ptr(),
+                  b['side_tmp'].data_ptr(), b['red'].data_ptr(), b['out'].data_ptr(), self.stream)
+        return float(np.sqrt(b['out'][0].item()))
+
+    def kappa(self) -> float:
+        _lib.call('obte_fourier_kappa', self.plan, self.b['T'].data_ptr(), self.b['out'][1:].data_ptr(),
+                  self.stream)
+        return float(self.b['out'][1].item())
+
+
+def Fourier(geometry, thermal_conductivity, boundary_conditions, effective_thermal_conductivity=None,
+            heat_source=[], global_temperature=0, modified_fourier=False, fourier_max_iter=40,
+            fourier_rtol=1e-8, verbose=True, cg_rtol=1e-13) -> SolverResults:
+    """Drop-in for ``openbte.fourier.Fourier`` (openbte/fourier.py:15-24)."""
+    kappa = np.asarray(thermal_conductivity, np.float64)[:geometry.dim, :geometry.dim]
+    host = build_fourier_host(geometry, kappa, boundary_conditions, effective_thermal_conductivity,
+                              heat_source, modified_fourier)
+    if modified_fourier:
+        host = host._replace(B=host.B + global_temperature)
+    dev = FourierDevice(host)
+    norm_C_old = 0.0
+    error, n_iter, kappa_eff = 1.0, 0, 0.0
+    while error > fourier_rtol and n_iter < fourier_max_iter:
+        n_iter += 1
+        dev.solve(rtol=cg_rtol)
+        dev.gradient()
+        norm_C = dev.correction()
+        error = 0.0 if norm_C == 0 else abs(norm_C - norm_C_old) / norm_C
+        norm_C_old = norm_C
+        if effective_thermal_conductivity is not None:
+            kappa_eff = dev.kappa()
+        if verbose:
+            print("Kappa (Fourier): {:E}, Iterations {:n}, Error {:E}".format(kappa_eff, n_iter, error))
+    # the reference's ``while ... else`` resets kappa_eff (openbte/fourier.py:149,179-180)
+    kappa_diffusive = kappa_eff
+    T = dev.b['T'].cpu().numpy().copy()
+    grad = dev.b['grad'].view(host.n, host.dim).cpu().numpy()
+    J = -grad @ kappa.T
+    variables = {'Temperature_Fourier': {'data': T, 'units': 'K'},
+                 'Flux_Fourier': {'data': J * 1e9, 'units': 'W/m/m'}}
+    return SolverResults(kappa_eff=0.0, variables=variables,
+                         aux={'n_iter': n_iter, 'kappa_diffusive': kappa_diffusive, 'cg': dev.cg_iters})

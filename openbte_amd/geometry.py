@@ -1,0 +1,183 @@
+"""Seeded synthetic geometries in place of the reference's shapely + gmsh chain.
+
+The reference turns polygons into a gmsh ``.geo`` file and meshes it in a
+subprocess (openbte/geometry.py:442-554, openbte/mesh.py:354-478); gmsh is not
+available here, so these generators produce the same ``Mesh`` contract
+directly:
+
+* ``pore_cell_2d``  -- periodic square cell with circular / square pores,
+  unstructured triangles from a Delaunay triangulation of jittered lattice
+  points.  Opposite edges carry identical node sets, as gmsh's ``Periodic Line``
+  does, so every ``Periodic_x_a`` side has a partner in ``Periodic_x_b``.
+* ``quad_cell_2d``  -- pore-free uniform quadrilateral periodic cell (the
+  reference reads quads too, openbte/mesh.py:91,168-172); used for closed-form
+  checks.
+* ``structured_tri_cell_2d`` -- nx x ny pixels, 4 triangles per pixel (the
+  differentiation workload's pixel grid).
+
+Region names follow the reference's periodic convention: ``Periodic_x_a``
+(x = -L/2), ``Periodic_x_b`` (x = +L/2, the kept side), likewise for y, and
+``Boundary`` for pore walls.
+"""
+from __future__ import annotations
+
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+from scipy.spatial import Delaunay
+
+from .mesh import build_mesh
+from .objects import Mesh
+
+_REGIONS = ['Periodic_x_a', 'Periodic_x_b', 'Periodic_y_a', 'Periodic_y_b', 'Boundary']
+
+
+def _edge_region(Lx: float, Ly: float):
+    def classify(side_nodes_xy, centroids):
+        tol = 1e-9 * max(Lx, Ly)
+        c = centroids
+        tag = np.full(len(c), 4, np.int64)
+        tag[np.abs(c[:, 0] + Lx / 2) < tol] = 0
+        tag[np.abs(c[:, 0] - Lx / 2) < tol] = 1
+        tag[np.abs(c[:, 1] + Ly / 2) < tol] = 2
+        tag[np.abs(c[:, 1] - Ly / 2) < tol] = 3
+        return tag
+    return classify
+
+
+def _pore_nodes(kind: str, center, size: float, h: float) -> np.ndarray:
+    cx, cy = center
+    if kind == 'circle':
+        r = size
+        n = max(8, int(round(2 * np.pi * r / h)))
+        th = 2 * np.pi * np.arange(n) / n
+        return np.stack([cx + r * np.cos(th), cy + r * np.sin(th)], axis=1)
+    if kind == 'square':
+        a = size  # half side
+        n = max(2, int(round(2 * a / h)))
+        t = np.linspace(-a, a, n + 1)[:-1]
+        bottom = np.stack([cx + t, np.full(n, cy - a)], axis=1)
+        right = np.stack([np.full(n, cx + a), cy + t], axis=1)
+        top = np.stack([cx - t, np.full(n, cy + a)], axis=1)
+        left = np.stack([np.full(n, cx - a), cy - t], axis=1)
+        return np.concatenate([bottom, right, top, left])
+    raise ValueError(kind)
+
+
+def _inside(kind: str, center, size: float, pts: np.ndarray, pad: float = 0.0) -> np.ndarray:
+    d = pts - np.asarray(center)[None, :]
+    if kind == 'circle':
+        return np.hypot(d[:, 0], d[:, 1]) < size + pad
+    return (np.abs(d[:, 0]) < size + pad) & (np.abs(d[:, 1]) < size + pad)
+
+
+def pore_area(kind: str, size: float) -> float:
+    return np.pi * size ** 2 if kind == 'circle' else (2 * size) ** 2
+
+
+def pore_cell_2d(L: float = 10.0,
+                 porosity: float = 0.25,
+                 n_cells: int = 2000,
+                 seed: int = 0,
+                 pores: Optional[Sequence[Tuple[str, Tuple[float, float], float]]] = None,
+                 jitter: float = 0.3) -> Mesh:
+    """Periodic square cell of side ``L`` (nm) with pores, ~``n_cells`` triangles.
+
+    ``pores`` is a list of ``(kind, (cx, cy), size)`` with kind ``'circle'``
+    (size = radius) or ``'square'`` (size = half side); by default one centred
+    circle of the given porosity (r = L sqrt(porosity/pi)).  Lattice points are
+    jittered by U(-jitter h, +jitter h) with ``numpy.random.default_rng(seed)``.
+    """
+    rng = np.random.default_rng(seed)
+    if pores is None:
+        pores = [] if porosity <= 0 else [('circle', (0.0, 0.0), L * np.sqrt(porosity / np.pi))]
+    solid = L * L - sum(pore_area(k, s) for k, _, s in pores)
+    h_target = np.sqrt(2.0 * solid / n_cells)
+    N = max(2, int(round(L / h_target)))
+    h = L / N
+    xs = -L / 2 + h * np.arange(N + 1)
+    # boundary nodes: identical node sets on opposite edges
+    edge = [np.stack([xs, np.full(N + 1, -L / 2)], 1), np.stack([xs, np.full(N + 1, L / 2)], 1),
+            np.stack([np.full(N - 1, -L / 2), xs[1:-1]], 1), np.stack([np.full(N - 1, L / 2), xs[1:-1]], 1)]
+    bnodes = np.concatenate(edge)
+    g = xs[1:-1]
+    X, Y = np.meshgrid(g, g, indexing='ij')
+    inner = np.stack([X.ravel(), Y.ravel()], 1) + rng.uniform(-jitter * h, jitter * h, (len(g) ** 2, 2))
+    keep = np.ones(len(inner), bool)
+    pnodes = []
+    for kind, c, s in pores:
+        keep &= ~_inside(kind, c, s, inner, pad=0.6 * h)
+        pnodes.append(_pore_nodes(kind, c, s, h))
+        lo = np.array(c) - s
+        hi = np.array(c) + s
+        if np.any(lo < -L / 2 + 1.5 * h) or np.any(hi > L / 2 - 1.5 * h):
+            raise ValueError("pore too close to the cell edge")
+    pts = np.concatenate([bnodes, inner[keep]] + pnodes)
+    tri = Delaunay(pts)
+    if len(tri.coplanar):
+        raise RuntimeError("Delaunay dropped points")
+    simp = tri.simplices.astype(np.int64)
+    cen = pts[simp].mean(axis=1)
+    ok = np.ones(len(simp), bool)
+    for kind, c, s in pores:
+        ok &= ~_inside(kind, c, s, cen)
+    simp = simp[ok]
+    # counter-clockwise orientation and no degenerate triangles
+    a = pts[simp]
+    det = (a[:, 1, 0] - a[:, 0, 0]) * (a[:, 2, 1] - a[:, 0, 1]) - (a[:, 1, 1] - a[:, 0, 1]) * (a[:, 2, 0] - a[:, 0, 0])
+    if np.any(np.abs(det) < 1e-10 * h * h):
+        raise RuntimeError("degenerate triangle")
+    flip = det < 0
+    simp[flip] = simp[flip][:, [0, 2, 1]]
+    used = np.unique(simp)
+    remap = -np.ones(len(pts), np.int64)
+    remap[used] = np.arange(len(used))
+    pts = pts[used]
+    simp = remap[simp]
+    mesh = build_mesh(pts, simp, _edge_region(L, L), _REGIONS,
+                      periodic={'Periodic_x': np.array([L, 0.0]), 'Periodic_y': np.array([0.0, L])})
+    # every wall side must lie on a pore perimeter
+    if 'Boundary' in mesh.side_physical_regions:
+        wc = mesh.side_centroids[mesh.side_physical_regions['Boundary']]
+        near = np.zeros(len(wc), bool)
+        for kind, c, s in pores:
+            near |= _inside(kind, c, s, wc, pad=0.1 * h) & ~_inside(kind, c, s, wc, pad=-0.2 * h)
+        if not np.all(near):
+            raise RuntimeError("wall side away from a pore perimeter")
+    return mesh
+
+
+def quad_cell_2d(L: float = 10.0, N: int = 20, Ly: Optional[float] = None, Ny: Optional[int] = None) -> Mesh:
+    """Uniform N x Ny quadrilateral periodic cell, no pores (closed-form checks)."""
+    Ly = L if Ly is None else Ly
+    Ny = N if Ny is None else Ny
+    xs = np.linspace(-L / 2, L / 2, N + 1)
+    ys = np.linspace(-Ly / 2, Ly / 2, Ny + 1)
+    X, Y = np.meshgrid(xs, ys, indexing='ij')
+    pts = np.stack([X.ravel(), Y.ravel()], 1)
+    idx = np.arange((N + 1) * (Ny + 1)).reshape(N + 1, Ny + 1)
+    q = np.stack([idx[:-1, :-1], idx[1:, :-1], idx[1:, 1:], idx[:-1, 1:]], axis=-1).reshape(-1, 4)
+    return build_mesh(pts, q, _edge_region(L, Ly), _REGIONS,
+                      periodic={'Periodic_x': np.array([L, 0.0]), 'Periodic_y': np.array([0.0, Ly])})
+
+
+def structured_tri_cell_2d(L: float = 10.0, N: int = 40) -> Mesh:
+    """N x N pixels of side L/N, each split into 4 triangles by its centre.
+
+    Cell ``4*p + k`` belongs to pixel ``p = i*N + j`` (x index i, y index j).
+    """
+    h = L / N
+    xs = -L / 2 + h * np.arange(N + 1)
+    X, Y = np.meshgrid(xs, xs, indexing='ij')
+    corners = np.stack([X.ravel(), Y.ravel()], 1)
+    ci = np.arange(N)
+    CX, CY = np.meshgrid(-L / 2 + h * (ci + 0.5), -L / 2 + h * (ci + 0.5), indexing='ij')
+    centres = np.stack([CX.ravel(), CY.ravel()], 1)
+    pts = np.concatenate([corners, centres])
+    idx = np.arange((N + 1) ** 2).reshape(N + 1, N + 1)
+    c = (N + 1) ** 2 + np.arange(N * N).reshape(N, N)
+    a, b, cc, d = idx[:-1, :-1], idx[1:, :-1], idx[1:, 1:], idx[:-1, 1:]
+    tris = np.stack([np.stack([a, b, c], -1), np.stack([b, cc, c], -1),
+                     np.stack([cc, d, c], -1), np.stack([d, a, c], -1)], axis=2).reshape(-1, 3)
+    return build_mesh(pts, tris, _edge_region(L, L), _REGIONS,
+                      periodic={'Periodic_x': np.array([L, 0.0]), 'Periodic_y': np.array([0.0, L])})

@@ -1,0 +1,176 @@
+"""CPU restatement of the reference's Fourier initial guess (TEST INFRASTRUCTURE).
+
+FP64 NumPy + SciPy SuperLU, written from openbte/fourier.py:15-185 and the mesh
+operators it calls (openbte/objects.py:139-172,312-405).  Only tests, the smoke
+check and the benchmark's CPU-baseline leg import it.
+
+Pinning (tests/test_oracle_fourier.py): exact linear temperature on pore-free
+periodic meshes; kappa_F = kappa_xx on orthogonal (uniform quad) meshes;
+the Laplacian solve equals scipy.sparse.linalg.spsolve of the same graph
+Laplacian up to the (max+min)/2 shift; LSQ gradient exact for linear fields.
+"""
+from __future__ import annotations
+
+import numpy as np
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+PINNED_CELL = 10  # openbte/fourier.py:43
+
+
+def decomposed_directions(geo, kappa):
+    """v_orth = (nA)^T k (nA) / ((nA).d);  v_nonorth = k nA - d v_orth
+    (openbte/objects.py:312-319)."""
+    na = geo.normal_areas
+    v_orth = np.einsum('ki,ij,kj->k', na, kappa, na) / np.einsum('ki,ki->k', na, geo.dists)
+    v_non = na @ kappa.T - geo.dists * v_orth[:, None]
+    return v_orth, v_non
+
+
+def _side_slot(geo):
+    """slot[s, 0/1] = position of side s in the side list of side_elem_map[s,0/1]."""
+    n, nv = geo.elem_side_map.shape
+    ns = len(geo.sides)
+    slot = np.full((ns, 2), -1, np.int64)
+    esm = np.asarray(geo.elem_side_map)
+    rows = np.repeat(np.arange(n), nv)
+    cols = np.tile(np.arange(nv), n)
+    flat = esm.reshape(-1)
+    first = geo.side_elem_map[flat, 0] == rows
+    slot[flat[first], 0] = cols[first]
+    slot[flat[~first], 1] = cols[~first]
+    return slot
+
+
+def gradient(geo, bcs, T, jump: bool = True):
+    """Least-squares cell gradient (openbte/objects.py:322-405): differences to
+    internal neighbours, +jump across periodic sides, T_wall - T on mixed
+    sides, zero on other boundary sides; g_c = W_c . delta_c."""
+    n, nv = geo.elem_side_map.shape
+    slot = _side_slot(geo)
+    sem = geo.side_elem_map
+    dd = np.zeros((n, nv))
+    inn = np.asarray(geo.internal)
+    i, j = sem[inn, 0], sem[inn, 1]
+    delta = T[j] - T[i]
+    dd[i, slot[inn, 0]] = delta
+    dd[j, slot[inn, 1]] = -delta
+    if jump:
+        for region, sides in geo.periodic_sides.items():
+            sides = np.asarray(sides)
+            jp = bcs.periodic[region]
+            np.add.at(dd, (sem[sides, 0], slot[sides, 0]), jp)
+            np.add.at(dd, (sem[sides, 1], slot[sides, 1]), -jp)
+    for key, value in bcs.mixed.items():
+        sides = np.asarray(geo.side_physical_regions[key])
+        i = sem[sides, 0]
+        dd[i, slot[sides, 0]] = value['value'] - T[i]
+    return np.einsum('kjs,ks->kj', geo.gradient_weights, dd)
+
+
+def diffusive_kappa(eff, T, geo, bcs, kappa):
+    """Integrated orthogonal flux through the contact (openbte/objects.py:139-172)."""
+    bc = bcs.get_bc_by_name(eff.contact)
+    v_orth, _ = decomposed_directions(geo, kappa[:geo.dim, :geo.dim])
+    if bc == 'Mixed':
+        sides = np.asarray(geo.side_physical_regions[eff.contact])
+        T2 = bcs.mixed[eff.contact]['value']
+        h = bcs.mixed[eff.contact]['boundary_conductance']
+        tmp = v_orth[sides] * h / (h + v_orth[sides] / geo.side_areas[sides])
+    else:
+        sides = np.asarray(geo.periodic_sides[eff.contact])
+        T2 = T[geo.side_elem_map[sides, 1]] + bcs.periodic[eff.contact]
+        tmp = v_orth[sides]
+    T1 = T[geo.side_elem_map[sides, 0]]
+    return float(np.sum((T1 - T2) * tmp) * eff.normalization)
+
+
+def assemble(geo, kappa, bcs, heat_source=(), modified_fourier=False):
+    """Two-point-flux Laplacian F (rows / V), RHS B (openbte/fourier.py:46-131)."""
+    n = geo.n_elems
+    v_orth, v_non = decomposed_directions(geo, kappa)
+    sem = geo.side_elem_map
+    inn = np.asarray(geo.internal)
+    i, j = sem[inn, 0], sem[inn, 1]
+    vo = v_orth[inn]
+    vi, vj = geo.elem_volumes[i], geo.elem_volumes[j]
+    rows = [i, i, j, j]
+    cols = [i, j, j, i]
+    vals = [vo / vi, -vo / vi, vo / vj, -vo / vj]
+    B = np.zeros(n)
+    for key, contact in bcs.mixed.items():
+        sides = np.asarray(geo.side_physical_regions[key])
+        ii = sem[sides, 0]
+        h = contact['boundary_conductance']
+        tmp = v_orth[sides] * h / (h + v_orth[sides] / geo.side_areas[sides])
+        rows.append(ii)
+        cols.append(ii)
+        vals.append(tmp / geo.elem_volumes[ii])
+        # assignment, not accumulation (openbte/fourier.py:86)
+        B[ii] = contact['value'] * tmp / geo.elem_volumes[ii]
+    for region, jump in bcs.periodic.items():
+        sides = np.asarray(geo.periodic_sides[region])
+        ii, jj = sem[sides, 0], sem[sides, 1]
+        np.add.at(B, ii, jump * v_orth[sides] / geo.elem_volumes[ii])
+        np.add.at(B, jj, -jump * v_orth[sides] / geo.elem_volumes[jj])
+    if len(heat_source) != 0:
+        B += (np.asarray(heat_source) * 1e-18) if not modified_fourier else np.asarray(heat_source)
+    F = sp.csc_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(n, n))
+    if modified_fourier:
+        F = F + sp.eye(n, format='csc')
+    elif len(bcs.mixed) == 0:
+        # zero row PINNED_CELL and set its diagonal to one (openbte/fourier.py:115-118)
+        F = F.tolil()
+        F[PINNED_CELL, :] = 0
+        F[PINNED_CELL, PINNED_CELL] = 1
+        F = F.tocsc()
+    return F, B, v_orth, v_non
+
+
+def fourier(geo, thermal_conductivity, bcs, effective_thermal_conductivity=None, heat_source=(),
+            global_temperature=0, modified_fourier=False, fourier_max_iter=40, fourier_rtol=1e-8,
+            verbose=False):
+    """Restatement of ``Fourier`` (openbte/fourier.py:15-185)."""
+    from openbte_amd.objects import SolverResults
+    kappa = np.asarray(thermal_conductivity, dtype=np.float64)[:geo.dim, :geo.dim]
+    F, B, v_orth, v_non = assemble(geo, kappa, bcs, heat_source, modified_fourier)
+    lu = spla.splu(F)
+    if modified_fourier:
+        B = B + global_temperature
+    elif len(bcs.mixed) == 0:
+        B[PINNED_CELL] = 0
+    inn = np.asarray(geo.internal)
+    inds = geo.side_elem_map[inn]
+    w = geo.interp_weights[inn]
+    vi = geo.elem_volumes[inds[:, 0]]
+    vj = geo.elem_volumes[inds[:, 1]]
+    vn = v_non[inn]
+    C = np.zeros_like(B)
+    norm_C_old = 0.0
+    error, n_iter, kappa_eff = 1.0, 0, 0.0
+    while error > fourier_rtol and n_iter < fourier_max_iter:
+        n_iter += 1
+        T = lu.solve(B + C)
+        if len(bcs.mixed) == 0 and not modified_fourier:
+            T -= (T.max() + T.min()) * 0.5
+        grad = gradient(geo, bcs, T)
+        C = np.zeros_like(B)
+        F_ave = w[:, None] * grad[inds[:, 0]] + (1 - w)[:, None] * grad[inds[:, 1]]
+        tmp = np.einsum('ki,ki->k', F_ave, vn)
+        np.add.at(C, inds[:, 0], tmp / vi)
+        np.add.at(C, inds[:, 1], -tmp / vj)
+        norm_C = np.linalg.norm(C)
+        error = 0.0 if norm_C == 0 else abs(norm_C - norm_C_old) / norm_C
+        norm_C_old = norm_C
+        if effective_thermal_conductivity is not None:
+            kappa_eff = diffusive_kappa(effective_thermal_conductivity, T, geo, bcs, kappa)
+        if verbose:
+            print("Kappa (Fourier): {:E}, Iterations {:n}, Error {:E}".format(kappa_eff, n_iter, error))
+    # the reference's ``while ... else`` always ends by resetting kappa_eff
+    # (openbte/fourier.py:149,179-180); the last computed value stays in aux
+    kappa_diffusive = kappa_eff
+    kappa_eff = 0.0
+    J = -grad @ kappa.T
+    variables = {'Temperature_Fourier': {'data': T, 'units': 'K'},
+                 'Flux_Fourier': {'data': J * 1e9, 'units': 'W/m/m'}}
+    return SolverResults(kappa_eff=kappa_eff, variables=variables, aux={'n_iter': n_iter, 'kappa_diffusive': kappa_diffusive})
