@@ -58,7 +58,7 @@ typedef struct obte_bte_plan {
   int32_t n_dirs;     /* L */
   int32_t n_mfp;      /* M */
   int32_t dim;        /* 2 or 3 */
-  int32_t mc;         /* MFP bins per chunk: 8, 16, 32 or 64 */
+  int32_t mc;         /* MFP bins per chunk: 4, 8, 16, 32 or 64 */
   int32_t n_chunks;   /* ceil(M / mc) */
   int32_t block;      /* threads per sweep workgroup (multiple of 64, <= 1024) */
   int32_t n_cut;      /* total cut faces over all directions */
@@ -74,10 +74,11 @@ typedef struct obte_bte_plan {
   const int32_t* pos;      /* [L][n] position of cell */
   const int32_t* lvl_off;  /* [L+1] offsets into lvl_ptr */
   const int32_t* lvl_ptr;  /* level start positions */
-  const int32_t* in_ptr;   /* [L][n+1] local entry pointers */
-  const int64_t* in_off;   /* [L+1] entry offsets */
-  const int32_t* in_src;   /* upstream position or ~cut */
-  const int32_t* in_face;  /* directed face of each entry */
+  int32_t kin;             /* inflow slots per position (nv - 1) */
+  int32_t ncut_max;        /* max cut faces of one direction */
+  const int32_t* nb_src;   /* [L][kin][n] upstream row: position (< n), interface row n + cut,
+                              or the position itself with a zero coefficient for unused slots */
+  const int32_t* nb_face;  /* [L][kin][n] directed face of the slot, -1 if unused */
   const int32_t* cut_off;  /* [L+1] cut offsets */
   const int32_t* cut_exit; /* position of the upstream (exit) cell of each cut */
 
@@ -104,17 +105,19 @@ typedef struct obte_bte_plan {
   double nano;             /* 1 for the nanowire perturbation -s_z (3D), else 0 */
 
   /* assembled operator (written by obte_bte_assemble) */
-  double* in_coef;         /* entries: -min(s . k, 0) */
+  double* nb_coef;         /* [L][kin][n] -min(s . k, 0) of each inflow slot */
   double* diag;            /* [L][n] D_l in position order */
   double* pert;            /* [L][n] perturbation in position order */
   double* q;               /* [L][n] pert + diffuse-wall source (per outer iteration) */
+  double* b0;              /* [L][n] DeltaT + heat in position order (per outer iteration) */
 
   /* state */
   const double* delta_t;   /* [n] lattice temperature of the previous iteration */
   const double* tb;        /* [n_wall] wall temperatures of the previous iteration */
-  double* t;               /* [L][n_chunks][n][mc] solution workspace */
-  double* part_t;          /* [L][n_chunks][n] */
-  double* part_j;          /* [L][n_chunks][n][dim] */
+  double* t;               /* [L][n_chunks][n + ncut_max][mc] solution workspace; rows >= n
+                              hold the interface inflows of the running sweep */
+  double* part_t;          /* [L][n_chunks][n] partial lattice temperature, sweep order */
+  double* part_j;          /* [L][n_chunks][n][dim] partial flux, sweep order */
   double* part_tb;         /* [L][n_chunks][n_wall] */
   double* kappa;           /* [M][L] */
   double* supp;            /* [M][L] */
@@ -125,8 +128,8 @@ typedef struct obte_bte_plan {
  * (openbte/bte.py:47-124). */
 int obte_bte_assemble(const obte_bte_plan_t* plan, obte_stream_t stream);
 
-/* Per-iteration source q = pert + Lambda-free diffuse-wall term -TB Gbm2
- * (openbte/bte.py:199-200,211-214). */
+/* Per-iteration sources in position order: q = pert - TB Gbm2 (diffuse
+ * walls) and b0 = DeltaT + heat (openbte/bte.py:199-200,211-214). */
 int obte_bte_source(const obte_bte_plan_t* plan, obte_stream_t stream);
 
 /* Level-scheduled upwind sweeps of all (l, m) systems (openbte/bte.py:217-218).
@@ -134,9 +137,11 @@ int obte_bte_source(const obte_bte_plan_t* plan, obte_stream_t stream);
  * mode 1: zero source, interface inflow y -> exit values x (transfer operator)
  * mode 2: as 0, plus the mode sums of openbte/bte.py:222-243 into the
  *         partial buffers and kappa / supp.
- * y, x: [n_chunks][n_cut][mc].  active: [L][n_chunks] (or NULL = all). */
+ * y, x: [n_chunks][n_cut][mc].  active: [L][n_chunks] (or NULL = all);
+ * lane_active: [L][n_chunks][mc] per-system flags used by mode 1 to skip
+ * converged MFP bins (or NULL). */
 int obte_bte_sweep(const obte_bte_plan_t* plan, int32_t mode, const double* y, double* x,
-                   const int32_t* active, obte_stream_t stream);
+                   const int32_t* active, const int32_t* lane_active, obte_stream_t stream);
 
 /* Interface Krylov layer: GCR with a recycled basis on the cut-face
  * equations y = R S(B, y).  The transfer operator does not depend on the
@@ -164,8 +169,9 @@ int obte_gcr_project(const obte_bte_plan_t* plan, const obte_gcr_t* g, obte_stre
 int obte_gcr_begin_extend(const obte_bte_plan_t* plan, const obte_gcr_t* g, obte_stream_t stream);
 /* c = u - x, orthogonalise against the basis, normalise, update y and r. */
 int obte_gcr_end_extend(const obte_bte_plan_t* plan, const obte_gcr_t* g, obte_stream_t stream);
-/* Clear the basis (k = 0) and y. */
-int obte_gcr_reset(const obte_bte_plan_t* plan, const obte_gcr_t* g, obte_stream_t stream);
+/* Clear the basis (k = 0); y = delta_t0 at each cut's exit cell (0 if
+ * delta_t0 is NULL). */
+int obte_gcr_reset(const obte_bte_plan_t* plan, const obte_gcr_t* g, const double* delta_t0, obte_stream_t stream);
 
 /* Fixed-order reduction of the partial buffers into DeltaT, J, TB
  * (openbte/bte.py:245-249 -> openbte/objects.py:86-101); norm2 = ||DeltaT||^2. */

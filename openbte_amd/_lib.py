@@ -23,13 +23,13 @@ class BtePlan(C.Structure):
         ('n', _i32), ('n_dirs', _i32), ('n_mfp', _i32), ('dim', _i32), ('mc', _i32), ('n_chunks', _i32),
         ('block', _i32), ('n_cut', _i32), ('n_wall', _i32), ('n_contact', _i32), ('n_faces', _i32),
         ('contact_kind', _i32), ('normalization', _f64), ('contact_jump', _f64),
-        ('order', _p), ('pos', _p), ('lvl_off', _p), ('lvl_ptr', _p), ('in_ptr', _p), ('in_off', _p),
-        ('in_src', _p), ('in_face', _p), ('cut_off', _p), ('cut_exit', _p),
+        ('order', _p), ('pos', _p), ('lvl_off', _p), ('lvl_ptr', _p), ('kin', _i32), ('ncut_max', _i32),
+        ('nb_src', _p), ('nb_face', _p), ('cut_off', _p), ('cut_exit', _p),
         ('dirs', _p), ('k_face', _p), ('jump_face', _p), ('cf_ptr', _p), ('cf_face', _p),
         ('wall_ptr', _p), ('wall_side', _p), ('wall_db', _p), ('wall_temp', _p), ('wall_cell', _p),
         ('wall_rinv', _p), ('wall_dbs', _p), ('ce1', _p), ('ce2', _p), ('c_na', _p), ('mfps', _p),
         ('t_coeff', _p), ('sigma', _p), ('heat', _p), ('nano', _f64),
-        ('in_coef', _p), ('diag', _p), ('pert', _p), ('q', _p),
+        ('nb_coef', _p), ('diag', _p), ('pert', _p), ('q', _p), ('b0', _p),
         ('delta_t', _p), ('tb', _p), ('t', _p), ('part_t', _p), ('part_j', _p), ('part_tb', _p),
         ('kappa', _p), ('supp', _p), ('scratch', _p),
     ]
@@ -64,11 +64,11 @@ _SIGNATURES = {
     'obte_sched_destroy': [_p],
     'obte_bte_assemble': [C.POINTER(BtePlan), _p],
     'obte_bte_source': [C.POINTER(BtePlan), _p],
-    'obte_bte_sweep': [C.POINTER(BtePlan), _i32, _p, _p, _p, _p],
+    'obte_bte_sweep': [C.POINTER(BtePlan), _i32, _p, _p, _p, _p, _p],
     'obte_gcr_project': [C.POINTER(BtePlan), C.POINTER(Gcr), _p],
     'obte_gcr_begin_extend': [C.POINTER(BtePlan), C.POINTER(Gcr), _p],
     'obte_gcr_end_extend': [C.POINTER(BtePlan), C.POINTER(Gcr), _p],
-    'obte_gcr_reset': [C.POINTER(BtePlan), C.POINTER(Gcr), _p],
+    'obte_gcr_reset': [C.POINTER(BtePlan), C.POINTER(Gcr), _p, _p],
     'obte_bte_reduce': [C.POINTER(BtePlan), _p, _p, _p, _p, _p],
     'obte_fourier_solve': [C.POINTER(FourierPlan), _p, _p, _p, _p, _p, _f64, _i32, _i32,
                            C.POINTER(_i32), C.POINTER(_f64), _p],

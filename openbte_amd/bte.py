@@ -28,14 +28,14 @@ import torch
 
 from . import _lib
 from .objects import SolverResults
-from .operator import HostOperator, Schedule, build_host_operator, build_schedule
+from .operator import HostOperator, Schedule, build_host_operator, build_schedule, fixed_width_inflow, max_inflow
 
 
 @dataclass
 class SweepConfig:
     """Launch and Krylov knobs (picked per workload by ``openbte_amd.tune``)."""
-    mc: int = 16             # MFP bins per workgroup chunk
-    block: int = 256         # threads per sweep workgroup
+    mc: int = 8              # MFP bins per workgroup chunk (looped by each lane)
+    block: int = 256         # threads per sweep workgroup (one cell per lane per level)
     kmax: int = 48           # recycled GCR basis capacity per system
     rtol: float = 1e-12      # interface residual tolerance
     max_extend: int = 64     # cap on basis extensions per outer iteration
@@ -78,18 +78,27 @@ class BTEDevice:
         nch = (M + mc - 1) // mc
         self.n_chunks = nch
         self.n_cut = int(sc.cut_off[-1])
+        self.ncut_max = int(np.diff(sc.cut_off).max()) if L > 0 else 0
         n_wall = len(op.i_diffuse)
-        n_ent = int(sc.in_off[-1])
+        self.kin = max(2, max_inflow(sc))
+        if self.kin > 3:
+            raise ValueError(f'{self.kin} inflow faces per cell: only up to 3 are supported')
+        nb_src, nb_face = fixed_width_inflow(sc, n, self.kin)
+        # the sweep kernel indexes T rows with these values: check them here
+        ncut_l = np.diff(sc.cut_off)
+        if nb_src.size and (nb_src.min() < 0 or np.any(nb_src.max(axis=(1, 2)) >= n + ncut_l)):
+            raise ValueError('inflow source row out of range')
+        if len(sc.cut_exit) and self.n_cut and (sc.cut_exit[:self.n_cut].min() < 0
+                                                 or sc.cut_exit[:self.n_cut].max() >= n):
+            raise ValueError('cut exit position out of range')
         self.buf = b = {}
         # schedule
         b['order'] = _t(sc.order, i32, dev)
         b['pos'] = _t(sc.pos, i32, dev)
         b['lvl_off'] = _t(sc.lvl_off, i32, dev)
         b['lvl_ptr'] = _t(sc.lvl_ptr, i32, dev)
-        b['in_ptr'] = _t(sc.in_ptr, i32, dev)
-        b['in_off'] = _t(sc.in_off, torch.int64, dev)
-        b['in_src'] = _t(sc.in_src, i32, dev)
-        b['in_face'] = _t(sc.in_face, i32, dev)
+        b['nb_src'] = _t(nb_src, i32, dev)
+        b['nb_face'] = _t(nb_face, i32, dev)
         b['cut_off'] = _t(sc.cut_off, i32, dev)
         b['cut_exit'] = _t(sc.cut_exit, i32, dev)
         # geometry / material
@@ -113,15 +122,17 @@ class BTEDevice:
         b['sigma'] = _t(op.sigma, f64, dev)
         b['heat'] = _t(op.heat, f64, dev)
         # assembled operator
-        b['in_coef'] = torch.zeros(max(1, n_ent), dtype=f64, device=dev)
+        b['nb_coef'] = torch.zeros(L * self.kin * n, dtype=f64, device=dev)
         b['diag'] = torch.zeros(L * n, dtype=f64, device=dev)
         b['pert'] = torch.zeros(L * n, dtype=f64, device=dev)
         b['q'] = torch.zeros(L * n, dtype=f64, device=dev)
+        b['b0'] = torch.zeros(L * n, dtype=f64, device=dev)
         # state
         b['dt'] = [torch.zeros(n, dtype=f64, device=dev), torch.zeros(n, dtype=f64, device=dev)]
         b['tb'] = [torch.zeros(max(1, n_wall), dtype=f64, device=dev) for _ in range(2)]
         b['j'] = torch.zeros(n * dim, dtype=f64, device=dev)
-        b['t'] = torch.empty(L * nch * n * mc, dtype=f64, device=dev)
+        # zero-initialised: unused inflow slots read a finite value times 0
+        b['t'] = torch.zeros(L * nch * (n + self.ncut_max) * mc, dtype=f64, device=dev)
         b['part_t'] = torch.empty(L * nch * n, dtype=f64, device=dev)
         b['part_j'] = torch.empty(L * nch * n * dim, dtype=f64, device=dev)
         b['part_tb'] = torch.empty(max(1, L * nch * n_wall), dtype=f64, device=dev)
@@ -157,10 +168,11 @@ class BTEDevice:
         P.mc, P.n_chunks, P.block = cfg.mc, self.n_chunks, cfg.block
         P.n_cut, P.n_wall, P.n_contact, P.n_faces = self.n_cut, len(op.i_diffuse), len(op.ce1), len(op.i_mat)
         P.contact_kind, P.normalization, P.contact_jump = op.contact_kind, op.normalization, op.contact_jump
-        for name in ('order', 'pos', 'lvl_off', 'lvl_ptr', 'in_ptr', 'in_off', 'in_src', 'in_face', 'cut_off',
+        P.kin, P.ncut_max = self.kin, self.ncut_max
+        for name in ('order', 'pos', 'lvl_off', 'lvl_ptr', 'nb_src', 'nb_face', 'cut_off',
                      'cut_exit', 'dirs', 'k_face', 'jump_face', 'cf_ptr', 'cf_face', 'wall_ptr', 'wall_side',
                      'wall_db', 'wall_temp', 'wall_cell', 'wall_rinv', 'wall_dbs', 'ce1', 'ce2', 'c_na', 'mfps',
-                     't_coeff', 'sigma', 'heat', 'in_coef', 'diag', 'pert', 'q', 't', 'part_t', 'part_j',
+                     't_coeff', 'sigma', 'heat', 'nb_coef', 'diag', 'pert', 'q', 'b0', 't', 'part_t', 'part_j',
                      'part_tb', 'kappa', 'supp', 'scratch'):
             setattr(P, name, b[name].data_ptr())
         P.nano = op.nano
@@ -187,27 +199,39 @@ class BTEDevice:
             self.buf['tb'][0][:len(self.op.i_diffuse)].copy_(d0[torch.as_tensor(self.op.i_diffuse).to(self.dev)])
         self.plan.delta_t = self.buf['dt'][0].data_ptr()
         self.plan.tb = self.buf['tb'][0].data_ptr()
-        _lib.call('obte_gcr_reset', self.plan, self.gcr, self.stream)
+        _lib.call('obte_gcr_reset', self.plan, self.gcr, self.buf['dt'][0].data_ptr(), self.stream)
 
     def _n_active(self) -> int:
         return int(self.gbuf['n_active'].item())
 
-    def outer_iteration(self, max_extend: Optional[int] = None):
-        """One Jacobi outer iteration; returns (norm of new DeltaT, #extensions)."""
+    def _sweep(self, mode, y, active, timer, lane_active=0):
+        if timer is not None:
+            ev0 = torch.cuda.Event(enable_timing=True)
+            ev1 = torch.cuda.Event(enable_timing=True)
+            ev0.record()
+        _lib.call('obte_bte_sweep', self.plan, mode, y, self.gbuf['x'].data_ptr(), active, lane_active, self.stream)
+        if timer is not None:
+            ev1.record()
+            timer.append((mode, ev0, ev1))
+
+    def outer_iteration(self, max_extend: Optional[int] = None, timer: Optional[list] = None):
+        """One Jacobi outer iteration; returns the number of basis extensions.
+        ``timer`` (optional list) receives (mode, start, end) HIP events of every
+        sweep launch, recorded on the launch stream."""
         P, G, st = self.plan, self.gcr, self.stream
         g = self.gbuf
         max_extend = self.cfg.max_extend if max_extend is None else max_extend
         _lib.call('obte_bte_source', P, st)
         ext = 0
         if self.n_cut > 0:
-            _lib.call('obte_bte_sweep', P, 0, g['y'].data_ptr(), g['x'].data_ptr(), 0, st)
+            self._sweep(0, g['y'].data_ptr(), 0, timer)
             _lib.call('obte_gcr_project', P, G, st)
             while ext < max_extend and self._n_active() > 0:
                 _lib.call('obte_gcr_begin_extend', P, G, st)
-                _lib.call('obte_bte_sweep', P, 1, g['w'].data_ptr(), g['x'].data_ptr(), g['active'].data_ptr(), st)
+                self._sweep(1, g['w'].data_ptr(), g['active'].data_ptr(), timer, g['lane_active'].data_ptr())
                 _lib.call('obte_gcr_end_extend', P, G, st)
                 ext += 1
-        _lib.call('obte_bte_sweep', P, 2, g['y'].data_ptr(), g['x'].data_ptr(), 0, st)
+        self._sweep(2, g['y'].data_ptr(), 0, timer)
         nxt = 1 - self.cur
         b = self.buf
         _lib.call('obte_bte_reduce', P, b['dt'][nxt].data_ptr(), b['j'].data_ptr(), b['tb'][nxt].data_ptr(),

@@ -67,16 +67,20 @@ __global__ void k_assemble(obte_bte_plan_t P) {
   if (P.nano != 0.0 && dim == 3) pert -= s[2];
   P.diag[gid] = D;
   P.pert[gid] = pert;
-  const int32_t* inptr = P.in_ptr + (int64_t)l * (P.n + 1);
-  const int64_t eb = P.in_off[l];
-  for (int e = inptr[p]; e < inptr[p + 1]; ++e) {
-    const int f = P.in_face[eb + e];
-    const double g = dotv(s, P.k_face + (int64_t)f * dim, dim);
-    P.in_coef[eb + e] = g < 0.0 ? -g : 0.0;
+  const int64_t base = (int64_t)l * P.kin * P.n + p;
+  for (int kk = 0; kk < P.kin; ++kk) {
+    const int f = P.nb_face[base + (int64_t)kk * P.n];
+    double cf = 0.0;
+    if (f >= 0) {
+      const double g = dotv(s, P.k_face + (int64_t)f * dim, dim);
+      cf = g < 0.0 ? -g : 0.0;
+    }
+    P.nb_coef[base + (int64_t)kk * P.n] = cf;
   }
 }
 
 // q = pert + sum over diffuse walls of (-TB_s) min(s . db, 0)   (openbte/bte.py:199-200,214)
+// b0 = DeltaT + heat, both in position order
 __global__ void k_source(obte_bte_plan_t P) {
   const int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int64_t total = (int64_t)P.n_dirs * P.n;
@@ -93,87 +97,206 @@ __global__ void k_source(obte_bte_plan_t P) {
     if (g < 0.0) q += -P.tb[sd] * g;
   }
   P.q[gid] = q;
+  P.b0[gid] = P.delta_t[c] + P.heat[c];
 }
 
 // ---------------------------------------------------------------------------
 // (b) level-scheduled upwind sweep, (c) fused mode sums
+//
+// One workgroup per (direction, chunk of MC MFP bins); one lane per cell of
+// the current level, looping over the MC bins of its chunk.  A cell's MC
+// values are one contiguous row of T, so every upwind gather is a run of
+// 16-byte vector loads and the mode sums are plain register accumulations.
+//
+// Software pipelining across levels: before the barrier that ends level k,
+// each lane already loads the source terms, diagonal, inflow slots and
+// coefficients of its cell of level k+1, so after the barrier only the
+// upwind gathers remain on the critical path.  Levels wider than the block
+// are finished with extra (non-prefetched) passes.
 // ---------------------------------------------------------------------------
-template <int MC, int MODE>
+template <int KIN>
+struct CellRegs {
+  int p;       // position, -1 if this lane has no cell
+  double b0, q, dd;
+  int src[KIN];
+  double cf[KIN];
+  int cell;
+  double dtc;
+};
+
+template <int MODE, int KIN>
+__device__ __forceinline__ void load_cell(CellRegs<KIN>& R, int p, int p1, int n, const int32_t* __restrict__ nsrc,
+                                          const double* __restrict__ ncoef, const double* __restrict__ b0,
+                                          const double* __restrict__ qv, const double* __restrict__ dg,
+                                          const int32_t* __restrict__ order, const double* __restrict__ dt) {
+  const bool ok = p < p1;
+  R.p = ok ? p : -1;
+  if (!ok) return;
+  R.dd = dg[p];
+  R.b0 = 0.0;
+  R.q = 0.0;
+  if (MODE != 1) {
+    R.b0 = b0[p];
+    R.q = qv[p];
+  }
+#pragma unroll
+  for (int kk = 0; kk < KIN; ++kk) {
+    R.src[kk] = nsrc[(int64_t)kk * n + p];
+    R.cf[kk] = ncoef[(int64_t)kk * n + p];
+  }
+  if (MODE == 2) {
+    R.cell = order[p];
+    R.dtc = dt[R.cell];
+  }
+}
+
+constexpr int kSub = 4;  // MFP bins per register block
+
+template <int MC, int MODE, int KIN>
+__device__ __forceinline__ void solve_cell(const CellRegs<KIN>& R, double* __restrict__ T, const double* s_lam,
+                                           const int* s_act, const double* s_tco, const double* s_sig, int dim,
+                                           double& pt, double& pj0, double& pj1, double& pj2) {
+  const double* __restrict__ rows[KIN];
+#pragma unroll
+  for (int kk = 0; kk < KIN; ++kk) rows[kk] = T + (int64_t)R.src[kk] * MC;
+  double* __restrict__ out = T + (int64_t)R.p * MC;
+  // one register block of kSub bins live at a time
+#pragma unroll 1
+  for (int mb = 0; mb < MC; mb += kSub) {
+    if (MODE == 1 && !s_act[mb / kSub]) continue;  // uniform: whole sub-block converged
+    constexpr int W = MC < kSub ? MC : kSub;
+    double acc[W];
+#pragma unroll
+    for (int j = 0; j < W; ++j) acc[j] = 0.0;
+#pragma unroll
+    for (int kk = 0; kk < KIN; ++kk) {
+      const double2* r2 = reinterpret_cast<const double2*>(rows[kk] + mb);
+#pragma unroll
+      for (int j = 0; j < W / 2; ++j) {
+        const double2 v = r2[j];
+        acc[2 * j] += R.cf[kk] * v.x;
+        acc[2 * j + 1] += R.cf[kk] * v.y;
+      }
+    }
+    double tv[W];
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+      const double lam = s_lam[mb + j];
+      tv[j] = (R.b0 + lam * (R.q + acc[j])) / (1.0 + lam * R.dd);
+    }
+    double2* o2 = reinterpret_cast<double2*>(out + mb);
+#pragma unroll
+    for (int j = 0; j < W / 2; ++j) o2[j] = make_double2(tv[2 * j], tv[2 * j + 1]);
+    if (MODE == 2) {
+#pragma unroll
+      for (int j = 0; j < W; ++j) {
+        const int m = mb + j;
+        pt += s_tco[m] * tv[j];
+        const double d = tv[j] - R.dtc;
+        pj0 += d * s_sig[3 * m + 0];
+        pj1 += d * s_sig[3 * m + 1];
+        if (dim == 3) pj2 += d * s_sig[3 * m + 2];
+      }
+    }
+  }
+}
+
+// Blocks are dealt to the 8 XCDs round-robin; give all chunks of one
+// direction the same XCD so they share its L2 for the direction's operator.
+__device__ __forceinline__ int task_of_block(int b, int ntask) {
+  constexpr int kXcd = 8;
+  const int per = (ntask + kXcd - 1) / kXcd;
+  const int xcd = b % kXcd;
+  const int idx = b / kXcd;
+  const int t = xcd * per + idx;
+  return t < ntask ? t : -1;
+}
+
+template <int MC, int MODE, int KIN>
 __global__ __launch_bounds__(kMaxBlock) void k_sweep(obte_bte_plan_t P, const double* __restrict__ y,
-                                                     double* __restrict__ x, const int32_t* __restrict__ active) {
+                                                     double* __restrict__ x, const int32_t* __restrict__ active,
+                                                     const int32_t* __restrict__ lane_active) {
   __shared__ double red[kMaxBlock];
-  const int task = blockIdx.x;
+  __shared__ double s_lam[MC], s_tco[MC], s_sig[3 * MC];
+  __shared__ int s_act[(MC + kSub - 1) / kSub];
+  const int ntask = P.n_dirs * P.n_chunks;
+  const int task = task_of_block(blockIdx.x, ntask);
+  if (task < 0) return;
   if (active != nullptr && active[task] == 0) return;  // uniform over the block
   const int L = P.n_dirs;
   const int l = task / P.n_chunks;
   const int chunk = task - l * P.n_chunks;
   const int n = P.n;
   const int dim = P.dim;
-  const int lane = threadIdx.x % MC;
-  const int slot = threadIdx.x / MC;
-  const int cpp = blockDim.x / MC;
-  const int mg = chunk * MC + lane;
-  const bool valid = mg < P.n_mfp;
-  const int mm = valid ? mg : P.n_mfp - 1;
-  const double lam = valid ? P.mfps[mm] : 0.0;
+  const int tid = threadIdx.x;
+  const int nthr = blockDim.x;
 
-  double* __restrict__ T = P.t + (int64_t)task * n * MC;
-  const int32_t* __restrict__ order = P.order + (int64_t)l * n;
-  const int32_t* __restrict__ inptr = P.in_ptr + (int64_t)l * (n + 1);
-  const int32_t* __restrict__ src_e = P.in_src + P.in_off[l];
-  const double* __restrict__ coef_e = P.in_coef + P.in_off[l];
+  if (tid < MC) {
+    const int mg = chunk * MC + tid;
+    const bool valid = mg < P.n_mfp;
+    s_lam[tid] = valid ? P.mfps[mg] : 0.0;
+    s_tco[tid] = (MODE == 2 && valid) ? P.t_coeff[(int64_t)mg * L + l] : 0.0;
+    for (int d = 0; d < 3; ++d)
+      s_sig[3 * tid + d] = (MODE == 2 && valid && d < dim) ? P.sigma[((int64_t)mg * L + l) * dim + d] : 0.0;
+  }
+  if (tid < (MC + kSub - 1) / kSub) {
+    int a = 1;
+    if (MODE == 1 && lane_active != nullptr) {
+      a = 0;
+      for (int j = tid * kSub; j < (tid + 1) * kSub && j < MC; ++j) a |= lane_active[(int64_t)task * MC + j];
+    }
+    s_act[tid] = a;
+  }
+
+  double* __restrict__ T = P.t + (int64_t)task * ((int64_t)n + P.ncut_max) * MC;
   const int32_t* __restrict__ lvl = P.lvl_ptr + P.lvl_off[l];
   const int nlev = P.lvl_off[l + 1] - P.lvl_off[l] - 1;
   const int cbase = P.cut_off[l];
   const int ncut = P.cut_off[l + 1] - cbase;
-  const double* __restrict__ yv = y + ((int64_t)chunk * P.n_cut + cbase) * MC;
-  const double* __restrict__ diag = P.diag + (int64_t)l * n;
-  const double* __restrict__ qv = P.q + (int64_t)l * n;
+  const int64_t lo = (int64_t)l * n;
+  const int32_t* __restrict__ nsrc = P.nb_src + lo * KIN;
+  const double* __restrict__ ncoef = P.nb_coef + lo * KIN;
+  const double* __restrict__ b0 = P.b0 + lo;
+  const double* __restrict__ qv = P.q + lo;
+  const double* __restrict__ dg = P.diag + lo;
+  const int32_t* __restrict__ order = P.order + lo;
 
-  double tco = 0.0, sg0 = 0.0, sg1 = 0.0, sg2 = 0.0;
-  if (MODE == 2 && valid) {
-    tco = P.t_coeff[(int64_t)mm * L + l];
-    const double* sg = P.sigma + ((int64_t)mm * L + l) * dim;
-    sg0 = sg[0];
-    sg1 = sg[1];
-    if (dim == 3) sg2 = sg[2];
+  // interface inflows of the cut faces -> rows n .. n + ncut of T
+  if (ncut > 0) {
+    const double* __restrict__ yv = y + ((int64_t)chunk * P.n_cut + cbase) * MC;
+    double* __restrict__ ti = T + (int64_t)n * MC;
+    for (int i = tid; i < ncut * MC; i += nthr) ti[i] = yv[i];
   }
+  __syncthreads();
+
+  // R holds this lane's cell of the current level; Rn is loaded at the start
+  // of a level for the next one, so its latency overlaps the level's work.
+  CellRegs<KIN> R, Rn;
+  if (nlev > 0) load_cell<MODE, KIN>(R, lvl[0] + tid, lvl[1], n, nsrc, ncoef, b0, qv, dg, order, P.delta_t);
 
   for (int k = 0; k < nlev; ++k) {
+    const int p0 = lvl[k];
     const int p1 = lvl[k + 1];
-    for (int p = lvl[k] + slot; p < p1; p += cpp) {
-      const int c = order[p];
-      double b = 0.0;
-      double dtc = 0.0;
-      if (MODE != 1) {
-        dtc = P.delta_t[c];
-        b = dtc + P.heat[c] + lam * qv[p];
-      }
-      double acc = 0.0;
-      const int e1 = inptr[p + 1];
-      for (int e = inptr[p]; e < e1; ++e) {
-        const int sidx = src_e[e];
-        const double v = sidx >= 0 ? T[(int64_t)sidx * MC + lane] : yv[(int64_t)(~sidx) * MC + lane];
-        acc += coef_e[e] * v;
-      }
-      const double t = (b + lam * acc) / (1.0 + lam * diag[p]);
-      T[(int64_t)p * MC + lane] = t;
-      if (MODE == 2) {
-        const double a0 = lane_group_sum<MC>(tco * t);
-        const double dj = t - dtc;
-        const double j0 = lane_group_sum<MC>(dj * sg0);
-        const double j1 = lane_group_sum<MC>(dj * sg1);
-        double j2 = 0.0;
-        if (dim == 3) j2 = lane_group_sum<MC>(dj * sg2);
-        if (lane == 0) {
-          const int64_t o = (int64_t)task * n + c;
-          P.part_t[o] = a0;
-          P.part_j[o * dim + 0] = j0;
-          P.part_j[o * dim + 1] = j1;
-          if (dim == 3) P.part_j[o * dim + 2] = j2;
+    if (k + 1 < nlev) load_cell<MODE, KIN>(Rn, p1 + tid, lvl[k + 2], n, nsrc, ncoef, b0, qv, dg, order, P.delta_t);
+    for (int base = p0; base < p1; base += nthr) {
+      CellRegs<KIN> Rx;
+      if (base != p0) load_cell<MODE, KIN>(Rx, base + tid, p1, n, nsrc, ncoef, b0, qv, dg, order, P.delta_t);
+      const CellRegs<KIN>& C = base == p0 ? R : Rx;
+      if (C.p >= 0) {
+        double pt = 0.0, pj0 = 0.0, pj1 = 0.0, pj2 = 0.0;
+        solve_cell<MC, MODE, KIN>(C, T, s_lam, s_act, s_tco, s_sig, dim, pt, pj0, pj1, pj2);
+        if (MODE == 2) {
+          // partial mode sums in sweep order (coalesced); the reduction maps
+          // positions back to cells
+          const int64_t o = (int64_t)task * n + C.p;
+          P.part_t[o] = pt;
+          P.part_j[o * dim + 0] = pj0;
+          P.part_j[o * dim + 1] = pj1;
+          if (dim == 3) P.part_j[o * dim + 2] = pj2;
         }
       }
     }
+    R = Rn;
     __syncthreads();
   }
 
@@ -181,54 +304,70 @@ __global__ __launch_bounds__(kMaxBlock) void k_sweep(obte_bte_plan_t P, const do
   if (x != nullptr) {
     double* __restrict__ xv = x + ((int64_t)chunk * P.n_cut + cbase) * MC;
     const int32_t* __restrict__ ex = P.cut_exit + cbase;
-    for (int f = slot; f < ncut; f += cpp) xv[(int64_t)f * MC + lane] = T[(int64_t)ex[f] * MC + lane];
+    for (int i = tid; i < ncut * MC; i += nthr) {
+      const int f = i / MC;
+      xv[i] = T[(int64_t)ex[f] * MC + (i - f * MC)];
+    }
   }
 
   if (MODE == 2) {
-    const int32_t* __restrict__ pos = P.pos + (int64_t)l * n;
+    const int32_t* __restrict__ pos = P.pos + lo;
     // wall temperatures: TB_s = -sum_ml T_ml[i_s] GG_ml,s  (openbte/bte.py:73-78,227)
-    for (int sd = slot; sd < P.n_wall; sd += cpp) {
+    for (int sd = tid; sd < P.n_wall; sd += nthr) {
       const double* db = P.wall_dbs + (int64_t)sd * dim;
-      double gb = sg0 * db[0] + sg1 * db[1];
-      if (dim == 3) gb += sg2 * db[2];
-      const double gg = (gb > 0.0 ? gb : 0.0) * P.wall_rinv[sd];
-      const double v = T[(int64_t)pos[P.wall_cell[sd]] * MC + lane];
-      const double tb = lane_group_sum<MC>(-v * gg);
-      if (lane == 0) P.part_tb[(int64_t)task * P.n_wall + sd] = tb;
+      const double rinv = P.wall_rinv[sd];
+      const double* row = T + (int64_t)pos[P.wall_cell[sd]] * MC;
+      double acc = 0.0;
+#pragma unroll 1
+      for (int m = 0; m < MC; ++m) {
+        double gb = s_sig[3 * m] * db[0] + s_sig[3 * m + 1] * db[1];
+        if (dim == 3) gb += s_sig[3 * m + 2] * db[2];
+        acc += -row[m] * ((gb > 0.0 ? gb : 0.0) * rinv);
+      }
+      P.part_tb[(int64_t)task * P.n_wall + sd] = acc;
     }
     // kappa and suppression of each mode (openbte/bte.py:232-243)
     if (P.contact_kind != 0) {
-      double kap = 0.0, sup = 0.0;
-      for (int sc = slot; sc < P.n_contact; sc += cpp) {
-        const double* na = P.c_na + (int64_t)sc * dim;
-        double sn = sg0 * na[0] + sg1 * na[1];
-        double area2 = na[0] * na[0] + na[1] * na[1];
-        if (dim == 3) {
-          sn += sg2 * na[2];
-          area2 += na[2] * na[2];
+#pragma unroll 1
+      for (int m = 0; m < MC; ++m) {
+        double kap = 0.0, sup = 0.0;
+        const double s0 = s_sig[3 * m], s1 = s_sig[3 * m + 1], s2 = s_sig[3 * m + 2];
+        for (int sc = tid; sc < P.n_contact; sc += nthr) {
+          const double* na = P.c_na + (int64_t)sc * dim;
+          double sn = s0 * na[0] + s1 * na[1];
+          double area2 = na[0] * na[0] + na[1] * na[1];
+          if (dim == 3) {
+            sn += s2 * na[2];
+            area2 += na[2] * na[2];
+          }
+          const double t1 = T[(int64_t)pos[P.ce1[sc]] * MC + m];
+          const double snp = sn > 0.0 ? sn : 0.0;
+          const double snm = sn < 0.0 ? sn : 0.0;
+          kap += t1 * snp + P.contact_jump * snm;
+          if (P.contact_kind == 1) kap += T[(int64_t)pos[P.ce2[sc]] * MC + m] * snm;
+          sup += t1 * sqrt(area2);
         }
-        const double t1 = T[(int64_t)pos[P.ce1[sc]] * MC + lane];
-        const double snp = sn > 0.0 ? sn : 0.0;
-        const double snm = sn < 0.0 ? sn : 0.0;
-        kap += t1 * snp + P.contact_jump * snm;
-        if (P.contact_kind == 1) kap += T[(int64_t)pos[P.ce2[sc]] * MC + lane] * snm;
-        sup += t1 * sqrt(area2);
-      }
-      red[threadIdx.x] = kap;
-      __syncthreads();
-      double ksum = 0.0;
-      if (slot == 0) {
-        for (int s2 = 0; s2 < cpp; ++s2) ksum += red[s2 * MC + lane];
-      }
-      __syncthreads();
-      red[threadIdx.x] = sup;
-      __syncthreads();
-      if (slot == 0) {
-        double ssum = 0.0;
-        for (int s2 = 0; s2 < cpp; ++s2) ssum += red[s2 * MC + lane];
-        if (valid) {
-          P.kappa[(int64_t)mm * L + l] = ksum * P.normalization;
-          P.supp[(int64_t)mm * L + l] = ssum * P.normalization;
+        // fixed-order block reductions
+        red[tid] = kap;
+        __syncthreads();
+        for (int o = nthr / 2; o > 0; o >>= 1) {
+          if (tid < o) red[tid] += red[tid + o];
+          __syncthreads();
+        }
+        const double ksum = red[0];
+        __syncthreads();
+        red[tid] = sup;
+        __syncthreads();
+        for (int o = nthr / 2; o > 0; o >>= 1) {
+          if (tid < o) red[tid] += red[tid + o];
+          __syncthreads();
+        }
+        const double ssum = red[0];
+        __syncthreads();
+        const int mg = chunk * MC + m;
+        if (tid == 0 && mg < P.n_mfp) {
+          P.kappa[(int64_t)mg * L + l] = ksum * P.normalization;
+          P.supp[(int64_t)mg * L + l] = ssum * P.normalization;
         }
       }
     }
@@ -401,10 +540,13 @@ __global__ __launch_bounds__(kMaxBlock) void k_gcr_end(obte_bte_plan_t P, obte_g
   double* cn = G.c + kb * v.stride + v.base;
   const double* w = G.w + v.base;
   const double* x = G.x + v.base;
+  // converged lanes carry (u, c) = (0, 0): the transfer sweep may have skipped
+  // their bins, so their exit values are not T_op(0)
+  const bool act = G.lane_active[(int64_t)task * MC + lane] != 0;
   for (int f = slot; f < v.ncut; f += cpp) {
     const int64_t o = (int64_t)f * MC + lane;
-    un[o] = w[o];
-    cn[o] = w[o] - x[o];
+    un[o] = act ? w[o] : 0.0;
+    cn[o] = act ? w[o] - x[o] : 0.0;
   }
   for (int pass = 0; pass < 2; ++pass) {  // two MGS passes for orthogonality
     for (int j = 0; j < kb; ++j) {
@@ -456,23 +598,62 @@ __global__ __launch_bounds__(kMaxBlock) void k_gcr_end(obte_bte_plan_t P, obte_g
 
 __global__ void k_zero_int(int32_t* p) { *p = 0; }
 
-__global__ void k_gcr_reset(obte_bte_plan_t P, obte_gcr_t G, int64_t ny) {
-  const int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (gid < ny) G.y[gid] = 0.0;
-  if (gid < (int64_t)P.n_dirs * P.n_chunks) G.k[gid] = 0;
+// y = initial lattice temperature at each cut's exit cell (or 0); basis cleared.
+// One block per direction.
+__global__ void k_gcr_reset(obte_bte_plan_t P, obte_gcr_t G, const double* __restrict__ dt0) {
+  const int l = blockIdx.x;
+  const int cb = P.cut_off[l];
+  const int ncut = P.cut_off[l + 1] - cb;
+  const int mc = P.mc;
+  const int64_t per = (int64_t)ncut * mc;
+  for (int64_t i = threadIdx.x; i < per * P.n_chunks; i += blockDim.x) {
+    const int chunk = (int)(i / per);
+    const int64_t r = i - chunk * per;
+    const int f = (int)(r / mc);
+    const int m = (int)(r - (int64_t)f * mc);
+    double v = 0.0;
+    if (dt0 != nullptr) v = dt0[P.order[(int64_t)l * P.n + P.cut_exit[cb + f]]];
+    G.y[((int64_t)chunk * P.n_cut + cb + f) * mc + m] = v;
+  }
+  for (int c = threadIdx.x; c < P.n_chunks; c += blockDim.x) G.k[(int64_t)l * P.n_chunks + c] = 0;
 }
 
-// Fixed-order reduction over (direction, chunk) partials; one thread per cell.
+// Stage 1: sum the chunks of each direction in sweep order (coalesced),
+// in place into chunk 0 of the direction.
+__global__ void k_reduce_chunks(obte_bte_plan_t P) {
+  const int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t total = (int64_t)P.n_dirs * P.n;
+  if (gid >= total || P.n_chunks == 1) return;
+  const int l = (int)(gid / P.n);
+  const int p = (int)(gid - (int64_t)l * P.n);
+  const int dim = P.dim;
+  const int64_t o0 = (int64_t)l * P.n_chunks * P.n + p;
+  double s = P.part_t[o0], j0 = P.part_j[o0 * dim], j1 = P.part_j[o0 * dim + 1];
+  double j2 = dim == 3 ? P.part_j[o0 * dim + 2] : 0.0;
+  for (int c = 1; c < P.n_chunks; ++c) {
+    const int64_t o = o0 + (int64_t)c * P.n;
+    s += P.part_t[o];
+    j0 += P.part_j[o * dim];
+    j1 += P.part_j[o * dim + 1];
+    if (dim == 3) j2 += P.part_j[o * dim + 2];
+  }
+  P.part_t[o0] = s;
+  P.part_j[o0 * dim] = j0;
+  P.part_j[o0 * dim + 1] = j1;
+  if (dim == 3) P.part_j[o0 * dim + 2] = j2;
+}
+
+// Stage 2: fixed-order sum over directions, gathering each direction's
+// partial at the cell's position; one thread per cell.
 __global__ void k_reduce_cells(obte_bte_plan_t P, double* dt, double* jn, double* blk) {
   __shared__ double red[256];
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  const int ntask = P.n_dirs * P.n_chunks;
   const int dim = P.dim;
   double s = 0.0;
   if (c < P.n) {
     double j0 = 0.0, j1 = 0.0, j2 = 0.0;
-    for (int t = 0; t < ntask; ++t) {
-      const int64_t o = (int64_t)t * P.n + c;
+    for (int l = 0; l < P.n_dirs; ++l) {
+      const int64_t o = (int64_t)l * P.n_chunks * P.n + P.pos[(int64_t)l * P.n + c];
       s += P.part_t[o];
       j0 += P.part_j[o * dim + 0];
       j1 += P.part_j[o * dim + 1];
@@ -518,11 +699,13 @@ int check_plan(const obte_bte_plan_t* P, const char* who) {
   if (!P) return obte_fail(OBTE_EINVAL, who);
   if (P->n <= 0 || P->n_dirs <= 0 || P->n_mfp <= 0 || (P->dim != 2 && P->dim != 3))
     return obte_fail(OBTE_EINVAL, "plan: invalid sizes");
-  if (P->mc != 8 && P->mc != 16 && P->mc != 32 && P->mc != 64)
-    return obte_fail(OBTE_EINVAL, "plan: mc must be 8, 16, 32 or 64");
+  if (P->mc != 4 && P->mc != 8 && P->mc != 16 && P->mc != 32 && P->mc != 64)
+    return obte_fail(OBTE_EINVAL, "plan: mc must be 4, 8, 16, 32 or 64");
   if (P->n_chunks != (P->n_mfp + P->mc - 1) / P->mc) return obte_fail(OBTE_EINVAL, "plan: n_chunks != ceil(M/mc)");
-  if (P->block < 64 || P->block > kMaxBlock || P->block % 64 != 0)
-    return obte_fail(OBTE_EINVAL, "plan: block must be a multiple of 64 in [64, 1024]");
+  if (P->block < 64 || P->block > kMaxBlock || (P->block & (P->block - 1)) != 0)
+    return obte_fail(OBTE_EINVAL, "plan: block must be a power of two in [64, 1024]");
+  if (P->kin != 2 && P->kin != 3) return obte_fail(OBTE_EINVAL, "plan: kin must be 2 or 3");
+  if (P->ncut_max < 0) return obte_fail(OBTE_EINVAL, "plan: ncut_max < 0");
   return OBTE_OK;
 }
 
@@ -532,18 +715,28 @@ int launch_status(const char* who) {
   return OBTE_OK;
 }
 
+template <int MC, int KIN>
+void launch_sweep_k(const obte_bte_plan_t& P, int mode, const double* y, double* x, const int32_t* active,
+                    const int32_t* lane_active, hipStream_t st) {
+  const int ntask = P.n_dirs * P.n_chunks;
+  const int nblk = ((ntask + 7) / 8) * 8;
+  dim3 grid(nblk), blk(P.block);
+  if (mode == 0) hipLaunchKernelGGL((k_sweep<MC, 0, KIN>), grid, blk, 0, st, P, y, x, active, lane_active);
+  else if (mode == 1) hipLaunchKernelGGL((k_sweep<MC, 1, KIN>), grid, blk, 0, st, P, y, x, active, lane_active);
+  else hipLaunchKernelGGL((k_sweep<MC, 2, KIN>), grid, blk, 0, st, P, y, x, active, lane_active);
+}
+
 template <int MC>
 void launch_sweep(const obte_bte_plan_t& P, int mode, const double* y, double* x, const int32_t* active,
-                  hipStream_t st) {
-  dim3 grid(P.n_dirs * P.n_chunks), blk(P.block);
-  if (mode == 0) hipLaunchKernelGGL((k_sweep<MC, 0>), grid, blk, 0, st, P, y, x, active);
-  else if (mode == 1) hipLaunchKernelGGL((k_sweep<MC, 1>), grid, blk, 0, st, P, y, x, active);
-  else hipLaunchKernelGGL((k_sweep<MC, 2>), grid, blk, 0, st, P, y, x, active);
+                  const int32_t* lane_active, hipStream_t st) {
+  if (P.kin <= 2) launch_sweep_k<MC, 2>(P, mode, y, x, active, lane_active, st);
+  else launch_sweep_k<MC, 3>(P, mode, y, x, active, lane_active, st);
 }
 
 template <template <int> class F, typename... A>
 void dispatch_mc(int mc, A&&... a) {
   switch (mc) {
+    case 4: F<4>::run(a...); break;
     case 8: F<8>::run(a...); break;
     case 16: F<16>::run(a...); break;
     case 32: F<32>::run(a...); break;
@@ -552,7 +745,7 @@ void dispatch_mc(int mc, A&&... a) {
 }
 
 template <int MC>
-struct SweepL { static void run(const obte_bte_plan_t& P, int mode, const double* y, double* x, const int32_t* a, hipStream_t st) { launch_sweep<MC>(P, mode, y, x, a, st); } };
+struct SweepL { static void run(const obte_bte_plan_t& P, int mode, const double* y, double* x, const int32_t* a, const int32_t* la, hipStream_t st) { launch_sweep<MC>(P, mode, y, x, a, la, st); } };
 template <int MC>
 struct ProjL { static void run(const obte_bte_plan_t& P, const obte_gcr_t& G, hipStream_t st) { hipLaunchKernelGGL((k_gcr_project<MC>), dim3(P.n_dirs * P.n_chunks), dim3(P.block), 0, st, P, G); } };
 template <int MC>
@@ -590,12 +783,12 @@ int obte_bte_source(const obte_bte_plan_t* plan, obte_stream_t stream) {
 }
 
 int obte_bte_sweep(const obte_bte_plan_t* plan, int32_t mode, const double* y, double* x, const int32_t* active,
-                   obte_stream_t stream) {
+                   const int32_t* lane_active, obte_stream_t stream) {
   int rc = check_plan(plan, "obte_bte_sweep: null plan");
   if (rc) return rc;
   if (mode < 0 || mode > 2) return obte_fail(OBTE_EINVAL, "obte_bte_sweep: mode must be 0, 1 or 2");
   if (plan->n_cut > 0 && !y) return obte_fail(OBTE_EINVAL, "obte_bte_sweep: y is required when cuts exist");
-  dispatch_mc<SweepL>(plan->mc, *plan, (int)mode, y, x, active, (hipStream_t)stream);
+  dispatch_mc<SweepL>(plan->mc, *plan, (int)mode, y, x, active, lane_active, (hipStream_t)stream);
   return launch_status("obte_bte_sweep: launch failed");
 }
 
@@ -622,12 +815,10 @@ int obte_gcr_end_extend(const obte_bte_plan_t* plan, const obte_gcr_t* g, obte_s
   return launch_status("obte_gcr_end_extend: launch failed");
 }
 
-int obte_gcr_reset(const obte_bte_plan_t* plan, const obte_gcr_t* g, obte_stream_t stream) {
+int obte_gcr_reset(const obte_bte_plan_t* plan, const obte_gcr_t* g, const double* delta_t0, obte_stream_t stream) {
   int rc = check_plan(plan, "obte_gcr_reset: null plan");
   if (rc || (rc = check_gcr(g))) return rc;
-  const int64_t ny = (int64_t)plan->n_chunks * plan->n_cut * plan->mc;
-  const int64_t nt = ny > (int64_t)plan->n_dirs * plan->n_chunks ? ny : (int64_t)plan->n_dirs * plan->n_chunks;
-  hipLaunchKernelGGL(k_gcr_reset, dim3(blocks_for(nt, 256)), dim3(256), 0, (hipStream_t)stream, *plan, *g, ny);
+  hipLaunchKernelGGL(k_gcr_reset, dim3(plan->n_dirs), dim3(256), 0, (hipStream_t)stream, *plan, *g, delta_t0);
   return launch_status("obte_gcr_reset: launch failed");
 }
 
@@ -639,6 +830,8 @@ int obte_bte_reduce(const obte_bte_plan_t* plan, double* delta_t_new, double* j_
   const unsigned nb = blocks_for(plan->n, 256);
   if (nb > 4096) return obte_fail(OBTE_ELIMIT, "obte_bte_reduce: more than 4096*256 cells");
   hipStream_t st = (hipStream_t)stream;
+  const int64_t total = (int64_t)plan->n_dirs * plan->n;
+  hipLaunchKernelGGL(k_reduce_chunks, dim3(blocks_for(total, 256)), dim3(256), 0, st, *plan);
   hipLaunchKernelGGL(k_reduce_cells, dim3(nb), dim3(256), 0, st, *plan, delta_t_new, j_new, plan->scratch);
   if (plan->n_wall > 0) {
     if (!tb_new) return obte_fail(OBTE_EINVAL, "obte_bte_reduce: null tb output");

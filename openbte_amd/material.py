@@ -17,6 +17,11 @@ import numpy as np
 from .objects import Material
 
 
+def _inv(gamma):
+    """1/gamma with empty bins mapped to 0 (openbte/material.py:163)."""
+    return np.divide(1.0, gamma, out=np.zeros_like(gamma), where=gamma != 0)
+
+
 def kappa_weights(n_mfp: int, mfp_min: float, mfp_max: float, kappa_bulk: float = 150.0,
                   median: float = 300.0, sigma_ln: float = 1.2, seed: int = 0,
                   n_draws: int = 1_000_000):
@@ -52,7 +57,7 @@ def synthetic_material_2d(n_mfp: int = 30, n_phi: int = 48, mfp_min: float = 1.0
     lam = mfps * 1e-9
     gamma = (2.0 * w / lam ** 2)[:, None] / n_phi * np.ones((1, n_phi))
     sigma = gamma[:, :, None] * lam[:, None, None] * s[None, :, :]
-    kappa = np.einsum('mli,ml,mlj->ij', sigma, 1.0 / gamma, sigma)
+    kappa = np.einsum('mli,ml,mlj->ij', sigma, _inv(gamma), sigma)
     t_coeff = gamma / gamma.sum()
     h = 2.0 / 3.0 * np.sum(gamma * (lam[:, None] * s[None, :, 0]).clip(min=0)) * 1e-9
     return Material(thermal_conductivity=kappa,
@@ -86,7 +91,7 @@ def synthetic_material_3d(n_mfp: int = 20, n_theta: int = 8, n_phi: int = 8,
     lam = mfps * 1e-9
     gamma = (3.0 * w / lam ** 2)[:, None] / n_dirs * np.ones((1, n_dirs))
     sigma = gamma[:, :, None] * lam[:, None, None] * s[None, :, :]
-    kappa = np.einsum('mli,ml,mlj->ij', sigma, 1.0 / gamma, sigma)
+    kappa = np.einsum('mli,ml,mlj->ij', sigma, _inv(gamma), sigma)
     t_coeff = gamma / gamma.sum()
     return Material(thermal_conductivity=kappa,
                     sigma=sigma * 1e-9,

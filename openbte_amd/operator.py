@@ -220,3 +220,35 @@ def build_schedule(op: HostOperator, dirs: Optional[np.ndarray] = None, cut_cand
     return Schedule(order=order, pos=pos, lvl_off=lvl_off, lvl_ptr=lvl_ptr, in_ptr=in_ptr, in_off=in_off,
                     in_src=in_src, in_face=in_face, cut_off=cut_off, cut_exit=cut_exit, cut_face=cut_face,
                     n_levels=nlev.astype(np.int64))
+
+
+def max_inflow(sched: Schedule) -> int:
+    """Largest number of inflow faces of one cell over all directions."""
+    return int(np.diff(sched.in_ptr.reshape(len(sched.n_levels), -1), axis=1).max(initial=0))
+
+
+def fixed_width_inflow(sched: Schedule, n: int, kin: int):
+    """Per-direction fixed-width inflow slots [L][kin][n] in position order.
+
+    Slot k of position p holds the k-th inflow entry of p: its upstream row
+    (a position, or n + cut index for a cut face) and its directed face.
+    Unused slots point at p itself with face -1 (their coefficient is 0).
+    Every row lies in [0, n + cuts of the direction)."""
+    L = len(sched.n_levels)
+    nb_src = np.empty((L, kin, n), np.int32)
+    nb_face = np.full((L, kin, n), -1, np.int32)
+    nb_src[:] = np.arange(n, dtype=np.int32)[None, None, :]
+    in_ptr = sched.in_ptr.reshape(L, n + 1).astype(np.int64)
+    for l in range(L):
+        e0, e1 = int(sched.in_off[l]), int(sched.in_off[l + 1])
+        ip = in_ptr[l]
+        cnt = np.diff(ip)
+        if cnt.size and cnt.max() > kin:
+            raise ValueError(f'direction {l}: {cnt.max()} inflow faces exceed {kin} slots')
+        rows = np.repeat(np.arange(n), cnt)
+        k = np.arange(e1 - e0) - ip[rows]
+        src = sched.in_src[e0:e1].astype(np.int64)
+        src = np.where(src >= 0, src, n + (~src))
+        nb_src[l, k, rows] = src
+        nb_face[l, k, rows] = sched.in_face[e0:e1]
+    return nb_src, nb_face

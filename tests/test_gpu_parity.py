@@ -38,7 +38,7 @@ def test_fourier_matches_oracle(small_case):
     assert dJ < 1e-6 * np.abs(ref.variables['Flux_Fourier']['data']).max()
 
 
-@pytest.mark.parametrize("mc,block", [(8, 64), (16, 256), (32, 128)])
+@pytest.mark.parametrize("mc,block", [(4, 64), (8, 128), (16, 256), (32, 64), (64, 128)])
 def test_bte_matches_oracle_small(small_case, mc, block):
     from oracle import bte_oracle as bo, fourier_oracle as fo
     from openbte_amd.bte import BTE_RTA, SweepConfig
@@ -56,3 +56,40 @@ def test_bte_matches_oracle_small(small_case, mc, block):
     np.testing.assert_allclose(out.aux['suppression'], ref.aux['suppression'], rtol=1e-8, atol=1e-12)
     Jr = ref.variables['Flux_BTE']['data']
     assert np.abs(out.variables['Flux_BTE']['data'] - Jr).max() < 1e-7 * np.abs(Jr).max()
+
+
+def test_bte_closed_form_quads():
+    """Uniform quads (kin = 2 or 3 inflow slots): exact closed-form kappa after one iteration."""
+    from oracle import fourier_oracle as fo
+    from openbte_amd.bte import BTE_RTA
+    from openbte_amd.geometry import quad_cell_2d
+    from openbte_amd.material import synthetic_material_2d
+    from openbte_amd.objects import BoundaryConditions, EffectiveThermalConductivity
+    L, N = 10.0, 12
+    geo = quad_cell_2d(L, N)
+    mat = synthetic_material_2d(5, 12, 1.0, 1.0e3)
+    bcs = BoundaryConditions(periodic={'Periodic_x': 1.0, 'Periodic_y': 0.0})
+    eff = EffectiveThermalConductivity(contact='Periodic_x', normalization=-1.0)
+    f = fo.fourier(geo, mat.thermal_conductivity, bcs, eff)
+    out = BTE_RTA(geo, mat, bcs, fourier=f, effective_thermal_conductivity=eff, bte_max_iter=5)
+    sx = mat.vmfp[:, 0]
+    sig = mat.sigma[:, :, 0]
+    expect = np.sum(mat.mfps[:, None] * sx[None, :] * sig) + 0.5 * (L / N) * np.sum(np.abs(sig))
+    assert abs(out.kappa_eff / expect - 1) < 1e-11
+    assert out.aux['n_iter'] == 1
+
+
+def test_bte_c1_scale_matches_oracle():
+    """~2,000-cell pore cell (30 MFP x 48 directions), 12 outer iterations."""
+    from oracle import bte_oracle as bo, fourier_oracle as fo
+    from openbte_amd.bte import BTE_RTA
+    geo, mat, bcs, eff = _case(10.0, 0.25, 2000, 30, 48, 0)
+    f = fo.fourier(geo, mat.thermal_conductivity, bcs, eff)
+    ref = bo.bte_rta(geo, mat, bcs, fourier=f, effective_thermal_conductivity=eff, n_workers=8)
+    out = BTE_RTA(geo, mat, bcs, fourier=f, effective_thermal_conductivity=eff)
+    assert out.aux['n_iter'] == ref.aux['n_iter']
+    rel = abs(out.kappa_eff / ref.kappa_eff - 1)
+    dT = np.abs(out.variables['Temperature_BTE']['data'] - ref.variables['Temperature_BTE']['data']).max()
+    print('C1 kappa', out.kappa_eff, ref.kappa_eff, rel, dT, out.aux['n_iter'])
+    assert rel < KAPPA_RTOL
+    assert dT < T_ATOL
