@@ -114,13 +114,15 @@ def cpu_baseline(geo, mat, bcs, eff, fourier_T, budget_s=20.0):
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--steps', type=int, default=10)
+    ap.add_argument('--steps', type=int, default=20)
     ap.add_argument('--warmup', type=int, default=2)
     ap.add_argument('--config', default='plain200k', choices=sorted(CONFIGS))
     ap.add_argument('--mc', type=int, default=None)
     ap.add_argument('--block', type=int, default=None)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--cpu-budget', type=float, default=20.0)
+    ap.add_argument('--copy-calibration', action='store_true',
+                    help='run one 1 GiB device-to-device copy (known bytes) for HBM counter calibration')
     ap.add_argument('--no-fourier', action='store_true',
                     help='start from DeltaT = 0 (BTE_RTA with fourier=None); for kernel profiling')
     args = ap.parse_args(argv)
@@ -145,12 +147,19 @@ def main(argv=None):
         cpu = cpu_baseline(geo, mat, bcs, eff, T_cpu, budget_s=args.cpu_budget)
         print(f'# cpu baseline {cpu["value"]:.3e} updates/s on {cpu["cores"]} cores', file=sys.stderr, flush=True)
     t_setup = time.perf_counter()
+    fourier_info = None
     if args.no_fourier:
         T0 = np.zeros(n)
     else:
+        torch.cuda.synchronize()
+        tf0 = time.perf_counter()
         f = Fourier(geo, mat.thermal_conductivity, bcs, eff, verbose=False)
         T0 = f.variables['Temperature_Fourier']['data']
-        print(f'# fourier done ({f.aux["n_iter"]} iterations)', file=sys.stderr, flush=True)
+        tf = time.perf_counter() - tf0
+        fourier_info = {'seconds': tf, 'iterations': f.aux['n_iter'],
+                        'cg_iterations': [c[0] for c in f.aux['cg']],
+                        'kappa_diffusive': f.aux['kappa_diffusive']}
+        print(f'# fourier done ({f.aux["n_iter"]} iterations, {tf:.2f}s)', file=sys.stderr, flush=True)
     cfg = SweepConfig()
     if args.mc:
         cfg.mc = args.mc
@@ -161,6 +170,13 @@ def main(argv=None):
     torch.cuda.synchronize()
     t_setup = time.perf_counter() - t_setup
     print(f'# setup {t_setup:.1f}s, cuts={dev.n_cut}', file=sys.stderr, flush=True)
+
+    if args.copy_calibration:
+        a = torch.ones(1 << 27, dtype=torch.float64, device='cuda')  # 1 GiB
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
 
     # warmup: throwaway solve
     dev.reset(T0)
@@ -177,9 +193,11 @@ def main(argv=None):
     t0 = time.perf_counter()
     for _ in range(args.steps):
         exts.append(dev.outer_iteration(timer=timer))
-        norms.append(dev.norm2)
+        norms.append(dev.norm2.clone())
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    nrm = [float(np.linalg.norm(T0))] + [float(np.sqrt(x.item())) for x in norms]
+    errors = [abs(nrm[i + 1] - nrm[i]) / nrm[i + 1] for i in range(args.steps)]
     kappa = float(dev.buf['kappa'].sum().item())
     updates = n * M * L * args.steps
     value = updates / dt
@@ -193,6 +211,14 @@ def main(argv=None):
           f'full sweep {t_full * 1e3:.3f} ms; transfer sweeps {len(part)} mean '
           f'{(np.mean([d for _, d in part]) * 1e3 if part else 0):.3f} ms; kappa {kappa:.6f}',
           file=sys.stderr, flush=True)
+
+    traffic = None
+    tf = os.path.join(ROOT, 'profiles', f'k_sweep_{args.config}_traffic.json')
+    if os.path.exists(tf):
+        rec = json.load(open(tf))
+        if rec.get('mc') == cfg.mc and rec.get('block') == cfg.block:
+            f_, w_ = rec['fetch_kib_per_dispatch'], rec['write_kib_per_dispatch']
+            traffic = 1024.0 * (f_['mode0'] + w_['mode0'] + f_['mode2'] + w_['mode2']) / 2
 
     out = {
         'metric': 'cell*mode updates/s (BTE outer iteration, all systems to direct-solver accuracy)',
@@ -210,9 +236,12 @@ def main(argv=None):
         'config': {'workload': f'{args.config}: L={CONFIGS[args.config][0]:g}nm porosity 0.25, n={n} cells, '
                                f'{M} MFP x {L} directions = {M * L} systems',
                    'mc': cfg.mc, 'block': cfg.block, 'kappa_eff_after_steps': kappa,
-                   'extensions_per_step': exts, 'n_cut_faces': dev.n_cut},
+                   'extensions_per_step': exts, 'n_cut_faces': dev.n_cut,
+                   'outer_error_per_step': errors, 'kmax': cfg.kmax, 'gcr_rtol': cfg.rtol,
+                   'fourier': fourier_info, 'setup_s': t_setup},
         'roofline': {'bound': 'hbm', 'achieved': achieved / 1e9, 'peak': HBM_PEAK / 1e9, 'unit': 'GB/s',
-                     'frac': achieved / HBM_PEAK, 'traffic': None,
+                     'frac': achieved / HBM_PEAK, 'traffic': traffic,
+                     'traffic_source': 'rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE (profiles/)' if traffic else None,
                      'kernel': 'k_sweep (full sweeps, modes 0/2)', 'bytes_per_launch': sweep_bytes,
                      'mean_launch_s': t_full},
         'cpu_baseline': cpu,

@@ -36,9 +36,10 @@ class SweepConfig:
     """Launch and Krylov knobs (picked per workload by ``openbte_amd.tune``)."""
     mc: int = 8              # MFP bins per workgroup chunk (looped by each lane)
     block: int = 256         # threads per sweep workgroup (one cell per lane per level)
-    kmax: int = 48           # recycled GCR basis capacity per system
-    rtol: float = 1e-12      # interface residual tolerance
+    kmax: int = 128          # recycled GCR basis capacity per system (capped by basis_bytes)
+    rtol: float = 1e-10      # interface residual tolerance
     max_extend: int = 64     # cap on basis extensions per outer iteration
+    basis_bytes: float = 24e9  # HBM budget of the recycled basis (u and c)
 
 
 @dataclass
@@ -141,6 +142,7 @@ class BTEDevice:
         b['scratch'] = torch.zeros(4096 + 8, dtype=f64, device=dev)
         # interface Krylov state
         ny = max(1, nch * self.n_cut * mc)
+        cfg.kmax = int(max(8, min(cfg.kmax, cfg.basis_bytes // (16 * ny))))
         g = self.gbuf = {}
         g['y'] = torch.zeros(ny, dtype=f64, device=dev)
         g['r'] = torch.zeros(ny, dtype=f64, device=dev)

@@ -1,0 +1,40 @@
+"""The C-ABI library loads and exports every symbol include/openbte_amd.h
+declares (no compute calls: no GPU is needed)."""
+import ctypes
+import os
+import re
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_header_symbols():
+    from openbte_amd import _lib
+    lib = _lib.load()
+    header = open(os.path.join(ROOT, 'include', 'openbte_amd.h')).read()
+    names = sorted(set(re.findall(r'\b(obte_[a-z0-9_]+)\s*\(', header)))
+    assert len(names) >= 15
+    for name in names:
+        assert hasattr(lib, name), name
+    assert set(_lib.EXPORTED) <= set(names)
+
+
+def test_error_convention_without_gpu():
+    from openbte_amd import _lib
+    lib = _lib.load()
+    rc = lib.obte_bte_assemble(None, None)
+    assert rc < 0
+    assert b'plan' in lib.obte_last_error()
+    h = ctypes.c_void_p()
+    rc = lib.obte_sched_create(0, 0, 2, None, None, None, None, 1, None, ctypes.byref(h))
+    assert rc == -1
+
+
+def test_struct_layout_matches_header():
+    """Every field of the C structs is mirrored, in order, by the ctypes structs."""
+    from openbte_amd import _lib
+    header = open(os.path.join(ROOT, 'include', 'openbte_amd.h')).read()
+    for cname, py in (('obte_bte_plan', _lib.BtePlan), ('obte_gcr', _lib.Gcr), ('obte_fourier_plan', _lib.FourierPlan)):
+        body = re.search(r'typedef struct ' + cname + r' \{(.*?)\}', header, re.S).group(1)
+        body = re.sub(r'/\*.*?\*/', '', body, flags=re.S)
+        fields = re.findall(r'\b(?:int32_t|int64_t|double|uint8_t)\s*\**\s*(\w+)\s*;', body)
+        assert fields == [f[0] for f in py._fields_], cname
