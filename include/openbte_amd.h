@@ -124,6 +124,22 @@ typedef struct obte_bte_plan {
   double* scratch;         /* >= 4096 doubles of reduction scratch */
 } obte_bte_plan_t;
 
+/* Input / output buffers of one sweep launch for one problem. */
+typedef struct obte_sweep_io {
+  const double* y;            /* [n_chunks][n_cut][mc] cut-face inflows */
+  double* x;                  /* [n_chunks][n_cut][mc] exit values (may be NULL) */
+  const int32_t* active;      /* [L][n_chunks] task flags (NULL = all) */
+  const int32_t* lane_active; /* [L][n_chunks][mc] per-system flags (mode 1; NULL = all) */
+} obte_sweep_io_t;
+
+/* Outputs of the mode-sum reduction of one problem. */
+typedef struct obte_reduce_io {
+  double* dt_new;  /* [n] */
+  double* j_new;   /* [n][dim] */
+  double* tb_new;  /* [n_wall] */
+  double* norm2;   /* [1] ||dt_new||^2 */
+} obte_reduce_io_t;
+
 /* Face-flux assembly of D_l, the inflow coefficients and the perturbation
  * (openbte/bte.py:47-124). */
 int obte_bte_assemble(const obte_bte_plan_t* plan, obte_stream_t stream);
@@ -169,14 +185,51 @@ int obte_gcr_project(const obte_bte_plan_t* plan, const obte_gcr_t* g, obte_stre
 int obte_gcr_begin_extend(const obte_bte_plan_t* plan, const obte_gcr_t* g, obte_stream_t stream);
 /* c = u - x, orthogonalise against the basis, normalise, update y and r. */
 int obte_gcr_end_extend(const obte_bte_plan_t* plan, const obte_gcr_t* g, obte_stream_t stream);
-/* Clear the basis (k = 0); y = delta_t0 at each cut's exit cell (0 if
- * delta_t0 is NULL). */
-int obte_gcr_reset(const obte_bte_plan_t* plan, const obte_gcr_t* g, const double* delta_t0, obte_stream_t stream);
+/* Clear the basis (k = 0); y = the plan's current delta_t at each cut's exit
+ * cell (use_delta_t != 0) or 0. */
+int obte_gcr_reset(const obte_bte_plan_t* plan, const obte_gcr_t* g, int32_t use_delta_t, obte_stream_t stream);
 
 /* Fixed-order reduction of the partial buffers into DeltaT, J, TB
  * (openbte/bte.py:245-249 -> openbte/objects.py:86-101); norm2 = ||DeltaT||^2. */
 int obte_bte_reduce(const obte_bte_plan_t* plan, double* delta_t_new, double* j_new, double* tb_new,
                     double* norm2, obte_stream_t stream);
+
+/* ---- batch of problems (many geometries / requests in one launch) -------
+ *
+ * All arrays below are device arrays.  Every problem keeps its own plan and
+ * Krylov state; the sweep runs as one persistent launch whose workgroups drain
+ * 8 XCD-local task queues.  A problem with live[p] == 0 is skipped by every
+ * kernel, so its state stays that of its last iteration. */
+typedef struct obte_bte_batch {
+  int32_t n_problems;
+  int32_t n_tasks;       /* sum over problems of n_dirs * n_chunks */
+  int32_t mc;            /* shared launch shape (equal to every plan's) */
+  int32_t block;
+  int32_t kin;
+  int32_t groups;        /* persistent workgroups of a sweep launch */
+  int32_t max_elems;     /* max n_dirs * n over problems */
+  int32_t max_dirs;      /* max n_dirs */
+  int32_t max_cells;     /* max n */
+  int32_t max_walls;     /* max n_wall */
+  const obte_bte_plan_t* plans;  /* [n_problems] */
+  const obte_gcr_t* gcrs;        /* [n_problems] */
+  const obte_sweep_io_t* io;     /* [3][n_problems]: sweep buffers of modes 0, 1, 2 */
+  const obte_reduce_io_t* rio;   /* [n_problems] */
+  const int32_t* live;           /* [n_problems] */
+  const int32_t* task_pid;       /* [n_tasks] problem of each task, grouped by queue */
+  const int32_t* task_local;     /* [n_tasks] l * n_chunks + chunk */
+  const int32_t* queue_off;      /* [9] task range of each XCD queue */
+  int32_t* queue_cnt;            /* [8] queue counters (cleared by each sweep launch) */
+} obte_bte_batch_t;
+
+int obte_batch_assemble(const obte_bte_batch_t* b, obte_stream_t stream);
+int obte_batch_source(const obte_bte_batch_t* b, obte_stream_t stream);
+int obte_batch_sweep(const obte_bte_batch_t* b, int32_t mode, obte_stream_t stream);
+/* op 0 project, 1 begin_extend, 2 end_extend (as the obte_gcr_* calls) */
+int obte_batch_gcr(const obte_bte_batch_t* b, int32_t op, obte_stream_t stream);
+int obte_batch_gcr_reset(const obte_bte_batch_t* b, int32_t use_delta_t, obte_stream_t stream);
+/* reduce into rio, then DeltaT, TB <- rio.dt_new, rio.tb_new for live problems */
+int obte_batch_reduce(const obte_bte_batch_t* b, obte_stream_t stream);
 
 /* ---- Fourier (openbte/fourier.py:15-185) ------------------------------ */
 

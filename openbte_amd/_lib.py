@@ -43,6 +43,26 @@ class Gcr(C.Structure):
     ]
 
 
+class SweepIO(C.Structure):
+    """Mirror of obte_sweep_io_t."""
+    _fields_ = [('y', _p), ('x', _p), ('active', _p), ('lane_active', _p)]
+
+
+class ReduceIO(C.Structure):
+    """Mirror of obte_reduce_io_t."""
+    _fields_ = [('dt_new', _p), ('j_new', _p), ('tb_new', _p), ('norm2', _p)]
+
+
+class Batch(C.Structure):
+    """Mirror of obte_bte_batch_t."""
+    _fields_ = [
+        ('n_problems', _i32), ('n_tasks', _i32), ('mc', _i32), ('block', _i32), ('kin', _i32), ('groups', _i32),
+        ('max_elems', _i32), ('max_dirs', _i32), ('max_cells', _i32), ('max_walls', _i32),
+        ('plans', _p), ('gcrs', _p), ('io', _p), ('rio', _p), ('live', _p), ('task_pid', _p), ('task_local', _p),
+        ('queue_off', _p), ('queue_cnt', _p),
+    ]
+
+
 class FourierPlan(C.Structure):
     """Mirror of obte_fourier_plan_t."""
     _fields_ = [
@@ -68,7 +88,13 @@ _SIGNATURES = {
     'obte_gcr_project': [C.POINTER(BtePlan), C.POINTER(Gcr), _p],
     'obte_gcr_begin_extend': [C.POINTER(BtePlan), C.POINTER(Gcr), _p],
     'obte_gcr_end_extend': [C.POINTER(BtePlan), C.POINTER(Gcr), _p],
-    'obte_gcr_reset': [C.POINTER(BtePlan), C.POINTER(Gcr), _p, _p],
+    'obte_gcr_reset': [C.POINTER(BtePlan), C.POINTER(Gcr), _i32, _p],
+    'obte_batch_assemble': [C.POINTER(Batch), _p],
+    'obte_batch_source': [C.POINTER(Batch), _p],
+    'obte_batch_sweep': [C.POINTER(Batch), _i32, _p],
+    'obte_batch_gcr': [C.POINTER(Batch), _i32, _p],
+    'obte_batch_gcr_reset': [C.POINTER(Batch), _i32, _p],
+    'obte_batch_reduce': [C.POINTER(Batch), _p],
     'obte_bte_reduce': [C.POINTER(BtePlan), _p, _p, _p, _p, _p],
     'obte_fourier_solve': [C.POINTER(FourierPlan), _p, _p, _p, _p, _p, _f64, _i32, _i32,
                            C.POINTER(_i32), C.POINTER(_f64), _p],

@@ -1,0 +1,223 @@
+"""Many BTE solves on the device at once: one persistent sweep launch over
+every (problem, direction, MFP chunk) task.
+
+Each problem keeps its own operator, schedule, Krylov basis and state (a
+``BTEDevice``); the batch adds device arrays of their plan structs and a task
+table.  Tasks of one (problem, direction) share one of 8 XCD-local queues,
+assigned greedily by cost (levels x cells) so that the queues balance, and
+each queue is ordered by decreasing cost.  Every problem follows its own
+outer iteration and stop rule (openbte/bte.py:193-256): a converged problem is
+marked not live and every kernel skips it, so its result is the one a solve
+of that problem alone produces.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+from typing import List, Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import _lib
+from .bte import BTEDevice, SweepConfig
+from .objects import SolverResults
+from .operator import build_host_operator
+
+N_QUEUES = 8
+
+
+def _struct_tensor(structs, dev) -> torch.Tensor:
+    """Copy an array of ctypes structs to a device byte tensor."""
+    raw = b''.join(bytes(s) for s in structs)
+    return torch.frombuffer(bytearray(raw), dtype=torch.uint8).to(dev)
+
+
+class BTEBatch:
+    """A batch of problems sharing the launch shape (mc, block, kin)."""
+
+    def __init__(self, ops: Sequence, cfg: Optional[SweepConfig] = None, scheds: Optional[Sequence] = None,
+                 groups: Optional[int] = None, device=None):
+        if not torch.cuda.is_available():
+            raise RuntimeError('openbte_amd.BTEBatch needs a ROCm GPU; there is no CPU fallback')
+        self.dev = device or torch.device('cuda', torch.cuda.current_device())
+        self.cfg = cfg or SweepConfig()
+        P = len(ops)
+        self.n_problems = P
+        self.n_active = torch.zeros(P, dtype=torch.int32, device=self.dev)
+        self.norm2 = torch.zeros(P, dtype=torch.float64, device=self.dev)
+        self.devs: List[BTEDevice] = []
+        for i, op in enumerate(ops):
+            d = BTEDevice(op, sched=None if scheds is None else scheds[i], cfg=SweepConfig(**vars(self.cfg)),
+                          device=self.dev, n_active=self.n_active[i:i + 1], norm2=self.norm2[i:i + 1])
+            self.devs.append(d)
+        kins = {d.kin for d in self.devs}
+        if len(kins) != 1:
+            raise ValueError('a batch needs one inflow-slot width (do not mix 2D and 3D problems)')
+        self.kin = kins.pop()
+        self.stream = torch.cuda.current_stream(self.dev).cuda_stream
+        self.live = torch.ones(P, dtype=torch.int32, device=self.dev)
+        self._build_tasks()
+        self._build_structs(groups)
+        self.n_iter = np.zeros(P, dtype=np.int64)
+        self.error = np.ones(P)
+        self.norm_old = np.zeros(P)
+
+    # -- task table --------------------------------------------------------
+    def _build_tasks(self):
+        items = []  # (cost, pid, l, nch)
+        for pid, d in enumerate(self.devs):
+            lev = d.sched.n_levels
+            for l in range(d.op.n_dirs):
+                # a direction's chunks run at about levels x (cells / block) passes each
+                cost = float(lev[l]) * max(1.0, d.op.n / d.cfg.block) * d.n_chunks
+                items.append((cost, pid, l, d.n_chunks))
+        items.sort(key=lambda t: -t[0])
+        load = np.zeros(N_QUEUES)
+        queues = [[] for _ in range(N_QUEUES)]
+        for cost, pid, l, nch in items:
+            q = int(np.argmin(load))
+            load[q] += cost
+            queues[q].append((cost, pid, l, nch))
+        tp, tl, off = [], [], [0]
+        for q in queues:
+            for cost, pid, l, nch in q:
+                for c in range(nch):
+                    tp.append(pid)
+                    tl.append(l * nch + c)
+            off.append(len(tp))
+        self.task_pid = torch.as_tensor(np.asarray(tp, np.int32)).to(self.dev)
+        self.task_local = torch.as_tensor(np.asarray(tl, np.int32)).to(self.dev)
+        self.queue_off = torch.as_tensor(np.asarray(off, np.int32)).to(self.dev)
+        self.queue_cnt = torch.zeros(N_QUEUES, dtype=torch.int32, device=self.dev)
+        self.n_tasks = len(tp)
+        self.queue_load = load
+
+    def _build_structs(self, groups):
+        devs = self.devs
+        P = self.n_problems
+        for d in devs:
+            # fixed state pointers: DeltaT/TB in slot 0, the reduction writes slot 1
+            d.plan.delta_t = d.buf['dt'][0].data_ptr()
+            d.plan.tb = d.buf['tb'][0].data_ptr()
+        self.plans_t = _struct_tensor([d.plan for d in devs], self.dev)
+        self.gcrs_t = _struct_tensor([d.gcr for d in devs], self.dev)
+        io = []
+        for mode in range(3):
+            for d in devs:
+                g = d.gbuf
+                s = _lib.SweepIO()
+                if mode == 1:
+                    s.y, s.x = g['w'].data_ptr(), g['x'].data_ptr()
+                    s.active, s.lane_active = g['active'].data_ptr(), g['lane_active'].data_ptr()
+                else:
+                    s.y, s.x = g['y'].data_ptr(), g['x'].data_ptr()
+                io.append(s)
+        self.io_t = _struct_tensor(io, self.dev)
+        rio = []
+        for d in devs:
+            r = _lib.ReduceIO()
+            r.dt_new = d.buf['dt'][1].data_ptr()
+            r.j_new = d.buf['j'].data_ptr()
+            r.tb_new = d.buf['tb'][1].data_ptr()
+            r.norm2 = d.norm2.data_ptr()
+            rio.append(r)
+        self.rio_t = _struct_tensor(rio, self.dev)
+        B = self.b = _lib.Batch()
+        B.n_problems, B.n_tasks = P, self.n_tasks
+        B.mc, B.block, B.kin = self.cfg.mc, self.cfg.block, self.kin
+        props = torch.cuda.get_device_properties(self.dev)
+        cus = getattr(props, 'multi_processor_count', 256)
+        B.groups = int(groups or min(self.n_tasks, cus * max(1, 1024 // self.cfg.block)))
+        B.max_elems = max(d.op.n * d.op.n_dirs for d in devs)
+        B.max_dirs = max(d.op.n_dirs for d in devs)
+        B.max_cells = max(d.op.n for d in devs)
+        B.max_walls = max(len(d.op.i_diffuse) for d in devs)
+        B.plans, B.gcrs, B.io, B.rio = (self.plans_t.data_ptr(), self.gcrs_t.data_ptr(), self.io_t.data_ptr(),
+                                        self.rio_t.data_ptr())
+        B.live = self.live.data_ptr()
+        B.task_pid, B.task_local = self.task_pid.data_ptr(), self.task_local.data_ptr()
+        B.queue_off, B.queue_cnt = self.queue_off.data_ptr(), self.queue_cnt.data_ptr()
+        _lib.call('obte_batch_assemble', self.b, self.stream)
+
+    # -- iteration ---------------------------------------------------------
+    def reset(self, delta_t0: Sequence[np.ndarray]) -> None:
+        for d, t0 in zip(self.devs, delta_t0):
+            x = torch.as_tensor(np.asarray(t0, dtype=np.float64)).to(self.dev)
+            d.buf['dt'][0].copy_(x)
+            if len(d.op.i_diffuse):
+                d.buf['tb'][0][:len(d.op.i_diffuse)].copy_(x[torch.as_tensor(d.op.i_diffuse).to(self.dev)])
+        self.live.fill_(1)
+        self.n_iter[:] = 0
+        self.error[:] = 1.0
+        self.norm_old = np.array([float(np.linalg.norm(t)) for t in delta_t0])
+        _lib.call('obte_batch_gcr_reset', self.b, 1, self.stream)
+
+    def outer_iteration(self, max_extend: Optional[int] = None) -> int:
+        """One outer iteration of every live problem; returns the extensions."""
+        b, st = self.b, self.stream
+        max_extend = self.cfg.max_extend if max_extend is None else max_extend
+        _lib.call('obte_batch_source', b, st)
+        _lib.call('obte_batch_sweep', b, 0, st)
+        _lib.call('obte_batch_gcr', b, 0, st)
+        ext = 0
+        while ext < max_extend and int(self.n_active.sum().item()) > 0:
+            _lib.call('obte_batch_gcr', b, 1, st)
+            _lib.call('obte_batch_sweep', b, 1, st)
+            _lib.call('obte_batch_gcr', b, 2, st)
+            ext += 1
+        _lib.call('obte_batch_sweep', b, 2, st)
+        _lib.call('obte_batch_reduce', b, st)
+        return ext
+
+    def solve(self, delta_t0, bte_max_iter: int = 100, bte_rtol: float = 1e-4, on_iteration=None):
+        """Iterate every problem to its own stop rule (openbte/bte.py:193-196,251-254)."""
+        self.reset(delta_t0)
+        live = np.ones(self.n_problems, bool)
+        while live.any():
+            self.outer_iteration()
+            nrm = np.sqrt(self.norm2.cpu().numpy())
+            for p in np.nonzero(live)[0]:
+                self.n_iter[p] += 1
+                self.error[p] = abs(nrm[p] - self.norm_old[p]) / nrm[p]
+                self.norm_old[p] = nrm[p]
+                if not (self.error[p] > bte_rtol and self.n_iter[p] < bte_max_iter):
+                    live[p] = False
+            self.live.copy_(torch.as_tensor(live.astype(np.int32)))
+            if on_iteration is not None:
+                on_iteration(self, live)
+        return self.n_iter.copy()
+
+    def results(self, p: int) -> SolverResults:
+        d = self.devs[p]
+        op = d.op
+        K = d.buf['kappa'].view(op.n_mfp, op.n_dirs).cpu().numpy().copy()
+        S = d.buf['supp'].view(op.n_mfp, op.n_dirs).cpu().numpy().copy()
+        DeltaT = d.buf['dt'][0].cpu().numpy().copy()
+        J = d.buf['j'].view(op.n, op.dim).cpu().numpy().copy()
+        if op.contact_kind == 0:
+            K[:] = 0
+            S[:] = 0
+        kappa = float(K.sum())
+        return SolverResults(kappa_eff=kappa,
+                             variables={'Temperature_BTE': {'data': DeltaT, 'units': 'K'},
+                                        'Flux_BTE': {'data': J * 1e9, 'units': 'W/m/m'}},
+                             aux={'suppression': S, 'kappa_eff': kappa, 'n_iter': int(self.n_iter[p])})
+
+
+def BTE_RTA_batch(problems, bte_max_iter: int = 100, bte_rtol: float = 1e-4,
+                  config: Optional[SweepConfig] = None) -> List[SolverResults]:
+    """Solve several problems together.  ``problems`` is a sequence of dicts
+    with the ``BTE_RTA`` arguments (geo, mat, bcs, fourier,
+    effective_thermal_conductivity, heat_source)."""
+    ops, t0 = [], []
+    for pr in problems:
+        geo = pr['geo']
+        ops.append(build_host_operator(geo, pr['mat'], pr['bcs'], pr.get('effective_thermal_conductivity'),
+                                       pr.get('heat_source')))
+        f = pr.get('fourier')
+        t0.append(np.zeros(geo.n_elems) if f is None else
+                  np.asarray(f.variables['Temperature_Fourier']['data'], np.float64))
+    batch = BTEBatch(ops, cfg=config)
+    batch.solve(t0, bte_max_iter=bte_max_iter, bte_rtol=bte_rtol)
+    return [batch.results(p) for p in range(len(problems))]

@@ -67,7 +67,7 @@ class BTEDevice:
     material.  Reusable across solves (warm state is reset by ``reset``)."""
 
     def __init__(self, op: HostOperator, sched: Optional[Schedule] = None, cfg: Optional[SweepConfig] = None,
-                 device=None):
+                 device=None, n_active=None, norm2=None):
         self.op = op
         self.cfg = cfg or SweepConfig()
         self.dev = device or _dev()
@@ -153,9 +153,10 @@ class BTEDevice:
         g['k'] = torch.zeros(L * nch, dtype=i32, device=dev)
         g['active'] = torch.zeros(L * nch, dtype=i32, device=dev)
         g['lane_active'] = torch.zeros(L * nch * mc, dtype=i32, device=dev)
-        g['n_active'] = torch.zeros(1, dtype=i32, device=dev)
+        # counters may live in a batch-wide array (one readback per iteration)
+        g['n_active'] = n_active if n_active is not None else torch.zeros(1, dtype=i32, device=dev)
         g['rnorm'] = torch.zeros(M * L, dtype=f64, device=dev)
-        self.norm2 = torch.zeros(1, dtype=f64, device=dev)
+        self.norm2 = norm2 if norm2 is not None else torch.zeros(1, dtype=f64, device=dev)
         self.cur = 0
         self.plan = self._make_plan()
         self.gcr = self._make_gcr()
@@ -201,7 +202,7 @@ class BTEDevice:
             self.buf['tb'][0][:len(self.op.i_diffuse)].copy_(d0[torch.as_tensor(self.op.i_diffuse).to(self.dev)])
         self.plan.delta_t = self.buf['dt'][0].data_ptr()
         self.plan.tb = self.buf['tb'][0].data_ptr()
-        _lib.call('obte_gcr_reset', self.plan, self.gcr, self.buf['dt'][0].data_ptr(), self.stream)
+        _lib.call('obte_gcr_reset', self.plan, self.gcr, 1, self.stream)
 
     def _n_active(self) -> int:
         return int(self.gbuf['n_active'].item())

@@ -38,10 +38,53 @@ __device__ __forceinline__ double lane_group_sum(double v) {
 }
 
 // ---------------------------------------------------------------------------
-// (a) face-flux assembly: D_l, inflow coefficients, perturbation
-//     (openbte/bte.py:47-124), one thread per (direction, position).
+// Problem sources.  Every kernel is written once against an accessor that
+// yields the plan / Krylov state / IO pointers of problem `pid`:
+//   One  -- a single problem whose structs are kernel arguments (the
+//           obte_bte_* / obte_gcr_* entry points);
+//   Many -- a batch of problems whose structs live in device arrays (the
+//           obte_batch_* entry points); `live` freezes finished problems.
 // ---------------------------------------------------------------------------
-__global__ void k_assemble(obte_bte_plan_t P) {
+struct One {
+  obte_bte_plan_t P;
+  obte_gcr_t G;
+  obte_sweep_io_t io;
+  obte_reduce_io_t rio;
+  __device__ const obte_bte_plan_t& plan(int) const { return P; }
+  __device__ const obte_gcr_t& gcr(int) const { return G; }
+  __device__ const obte_sweep_io_t& sio(int) const { return io; }
+  __device__ const obte_reduce_io_t& red(int) const { return rio; }
+  __device__ bool live(int) const { return true; }
+  __device__ int pid_of(int) const { return 0; }
+  __device__ int local_of(int t) const { return t; }
+};
+
+struct Many {
+  const obte_bte_plan_t* P;
+  const obte_gcr_t* G;
+  const obte_sweep_io_t* io;
+  const obte_reduce_io_t* rio;
+  const int32_t* livef;
+  const int32_t* tpid;
+  const int32_t* tloc;
+  __device__ const obte_bte_plan_t& plan(int p) const { return P[p]; }
+  __device__ const obte_gcr_t& gcr(int p) const { return G[p]; }
+  __device__ const obte_sweep_io_t& sio(int p) const { return io[p]; }
+  __device__ const obte_reduce_io_t& red(int p) const { return rio[p]; }
+  __device__ bool live(int p) const { return livef[p] != 0; }
+  __device__ int pid_of(int t) const { return tpid[t]; }
+  __device__ int local_of(int t) const { return tloc[t]; }
+};
+
+// ---------------------------------------------------------------------------
+// (a) face-flux assembly: D_l, inflow coefficients, perturbation
+//     (openbte/bte.py:47-124), one thread per (direction, position);
+//     blockIdx.y = problem.
+// ---------------------------------------------------------------------------
+template <class S>
+__global__ void k_assemble(S src) {
+  const int pid = blockIdx.y;
+  const obte_bte_plan_t& P = src.plan(pid);
   const int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int64_t total = (int64_t)P.n_dirs * P.n;
   if (gid >= total) return;
@@ -81,7 +124,11 @@ __global__ void k_assemble(obte_bte_plan_t P) {
 
 // q = pert + sum over diffuse walls of (-TB_s) min(s . db, 0)   (openbte/bte.py:199-200,214)
 // b0 = DeltaT + heat, both in position order
-__global__ void k_source(obte_bte_plan_t P) {
+template <class S>
+__global__ void k_source(S src) {
+  const int pid = blockIdx.y;
+  if (!src.live(pid)) return;
+  const obte_bte_plan_t& P = src.plan(pid);
   const int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int64_t total = (int64_t)P.n_dirs * P.n;
   if (gid >= total) return;
@@ -103,20 +150,20 @@ __global__ void k_source(obte_bte_plan_t P) {
 // ---------------------------------------------------------------------------
 // (b) level-scheduled upwind sweep, (c) fused mode sums
 //
-// One workgroup per (direction, chunk of MC MFP bins); one lane per cell of
-// the current level, looping over the MC bins of its chunk.  A cell's MC
-// values are one contiguous row of T, so every upwind gather is a run of
-// 16-byte vector loads and the mode sums are plain register accumulations.
+// One workgroup per task = (problem, direction, chunk of MC MFP bins); one
+// lane per cell of the current level, looping over the MC bins of its chunk.
+// A cell's MC values are one contiguous row of T, so every upwind gather is
+// a run of 16-byte vector loads and the mode sums are register accumulations.
 //
-// Software pipelining across levels: before the barrier that ends level k,
-// each lane already loads the source terms, diagonal, inflow slots and
-// coefficients of its cell of level k+1, so after the barrier only the
-// upwind gathers remain on the critical path.  Levels wider than the block
-// are finished with extra (non-prefetched) passes.
+// Software pipelining across levels: at the start of level k each lane
+// loads the source terms, diagonal, inflow slots and coefficients of its
+// cell of level k+1, so their latency overlaps level k's work and after the
+// barrier only the upwind gathers remain on the critical path.  Levels
+// wider than the block are finished with extra (non-prefetched) passes.
 // ---------------------------------------------------------------------------
 template <int KIN>
 struct CellRegs {
-  int p;       // position, -1 if this lane has no cell
+  int p;  // position, -1 if this lane has no cell
   double b0, q, dd;
   int src[KIN];
   double cf[KIN];
@@ -201,31 +248,21 @@ __device__ __forceinline__ void solve_cell(const CellRegs<KIN>& R, double* __res
   }
 }
 
-// Blocks are dealt to the 8 XCDs round-robin; give all chunks of one
-// direction the same XCD so they share its L2 for the direction's operator.
-__device__ __forceinline__ int task_of_block(int b, int ntask) {
-  constexpr int kXcd = 8;
-  const int per = (ntask + kXcd - 1) / kXcd;
-  const int xcd = b % kXcd;
-  const int idx = b / kXcd;
-  const int t = xcd * per + idx;
-  return t < ntask ? t : -1;
-}
+template <int MC>
+struct SweepShared {
+  double red[kMaxBlock];
+  double lam[MC], tco[MC], sig[3 * MC];
+  int act[(MC + kSub - 1) / kSub];
+};
 
-template <int MC, int MODE, int KIN>
-__global__ __launch_bounds__(kMaxBlock) void k_sweep(obte_bte_plan_t P, const double* __restrict__ y,
-                                                     double* __restrict__ x, const int32_t* __restrict__ active,
-                                                     const int32_t* __restrict__ lane_active) {
-  __shared__ double red[kMaxBlock];
-  __shared__ double s_lam[MC], s_tco[MC], s_sig[3 * MC];
-  __shared__ int s_act[(MC + kSub - 1) / kSub];
-  const int ntask = P.n_dirs * P.n_chunks;
-  const int task = task_of_block(blockIdx.x, ntask);
-  if (task < 0) return;
-  if (active != nullptr && active[task] == 0) return;  // uniform over the block
+// One (problem pid, task lt = l * n_chunks + chunk) sweep by the whole block.
+template <class S, int MC, int MODE, int KIN>
+__device__ void sweep_task(const S& src, int pid, int lt, SweepShared<MC>& sh) {
+  const obte_bte_plan_t& P = src.plan(pid);
+  const obte_sweep_io_t& io = src.sio(pid);
   const int L = P.n_dirs;
-  const int l = task / P.n_chunks;
-  const int chunk = task - l * P.n_chunks;
+  const int l = lt / P.n_chunks;
+  const int chunk = lt - l * P.n_chunks;
   const int n = P.n;
   const int dim = P.dim;
   const int tid = threadIdx.x;
@@ -234,21 +271,21 @@ __global__ __launch_bounds__(kMaxBlock) void k_sweep(obte_bte_plan_t P, const do
   if (tid < MC) {
     const int mg = chunk * MC + tid;
     const bool valid = mg < P.n_mfp;
-    s_lam[tid] = valid ? P.mfps[mg] : 0.0;
-    s_tco[tid] = (MODE == 2 && valid) ? P.t_coeff[(int64_t)mg * L + l] : 0.0;
+    sh.lam[tid] = valid ? P.mfps[mg] : 0.0;
+    sh.tco[tid] = (MODE == 2 && valid) ? P.t_coeff[(int64_t)mg * L + l] : 0.0;
     for (int d = 0; d < 3; ++d)
-      s_sig[3 * tid + d] = (MODE == 2 && valid && d < dim) ? P.sigma[((int64_t)mg * L + l) * dim + d] : 0.0;
+      sh.sig[3 * tid + d] = (MODE == 2 && valid && d < dim) ? P.sigma[((int64_t)mg * L + l) * dim + d] : 0.0;
   }
   if (tid < (MC + kSub - 1) / kSub) {
     int a = 1;
-    if (MODE == 1 && lane_active != nullptr) {
+    if (MODE == 1 && io.lane_active != nullptr) {
       a = 0;
-      for (int j = tid * kSub; j < (tid + 1) * kSub && j < MC; ++j) a |= lane_active[(int64_t)task * MC + j];
+      for (int j = tid * kSub; j < (tid + 1) * kSub && j < MC; ++j) a |= io.lane_active[(int64_t)lt * MC + j];
     }
-    s_act[tid] = a;
+    sh.act[tid] = a;
   }
 
-  double* __restrict__ T = P.t + (int64_t)task * ((int64_t)n + P.ncut_max) * MC;
+  double* __restrict__ T = P.t + (int64_t)lt * ((int64_t)n + P.ncut_max) * MC;
   const int32_t* __restrict__ lvl = P.lvl_ptr + P.lvl_off[l];
   const int nlev = P.lvl_off[l + 1] - P.lvl_off[l] - 1;
   const int cbase = P.cut_off[l];
@@ -263,7 +300,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_sweep(obte_bte_plan_t P, const do
 
   // interface inflows of the cut faces -> rows n .. n + ncut of T
   if (ncut > 0) {
-    const double* __restrict__ yv = y + ((int64_t)chunk * P.n_cut + cbase) * MC;
+    const double* __restrict__ yv = io.y + ((int64_t)chunk * P.n_cut + cbase) * MC;
     double* __restrict__ ti = T + (int64_t)n * MC;
     for (int i = tid; i < ncut * MC; i += nthr) ti[i] = yv[i];
   }
@@ -284,11 +321,11 @@ __global__ __launch_bounds__(kMaxBlock) void k_sweep(obte_bte_plan_t P, const do
       const CellRegs<KIN>& C = base == p0 ? R : Rx;
       if (C.p >= 0) {
         double pt = 0.0, pj0 = 0.0, pj1 = 0.0, pj2 = 0.0;
-        solve_cell<MC, MODE, KIN>(C, T, s_lam, s_act, s_tco, s_sig, dim, pt, pj0, pj1, pj2);
+        solve_cell<MC, MODE, KIN>(C, T, sh.lam, sh.act, sh.tco, sh.sig, dim, pt, pj0, pj1, pj2);
         if (MODE == 2) {
           // partial mode sums in sweep order (coalesced); the reduction maps
           // positions back to cells
-          const int64_t o = (int64_t)task * n + C.p;
+          const int64_t o = (int64_t)lt * n + C.p;
           P.part_t[o] = pt;
           P.part_j[o * dim + 0] = pj0;
           P.part_j[o * dim + 1] = pj1;
@@ -301,8 +338,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_sweep(obte_bte_plan_t P, const do
   }
 
   // exit values of the cut faces (upstream cells across the cut)
-  if (x != nullptr) {
-    double* __restrict__ xv = x + ((int64_t)chunk * P.n_cut + cbase) * MC;
+  if (io.x != nullptr) {
+    double* __restrict__ xv = io.x + ((int64_t)chunk * P.n_cut + cbase) * MC;
     const int32_t* __restrict__ ex = P.cut_exit + cbase;
     for (int i = tid; i < ncut * MC; i += nthr) {
       const int f = i / MC;
@@ -320,18 +357,18 @@ __global__ __launch_bounds__(kMaxBlock) void k_sweep(obte_bte_plan_t P, const do
       double acc = 0.0;
 #pragma unroll 1
       for (int m = 0; m < MC; ++m) {
-        double gb = s_sig[3 * m] * db[0] + s_sig[3 * m + 1] * db[1];
-        if (dim == 3) gb += s_sig[3 * m + 2] * db[2];
+        double gb = sh.sig[3 * m] * db[0] + sh.sig[3 * m + 1] * db[1];
+        if (dim == 3) gb += sh.sig[3 * m + 2] * db[2];
         acc += -row[m] * ((gb > 0.0 ? gb : 0.0) * rinv);
       }
-      P.part_tb[(int64_t)task * P.n_wall + sd] = acc;
+      P.part_tb[(int64_t)lt * P.n_wall + sd] = acc;
     }
     // kappa and suppression of each mode (openbte/bte.py:232-243)
     if (P.contact_kind != 0) {
 #pragma unroll 1
       for (int m = 0; m < MC; ++m) {
         double kap = 0.0, sup = 0.0;
-        const double s0 = s_sig[3 * m], s1 = s_sig[3 * m + 1], s2 = s_sig[3 * m + 2];
+        const double s0 = sh.sig[3 * m], s1 = sh.sig[3 * m + 1], s2 = sh.sig[3 * m + 2];
         for (int sc = tid; sc < P.n_contact; sc += nthr) {
           const double* na = P.c_na + (int64_t)sc * dim;
           double sn = s0 * na[0] + s1 * na[1];
@@ -348,21 +385,21 @@ __global__ __launch_bounds__(kMaxBlock) void k_sweep(obte_bte_plan_t P, const do
           sup += t1 * sqrt(area2);
         }
         // fixed-order block reductions
-        red[tid] = kap;
+        sh.red[tid] = kap;
         __syncthreads();
         for (int o = nthr / 2; o > 0; o >>= 1) {
-          if (tid < o) red[tid] += red[tid + o];
+          if (tid < o) sh.red[tid] += sh.red[tid + o];
           __syncthreads();
         }
-        const double ksum = red[0];
+        const double ksum = sh.red[0];
         __syncthreads();
-        red[tid] = sup;
+        sh.red[tid] = sup;
         __syncthreads();
         for (int o = nthr / 2; o > 0; o >>= 1) {
-          if (tid < o) red[tid] += red[tid + o];
+          if (tid < o) sh.red[tid] += sh.red[tid + o];
           __syncthreads();
         }
-        const double ssum = red[0];
+        const double ssum = sh.red[0];
         __syncthreads();
         const int mg = chunk * MC + m;
         if (tid == 0 && mg < P.n_mfp) {
@@ -372,6 +409,68 @@ __global__ __launch_bounds__(kMaxBlock) void k_sweep(obte_bte_plan_t P, const do
       }
     }
   }
+  __syncthreads();  // shared state is reused by the next task of this block
+}
+
+// Blocks are dealt to the 8 XCDs round-robin; give all chunks of one
+// direction the same XCD so they share its L2 for the direction's operator.
+__device__ __forceinline__ int task_of_block(int b, int ntask) {
+  constexpr int kXcd = 8;
+  const int per = (ntask + kXcd - 1) / kXcd;
+  const int xcd = b % kXcd;
+  const int idx = b / kXcd;
+  const int t = xcd * per + idx;
+  return t < ntask ? t : -1;
+}
+
+// single problem: one task per block
+template <int MC, int MODE, int KIN>
+__global__ __launch_bounds__(kMaxBlock) void k_sweep(One src, int ntask) {
+  __shared__ SweepShared<MC> sh;
+  const int t = task_of_block(blockIdx.x, ntask);
+  if (t < 0) return;
+  const obte_sweep_io_t& io = src.io;
+  if (io.active != nullptr && io.active[t] == 0) return;  // uniform over the block
+  sweep_task<One, MC, MODE, KIN>(src, 0, t, sh);
+}
+
+// batch: persistent workgroups draining 8 XCD-local task queues (tasks of one
+// (problem, direction) share a queue; a drained queue steals from the next)
+template <int MC, int MODE, int KIN>
+__global__ __launch_bounds__(kMaxBlock) void k_sweep_batch(Many src, const int32_t* __restrict__ qoff,
+                                                           int32_t* __restrict__ qcnt) {
+  __shared__ SweepShared<MC> sh;
+  __shared__ int s_task;
+  const int xcd = blockIdx.x % 8;
+  int q = 0;
+  for (;;) {
+    if (threadIdx.x == 0) {
+      int t = -1;
+      for (; q < 8; ++q) {
+        const int qq = (xcd + q) & 7;
+        const int i = atomicAdd(&qcnt[qq], 1) + qoff[qq];
+        if (i < qoff[qq + 1]) {
+          t = i;
+          break;
+        }
+      }
+      s_task = t;
+    }
+    __syncthreads();
+    const int t = s_task;
+    __syncthreads();
+    if (t < 0) return;
+    const int pid = src.pid_of(t);
+    const int lt = src.local_of(t);
+    if (!src.live(pid)) continue;
+    const obte_sweep_io_t& io = src.sio(pid);
+    if (io.active != nullptr && io.active[lt] == 0) continue;
+    sweep_task<Many, MC, MODE, KIN>(src, pid, lt, sh);
+  }
+}
+
+__global__ void k_zero_queues(int32_t* q) {
+  if (threadIdx.x < 8) q[threadIdx.x] = 0;
 }
 
 // Block-wide per-lane sum over the slots (fixed order); result broadcast.
@@ -425,10 +524,11 @@ __device__ __forceinline__ GcrView gcr_view(const obte_bte_plan_t& P, int l, int
   return v;
 }
 
-// Convergence test of one lane; updates flags.  Returns 1 if this lane stays active.
+// Convergence test of one lane; updates flags.
 template <int MC>
-__device__ int gcr_flags(const obte_bte_plan_t& P, const obte_gcr_t& G, int task, int l, int chunk,
-                         const GcrView& v, double rn2, double ymax, double* red, double* bc) {
+__device__ void gcr_flags(const obte_bte_plan_t& P, const obte_gcr_t& G, int task, int l, int chunk,
+                          const GcrView& v, double rn2, double ymax, double* red, double* bc) {
+  __shared__ int blk_any;
   const int lane = threadIdx.x % MC;
   const int slot = threadIdx.x / MC;
   const int mg = chunk * MC + lane;
@@ -436,9 +536,7 @@ __device__ int gcr_flags(const obte_bte_plan_t& P, const obte_gcr_t& G, int task
   const double rn = sqrt(rn2);
   const double tol = G.rtol * fmax(1.0, ymax) * sqrt((double)(v.ncut > 0 ? v.ncut : 1));
   const int act = (valid && v.ncut > 0 && rn > tol) ? 1 : 0;
-  // any active lane in the block?
   const double any = block_lane_max<MC>(act ? 1.0 : 0.0, red, bc);
-  __shared__ int blk_any;
   if (threadIdx.x == 0) blk_any = 0;
   __syncthreads();
   if (slot == 0) {
@@ -451,15 +549,18 @@ __device__ int gcr_flags(const obte_bte_plan_t& P, const obte_gcr_t& G, int task
     G.active[task] = blk_any;
     if (blk_any) atomicAdd(G.n_active, 1);
   }
-  return act;
 }
 
 // r = x - y; MGS projection of r on the recycled basis; y += U a.
-template <int MC>
-__global__ __launch_bounds__(kMaxBlock) void k_gcr_project(obte_bte_plan_t P, obte_gcr_t G) {
+template <class S, int MC>
+__global__ __launch_bounds__(kMaxBlock) void k_gcr_project(S src) {
   __shared__ double red[kMaxBlock];
   __shared__ double bc[64];
-  const int task = blockIdx.x;
+  const int pid = src.pid_of(blockIdx.x);
+  if (!src.live(pid)) return;
+  const int task = src.local_of(blockIdx.x);
+  const obte_bte_plan_t& P = src.plan(pid);
+  const obte_gcr_t& G = src.gcr(pid);
   const int l = task / P.n_chunks;
   const int chunk = task - l * P.n_chunks;
   const int lane = threadIdx.x % MC;
@@ -501,9 +602,13 @@ __global__ __launch_bounds__(kMaxBlock) void k_gcr_project(obte_bte_plan_t P, ob
 }
 
 // w = r on unconverged lanes, 0 elsewhere (inflow of the transfer sweep).
-template <int MC>
-__global__ __launch_bounds__(kMaxBlock) void k_gcr_begin(obte_bte_plan_t P, obte_gcr_t G) {
-  const int task = blockIdx.x;
+template <class S, int MC>
+__global__ __launch_bounds__(kMaxBlock) void k_gcr_begin(S src) {
+  const int pid = src.pid_of(blockIdx.x);
+  if (!src.live(pid)) return;
+  const int task = src.local_of(blockIdx.x);
+  const obte_bte_plan_t& P = src.plan(pid);
+  const obte_gcr_t& G = src.gcr(pid);
   if (G.active[task] == 0) return;
   const int l = task / P.n_chunks;
   const int chunk = task - l * P.n_chunks;
@@ -522,11 +627,15 @@ __global__ __launch_bounds__(kMaxBlock) void k_gcr_begin(obte_bte_plan_t P, obte
 
 // c = w - x (= (I - T_op) w); orthogonalise against the basis, normalise,
 // append, then y += a u, r -= a c with a = c . r.
-template <int MC>
-__global__ __launch_bounds__(kMaxBlock) void k_gcr_end(obte_bte_plan_t P, obte_gcr_t G) {
+template <class S, int MC>
+__global__ __launch_bounds__(kMaxBlock) void k_gcr_end(S src) {
   __shared__ double red[kMaxBlock];
   __shared__ double bc[64];
-  const int task = blockIdx.x;
+  const int pid = src.pid_of(blockIdx.x);
+  if (!src.live(pid)) return;
+  const int task = src.local_of(blockIdx.x);
+  const obte_bte_plan_t& P = src.plan(pid);
+  const obte_gcr_t& G = src.gcr(pid);
   if (G.active[task] == 0) return;
   const int l = task / P.n_chunks;
   const int chunk = task - l * P.n_chunks;
@@ -596,12 +705,22 @@ __global__ __launch_bounds__(kMaxBlock) void k_gcr_end(obte_bte_plan_t P, obte_g
   gcr_flags<MC>(P, G, task, l, chunk, v, rn2, ymax, red, bc);
 }
 
-__global__ void k_zero_int(int32_t* p) { *p = 0; }
+// zero the active-task counters of every problem (gridDim.x = problems)
+template <class S>
+__global__ void k_zero_active(S src) {
+  if (threadIdx.x == 0) *src.gcr(blockIdx.x).n_active = 0;
+}
 
-// y = initial lattice temperature at each cut's exit cell (or 0); basis cleared.
-// One block per direction.
-__global__ void k_gcr_reset(obte_bte_plan_t P, obte_gcr_t G, const double* __restrict__ dt0) {
+// y = initial lattice temperature at each cut's exit cell (or 0); basis
+// cleared.  One block per (direction, problem).
+template <class S>
+__global__ void k_gcr_reset(S src, int use_dt) {
+  const int pid = blockIdx.y;
+  if (!src.live(pid)) return;
+  const obte_bte_plan_t& P = src.plan(pid);
+  const obte_gcr_t& G = src.gcr(pid);
   const int l = blockIdx.x;
+  if (l >= P.n_dirs) return;
   const int cb = P.cut_off[l];
   const int ncut = P.cut_off[l + 1] - cb;
   const int mc = P.mc;
@@ -612,7 +731,7 @@ __global__ void k_gcr_reset(obte_bte_plan_t P, obte_gcr_t G, const double* __res
     const int f = (int)(r / mc);
     const int m = (int)(r - (int64_t)f * mc);
     double v = 0.0;
-    if (dt0 != nullptr) v = dt0[P.order[(int64_t)l * P.n + P.cut_exit[cb + f]]];
+    if (use_dt) v = P.delta_t[P.order[(int64_t)l * P.n + P.cut_exit[cb + f]]];
     G.y[((int64_t)chunk * P.n_cut + cb + f) * mc + m] = v;
   }
   for (int c = threadIdx.x; c < P.n_chunks; c += blockDim.x) G.k[(int64_t)l * P.n_chunks + c] = 0;
@@ -620,7 +739,11 @@ __global__ void k_gcr_reset(obte_bte_plan_t P, obte_gcr_t G, const double* __res
 
 // Stage 1: sum the chunks of each direction in sweep order (coalesced),
 // in place into chunk 0 of the direction.
-__global__ void k_reduce_chunks(obte_bte_plan_t P) {
+template <class S>
+__global__ void k_reduce_chunks(S src) {
+  const int pid = blockIdx.y;
+  if (!src.live(pid)) return;
+  const obte_bte_plan_t& P = src.plan(pid);
   const int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int64_t total = (int64_t)P.n_dirs * P.n;
   if (gid >= total || P.n_chunks == 1) return;
@@ -645,8 +768,14 @@ __global__ void k_reduce_chunks(obte_bte_plan_t P) {
 
 // Stage 2: fixed-order sum over directions, gathering each direction's
 // partial at the cell's position; one thread per cell.
-__global__ void k_reduce_cells(obte_bte_plan_t P, double* dt, double* jn, double* blk) {
+template <class S>
+__global__ void k_reduce_cells(S src) {
   __shared__ double red[256];
+  const int pid = blockIdx.y;
+  if (!src.live(pid)) return;
+  const obte_bte_plan_t& P = src.plan(pid);
+  const obte_reduce_io_t& R = src.red(pid);
+  if ((int64_t)blockIdx.x * blockDim.x >= P.n) return;  // uniform
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   const int dim = P.dim;
   double s = 0.0;
@@ -659,10 +788,10 @@ __global__ void k_reduce_cells(obte_bte_plan_t P, double* dt, double* jn, double
       j1 += P.part_j[o * dim + 1];
       if (dim == 3) j2 += P.part_j[o * dim + 2];
     }
-    dt[c] = s;
-    jn[(int64_t)c * dim + 0] = j0;
-    jn[(int64_t)c * dim + 1] = j1;
-    if (dim == 3) jn[(int64_t)c * dim + 2] = j2;
+    R.dt_new[c] = s;
+    R.j_new[(int64_t)c * dim + 0] = j0;
+    R.j_new[(int64_t)c * dim + 1] = j1;
+    if (dim == 3) R.j_new[(int64_t)c * dim + 2] = j2;
   }
   red[threadIdx.x] = s * s;
   __syncthreads();
@@ -670,29 +799,51 @@ __global__ void k_reduce_cells(obte_bte_plan_t P, double* dt, double* jn, double
     if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
     __syncthreads();
   }
-  if (threadIdx.x == 0) blk[blockIdx.x] = red[0];
+  if (threadIdx.x == 0) P.scratch[blockIdx.x] = red[0];
 }
 
-__global__ void k_reduce_walls(obte_bte_plan_t P, double* tb) {
+template <class S>
+__global__ void k_reduce_walls(S src) {
+  const int pid = blockIdx.y;
+  if (!src.live(pid)) return;
+  const obte_bte_plan_t& P = src.plan(pid);
+  const obte_reduce_io_t& R = src.red(pid);
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= P.n_wall) return;
   const int ntask = P.n_dirs * P.n_chunks;
   double v = 0.0;
   for (int t = 0; t < ntask; ++t) v += P.part_tb[(int64_t)t * P.n_wall + s];
-  tb[s] = v;
+  R.tb_new[s] = v;
 }
 
-__global__ void k_reduce_final(const double* blk, int nblk, double* out) {
+// norm2 = sum of the block partials in order; one block per problem
+template <class S>
+__global__ void k_reduce_final(S src) {
   __shared__ double red[1024];
+  const int pid = blockIdx.x;
+  if (!src.live(pid)) return;
+  const obte_bte_plan_t& P = src.plan(pid);
+  const int nblk = (P.n + 255) / 256;
   double s = 0.0;
-  for (int i = threadIdx.x; i < nblk; i += blockDim.x) s += blk[i];
+  for (int i = threadIdx.x; i < nblk; i += blockDim.x) s += P.scratch[i];
   red[threadIdx.x] = s;
   __syncthreads();
   for (int o = blockDim.x / 2; o > 0; o >>= 1) {
     if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
     __syncthreads();
   }
-  if (threadIdx.x == 0) *out = red[0];
+  if (threadIdx.x == 0) *src.red(pid).norm2 = red[0];
+}
+
+// batch: DeltaT, TB <- the reduced values (pointers of the plans stay fixed)
+__global__ void k_commit(Many src) {
+  const int pid = blockIdx.y;
+  if (!src.live(pid)) return;
+  const obte_bte_plan_t& P = src.plan(pid);
+  const obte_reduce_io_t& R = src.red(pid);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < P.n) const_cast<double*>(P.delta_t)[i] = R.dt_new[i];
+  if (i < P.n_wall) const_cast<double*>(P.tb)[i] = R.tb_new[i];
 }
 
 int check_plan(const obte_bte_plan_t* P, const char* who) {
@@ -706,6 +857,7 @@ int check_plan(const obte_bte_plan_t* P, const char* who) {
     return obte_fail(OBTE_EINVAL, "plan: block must be a power of two in [64, 1024]");
   if (P->kin != 2 && P->kin != 3) return obte_fail(OBTE_EINVAL, "plan: kin must be 2 or 3");
   if (P->ncut_max < 0) return obte_fail(OBTE_EINVAL, "plan: ncut_max < 0");
+  if ((P->n + 255) / 256 > 4096) return obte_fail(OBTE_ELIMIT, "plan: more than 4096*256 cells (scratch size)");
   return OBTE_OK;
 }
 
@@ -715,45 +867,51 @@ int launch_status(const char* who) {
   return OBTE_OK;
 }
 
-template <int MC, int KIN>
-void launch_sweep_k(const obte_bte_plan_t& P, int mode, const double* y, double* x, const int32_t* active,
-                    const int32_t* lane_active, hipStream_t st) {
-  const int ntask = P.n_dirs * P.n_chunks;
-  const int nblk = ((ntask + 7) / 8) * 8;
-  dim3 grid(nblk), blk(P.block);
-  if (mode == 0) hipLaunchKernelGGL((k_sweep<MC, 0, KIN>), grid, blk, 0, st, P, y, x, active, lane_active);
-  else if (mode == 1) hipLaunchKernelGGL((k_sweep<MC, 1, KIN>), grid, blk, 0, st, P, y, x, active, lane_active);
-  else hipLaunchKernelGGL((k_sweep<MC, 2, KIN>), grid, blk, 0, st, P, y, x, active, lane_active);
-}
+inline unsigned blocks_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
-template <int MC>
-void launch_sweep(const obte_bte_plan_t& P, int mode, const double* y, double* x, const int32_t* active,
-                  const int32_t* lane_active, hipStream_t st) {
-  if (P.kin <= 2) launch_sweep_k<MC, 2>(P, mode, y, x, active, lane_active, st);
-  else launch_sweep_k<MC, 3>(P, mode, y, x, active, lane_active, st);
-}
-
-template <template <int> class F, typename... A>
-void dispatch_mc(int mc, A&&... a) {
+template <template <int, int> class F, typename... A>
+void dispatch(int mc, int kin, A&&... a) {
+  const bool k3 = kin == 3;
   switch (mc) {
-    case 4: F<4>::run(a...); break;
-    case 8: F<8>::run(a...); break;
-    case 16: F<16>::run(a...); break;
-    case 32: F<32>::run(a...); break;
-    default: F<64>::run(a...); break;
+    case 4: k3 ? F<4, 3>::run(a...) : F<4, 2>::run(a...); break;
+    case 8: k3 ? F<8, 3>::run(a...) : F<8, 2>::run(a...); break;
+    case 16: k3 ? F<16, 3>::run(a...) : F<16, 2>::run(a...); break;
+    case 32: k3 ? F<32, 3>::run(a...) : F<32, 2>::run(a...); break;
+    default: k3 ? F<64, 3>::run(a...) : F<64, 2>::run(a...); break;
   }
 }
 
-template <int MC>
-struct SweepL { static void run(const obte_bte_plan_t& P, int mode, const double* y, double* x, const int32_t* a, const int32_t* la, hipStream_t st) { launch_sweep<MC>(P, mode, y, x, a, la, st); } };
-template <int MC>
-struct ProjL { static void run(const obte_bte_plan_t& P, const obte_gcr_t& G, hipStream_t st) { hipLaunchKernelGGL((k_gcr_project<MC>), dim3(P.n_dirs * P.n_chunks), dim3(P.block), 0, st, P, G); } };
-template <int MC>
-struct BeginL { static void run(const obte_bte_plan_t& P, const obte_gcr_t& G, hipStream_t st) { hipLaunchKernelGGL((k_gcr_begin<MC>), dim3(P.n_dirs * P.n_chunks), dim3(P.block), 0, st, P, G); } };
-template <int MC>
-struct EndL { static void run(const obte_bte_plan_t& P, const obte_gcr_t& G, hipStream_t st) { hipLaunchKernelGGL((k_gcr_end<MC>), dim3(P.n_dirs * P.n_chunks), dim3(P.block), 0, st, P, G); } };
+template <int MC, int KIN>
+struct SweepOne {
+  static void run(const One& src, int mode, int ntask, int block, hipStream_t st) {
+    dim3 grid(((ntask + 7) / 8) * 8), blk(block);
+    if (mode == 0) hipLaunchKernelGGL((k_sweep<MC, 0, KIN>), grid, blk, 0, st, src, ntask);
+    else if (mode == 1) hipLaunchKernelGGL((k_sweep<MC, 1, KIN>), grid, blk, 0, st, src, ntask);
+    else hipLaunchKernelGGL((k_sweep<MC, 2, KIN>), grid, blk, 0, st, src, ntask);
+  }
+};
 
-inline unsigned blocks_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+template <int MC, int KIN>
+struct SweepMany {
+  static void run(const Many& src, int mode, int groups, int block, const int32_t* qoff, int32_t* qcnt,
+                  hipStream_t st) {
+    dim3 grid(groups), blk(block);
+    if (mode == 0) hipLaunchKernelGGL((k_sweep_batch<MC, 0, KIN>), grid, blk, 0, st, src, qoff, qcnt);
+    else if (mode == 1) hipLaunchKernelGGL((k_sweep_batch<MC, 1, KIN>), grid, blk, 0, st, src, qoff, qcnt);
+    else hipLaunchKernelGGL((k_sweep_batch<MC, 2, KIN>), grid, blk, 0, st, src, qoff, qcnt);
+  }
+};
+
+// GCR kernels: op 0 project, 1 begin, 2 end
+template <int MC, int KIN>
+struct GcrL {
+  template <class S>
+  static void run(const S& src, int op, int nblocks, int block, hipStream_t st) {
+    if (op == 0) hipLaunchKernelGGL((k_gcr_project<S, MC>), dim3(nblocks), dim3(block), 0, st, src);
+    else if (op == 1) hipLaunchKernelGGL((k_gcr_begin<S, MC>), dim3(nblocks), dim3(block), 0, st, src);
+    else hipLaunchKernelGGL((k_gcr_end<S, MC>), dim3(nblocks), dim3(block), 0, st, src);
+  }
+};
 
 int check_gcr(const obte_gcr_t* G) {
   if (!G || !G->y || !G->r || !G->x || !G->u || !G->c || !G->k || !G->active || !G->lane_active || !G->n_active ||
@@ -762,15 +920,51 @@ int check_gcr(const obte_gcr_t* G) {
   return OBTE_OK;
 }
 
+One make_one(const obte_bte_plan_t* plan, const obte_gcr_t* g) {
+  One o{};
+  o.P = *plan;
+  if (g) o.G = *g;
+  return o;
+}
+
+int check_batch(const obte_bte_batch_t* B, const char* who) {
+  if (!B || !B->plans || !B->gcrs || !B->io || !B->rio || !B->live || !B->task_pid || !B->task_local ||
+      !B->queue_off || !B->queue_cnt || B->n_problems <= 0 || B->n_tasks <= 0)
+    return obte_fail(OBTE_EINVAL, who);
+  if (B->mc != 4 && B->mc != 8 && B->mc != 16 && B->mc != 32 && B->mc != 64)
+    return obte_fail(OBTE_EINVAL, "batch: mc must be 4, 8, 16, 32 or 64");
+  if (B->block < 64 || B->block > kMaxBlock || (B->block & (B->block - 1)) != 0)
+    return obte_fail(OBTE_EINVAL, "batch: block must be a power of two in [64, 1024]");
+  if (B->kin != 2 && B->kin != 3) return obte_fail(OBTE_EINVAL, "batch: kin must be 2 or 3");
+  if (B->groups <= 0 || B->max_elems <= 0 || B->max_dirs <= 0 || B->max_cells <= 0)
+    return obte_fail(OBTE_EINVAL, "batch: invalid launch sizes");
+  return OBTE_OK;
+}
+
+Many make_many(const obte_bte_batch_t* B, int mode) {
+  Many m{};
+  m.P = B->plans;
+  m.G = B->gcrs;
+  m.io = B->io + (int64_t)mode * B->n_problems;
+  m.rio = B->rio;
+  m.livef = B->live;
+  m.tpid = B->task_pid;
+  m.tloc = B->task_local;
+  return m;
+}
+
 }  // namespace
 
 extern "C" {
+
+// ---- single problem -------------------------------------------------------
 
 int obte_bte_assemble(const obte_bte_plan_t* plan, obte_stream_t stream) {
   int rc = check_plan(plan, "obte_bte_assemble: null plan");
   if (rc) return rc;
   const int64_t total = (int64_t)plan->n_dirs * plan->n;
-  hipLaunchKernelGGL(k_assemble, dim3(blocks_for(total, 256)), dim3(256), 0, (hipStream_t)stream, *plan);
+  hipLaunchKernelGGL((k_assemble<One>), dim3(blocks_for(total, 256), 1), dim3(256), 0, (hipStream_t)stream,
+                     make_one(plan, nullptr));
   return launch_status("obte_bte_assemble: launch failed");
 }
 
@@ -778,7 +972,8 @@ int obte_bte_source(const obte_bte_plan_t* plan, obte_stream_t stream) {
   int rc = check_plan(plan, "obte_bte_source: null plan");
   if (rc) return rc;
   const int64_t total = (int64_t)plan->n_dirs * plan->n;
-  hipLaunchKernelGGL(k_source, dim3(blocks_for(total, 256)), dim3(256), 0, (hipStream_t)stream, *plan);
+  hipLaunchKernelGGL((k_source<One>), dim3(blocks_for(total, 256), 1), dim3(256), 0, (hipStream_t)stream,
+                     make_one(plan, nullptr));
   return launch_status("obte_bte_source: launch failed");
 }
 
@@ -788,37 +983,43 @@ int obte_bte_sweep(const obte_bte_plan_t* plan, int32_t mode, const double* y, d
   if (rc) return rc;
   if (mode < 0 || mode > 2) return obte_fail(OBTE_EINVAL, "obte_bte_sweep: mode must be 0, 1 or 2");
   if (plan->n_cut > 0 && !y) return obte_fail(OBTE_EINVAL, "obte_bte_sweep: y is required when cuts exist");
-  dispatch_mc<SweepL>(plan->mc, *plan, (int)mode, y, x, active, lane_active, (hipStream_t)stream);
+  One src = make_one(plan, nullptr);
+  src.io.y = y;
+  src.io.x = x;
+  src.io.active = active;
+  src.io.lane_active = lane_active;
+  dispatch<SweepOne>(plan->mc, plan->kin, src, (int)mode, plan->n_dirs * plan->n_chunks, plan->block,
+                     (hipStream_t)stream);
   return launch_status("obte_bte_sweep: launch failed");
 }
 
-int obte_gcr_project(const obte_bte_plan_t* plan, const obte_gcr_t* g, obte_stream_t stream) {
-  int rc = check_plan(plan, "obte_gcr_project: null plan");
+static int gcr_one(const obte_bte_plan_t* plan, const obte_gcr_t* g, int op, obte_stream_t stream, const char* who) {
+  int rc = check_plan(plan, who);
   if (rc || (rc = check_gcr(g))) return rc;
-  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, (hipStream_t)stream, g->n_active);
-  dispatch_mc<ProjL>(plan->mc, *plan, *g, (hipStream_t)stream);
-  return launch_status("obte_gcr_project: launch failed");
+  One src = make_one(plan, g);
+  hipStream_t st = (hipStream_t)stream;
+  if (op != 1) hipLaunchKernelGGL((k_zero_active<One>), dim3(1), dim3(64), 0, st, src);
+  dispatch<GcrL>(plan->mc, 2, src, op, plan->n_dirs * plan->n_chunks, plan->block, st);
+  return launch_status(who);
+}
+
+int obte_gcr_project(const obte_bte_plan_t* plan, const obte_gcr_t* g, obte_stream_t stream) {
+  return gcr_one(plan, g, 0, stream, "obte_gcr_project");
 }
 
 int obte_gcr_begin_extend(const obte_bte_plan_t* plan, const obte_gcr_t* g, obte_stream_t stream) {
-  int rc = check_plan(plan, "obte_gcr_begin_extend: null plan");
-  if (rc || (rc = check_gcr(g))) return rc;
-  dispatch_mc<BeginL>(plan->mc, *plan, *g, (hipStream_t)stream);
-  return launch_status("obte_gcr_begin_extend: launch failed");
+  return gcr_one(plan, g, 1, stream, "obte_gcr_begin_extend");
 }
 
 int obte_gcr_end_extend(const obte_bte_plan_t* plan, const obte_gcr_t* g, obte_stream_t stream) {
-  int rc = check_plan(plan, "obte_gcr_end_extend: null plan");
-  if (rc || (rc = check_gcr(g))) return rc;
-  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, (hipStream_t)stream, g->n_active);
-  dispatch_mc<EndL>(plan->mc, *plan, *g, (hipStream_t)stream);
-  return launch_status("obte_gcr_end_extend: launch failed");
+  return gcr_one(plan, g, 2, stream, "obte_gcr_end_extend");
 }
 
-int obte_gcr_reset(const obte_bte_plan_t* plan, const obte_gcr_t* g, const double* delta_t0, obte_stream_t stream) {
+int obte_gcr_reset(const obte_bte_plan_t* plan, const obte_gcr_t* g, int32_t use_delta_t, obte_stream_t stream) {
   int rc = check_plan(plan, "obte_gcr_reset: null plan");
   if (rc || (rc = check_gcr(g))) return rc;
-  hipLaunchKernelGGL(k_gcr_reset, dim3(plan->n_dirs), dim3(256), 0, (hipStream_t)stream, *plan, *g, delta_t0);
+  hipLaunchKernelGGL((k_gcr_reset<One>), dim3(plan->n_dirs, 1), dim3(256), 0, (hipStream_t)stream,
+                     make_one(plan, g), (int)use_delta_t);
   return launch_status("obte_gcr_reset: launch failed");
 }
 
@@ -827,18 +1028,86 @@ int obte_bte_reduce(const obte_bte_plan_t* plan, double* delta_t_new, double* j_
   int rc = check_plan(plan, "obte_bte_reduce: null plan");
   if (rc) return rc;
   if (!delta_t_new || !j_new || !norm2 || !plan->scratch) return obte_fail(OBTE_EINVAL, "obte_bte_reduce: null output");
-  const unsigned nb = blocks_for(plan->n, 256);
-  if (nb > 4096) return obte_fail(OBTE_ELIMIT, "obte_bte_reduce: more than 4096*256 cells");
+  if (plan->n_wall > 0 && !tb_new) return obte_fail(OBTE_EINVAL, "obte_bte_reduce: null tb output");
   hipStream_t st = (hipStream_t)stream;
+  One src = make_one(plan, nullptr);
+  src.rio.dt_new = delta_t_new;
+  src.rio.j_new = j_new;
+  src.rio.tb_new = tb_new;
+  src.rio.norm2 = norm2;
   const int64_t total = (int64_t)plan->n_dirs * plan->n;
-  hipLaunchKernelGGL(k_reduce_chunks, dim3(blocks_for(total, 256)), dim3(256), 0, st, *plan);
-  hipLaunchKernelGGL(k_reduce_cells, dim3(nb), dim3(256), 0, st, *plan, delta_t_new, j_new, plan->scratch);
-  if (plan->n_wall > 0) {
-    if (!tb_new) return obte_fail(OBTE_EINVAL, "obte_bte_reduce: null tb output");
-    hipLaunchKernelGGL(k_reduce_walls, dim3(blocks_for(plan->n_wall, 256)), dim3(256), 0, st, *plan, tb_new);
-  }
-  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(1024), 0, st, plan->scratch, (int)nb, norm2);
+  hipLaunchKernelGGL((k_reduce_chunks<One>), dim3(blocks_for(total, 256), 1), dim3(256), 0, st, src);
+  hipLaunchKernelGGL((k_reduce_cells<One>), dim3(blocks_for(plan->n, 256), 1), dim3(256), 0, st, src);
+  if (plan->n_wall > 0)
+    hipLaunchKernelGGL((k_reduce_walls<One>), dim3(blocks_for(plan->n_wall, 256), 1), dim3(256), 0, st, src);
+  hipLaunchKernelGGL((k_reduce_final<One>), dim3(1), dim3(1024), 0, st, src);
   return launch_status("obte_bte_reduce: launch failed");
+}
+
+// ---- batch of problems ----------------------------------------------------
+
+int obte_batch_assemble(const obte_bte_batch_t* b, obte_stream_t stream) {
+  int rc = check_batch(b, "obte_batch_assemble: invalid batch");
+  if (rc) return rc;
+  hipLaunchKernelGGL((k_assemble<Many>), dim3(blocks_for(b->max_elems, 256), b->n_problems), dim3(256), 0,
+                     (hipStream_t)stream, make_many(b, 0));
+  return launch_status("obte_batch_assemble: launch failed");
+}
+
+int obte_batch_source(const obte_bte_batch_t* b, obte_stream_t stream) {
+  int rc = check_batch(b, "obte_batch_source: invalid batch");
+  if (rc) return rc;
+  hipLaunchKernelGGL((k_source<Many>), dim3(blocks_for(b->max_elems, 256), b->n_problems), dim3(256), 0,
+                     (hipStream_t)stream, make_many(b, 0));
+  return launch_status("obte_batch_source: launch failed");
+}
+
+int obte_batch_sweep(const obte_bte_batch_t* b, int32_t mode, obte_stream_t stream) {
+  int rc = check_batch(b, "obte_batch_sweep: invalid batch");
+  if (rc) return rc;
+  if (mode < 0 || mode > 2) return obte_fail(OBTE_EINVAL, "obte_batch_sweep: mode must be 0, 1 or 2");
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_zero_queues, dim3(1), dim3(64), 0, st, b->queue_cnt);
+  dispatch<SweepMany>(b->mc, b->kin, make_many(b, mode), (int)mode, b->groups, b->block, b->queue_off, b->queue_cnt,
+                      st);
+  return launch_status("obte_batch_sweep: launch failed");
+}
+
+int obte_batch_gcr(const obte_bte_batch_t* b, int32_t op, obte_stream_t stream) {
+  int rc = check_batch(b, "obte_batch_gcr: invalid batch");
+  if (rc) return rc;
+  if (op < 0 || op > 2) return obte_fail(OBTE_EINVAL, "obte_batch_gcr: op must be 0, 1 or 2");
+  hipStream_t st = (hipStream_t)stream;
+  Many src = make_many(b, 0);
+  if (op != 1) hipLaunchKernelGGL((k_zero_active<Many>), dim3(b->n_problems), dim3(64), 0, st, src);
+  dispatch<GcrL>(b->mc, 2, src, (int)op, b->n_tasks, b->block, st);
+  return launch_status("obte_batch_gcr: launch failed");
+}
+
+int obte_batch_gcr_reset(const obte_bte_batch_t* b, int32_t use_delta_t, obte_stream_t stream) {
+  int rc = check_batch(b, "obte_batch_gcr_reset: invalid batch");
+  if (rc) return rc;
+  hipLaunchKernelGGL((k_gcr_reset<Many>), dim3(b->max_dirs, b->n_problems), dim3(256), 0, (hipStream_t)stream,
+                     make_many(b, 0), (int)use_delta_t);
+  return launch_status("obte_batch_gcr_reset: launch failed");
+}
+
+int obte_batch_reduce(const obte_bte_batch_t* b, obte_stream_t stream) {
+  int rc = check_batch(b, "obte_batch_reduce: invalid batch");
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  Many src = make_many(b, 0);
+  hipLaunchKernelGGL((k_reduce_chunks<Many>), dim3(blocks_for(b->max_elems, 256), b->n_problems), dim3(256), 0, st,
+                     src);
+  hipLaunchKernelGGL((k_reduce_cells<Many>), dim3(blocks_for(b->max_cells, 256), b->n_problems), dim3(256), 0, st,
+                     src);
+  if (b->max_walls > 0)
+    hipLaunchKernelGGL((k_reduce_walls<Many>), dim3(blocks_for(b->max_walls, 256), b->n_problems), dim3(256), 0, st,
+                       src);
+  hipLaunchKernelGGL((k_reduce_final<Many>), dim3(b->n_problems), dim3(1024), 0, st, src);
+  hipLaunchKernelGGL(k_commit, dim3(blocks_for(b->max_cells > b->max_walls ? b->max_cells : b->max_walls, 256),
+                                    b->n_problems), dim3(256), 0, st, src);
+  return launch_status("obte_batch_reduce: launch failed");
 }
 
 }  // extern "C"

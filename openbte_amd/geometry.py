@@ -181,3 +181,48 @@ def structured_tri_cell_2d(L: float = 10.0, N: int = 40) -> Mesh:
                      np.stack([cc, d, c], -1), np.stack([d, a, c], -1)], axis=2).reshape(-1, 3)
     return build_mesh(pts, tris, _edge_region(L, L), _REGIONS,
                       periodic={'Periodic_x': np.array([L, 0.0]), 'Periodic_y': np.array([0.0, L])})
+
+
+def random_pore_cell_2d(seed: int, n_cells: int, h_ref: float = 0.274, max_pores: int = 6,
+                        porosity_range=(0.05, 0.40)) -> Mesh:
+    """Periodic cell with 1..max_pores non-overlapping circular / square pores.
+
+    Total porosity ~ U(porosity_range), split among the pores by a Dirichlet
+    draw; the cell side grows with the cell count so the mesh spacing stays
+    near ``h_ref`` nm.  Placement is rejection-sampled with a clearance of 3
+    mesh spacings between pores and from the cell edges.
+    """
+    rng = np.random.default_rng(seed)
+    for _attempt in range(200):
+        k = int(rng.integers(1, max_pores + 1))
+        phi = float(rng.uniform(*porosity_range))
+        L = float(np.sqrt(n_cells * h_ref ** 2 / (2.0 * (1.0 - phi))))
+        h = np.sqrt(2.0 * L * L * (1 - phi) / n_cells)
+        shares = rng.dirichlet(np.ones(k)) * phi * L * L
+        kinds = ['circle' if rng.random() < 0.5 else 'square' for _ in range(k)]
+        sizes = [np.sqrt(a / np.pi) if kd == 'circle' else 0.5 * np.sqrt(a) for kd, a in zip(kinds, shares)]
+        radius = [s if kd == 'circle' else s * np.sqrt(2.0) for kd, s in zip(kinds, sizes)]
+        centres = []
+        ok = True
+        for r in radius:
+            lo = -L / 2 + r + 3 * h
+            hi = L / 2 - r - 3 * h
+            if lo >= hi:
+                ok = False
+                break
+            for _ in range(500):
+                c = rng.uniform(lo, hi, 2)
+                if all(np.hypot(*(c - c2)) > r + r2 + 3 * h for c2, r2 in zip(centres, radius)):
+                    centres.append(c)
+                    break
+            else:
+                ok = False
+                break
+        if not ok:
+            continue
+        pores = [(kd, (float(c[0]), float(c[1])), float(s)) for kd, c, s in zip(kinds, centres, sizes)]
+        try:
+            return pore_cell_2d(L, phi, n_cells, seed=seed, pores=pores)
+        except (RuntimeError, ValueError):
+            continue
+    raise RuntimeError(f'could not place pores for seed {seed}')

@@ -33,7 +33,9 @@ def test_struct_layout_matches_header():
     """Every field of the C structs is mirrored, in order, by the ctypes structs."""
     from openbte_amd import _lib
     header = open(os.path.join(ROOT, 'include', 'openbte_amd.h')).read()
-    for cname, py in (('obte_bte_plan', _lib.BtePlan), ('obte_gcr', _lib.Gcr), ('obte_fourier_plan', _lib.FourierPlan)):
+    for cname, py in (('obte_bte_plan', _lib.BtePlan), ('obte_gcr', _lib.Gcr), ('obte_fourier_plan', _lib.FourierPlan),
+                      ('obte_sweep_io', _lib.SweepIO), ('obte_reduce_io', _lib.ReduceIO),
+                      ('obte_bte_batch', _lib.Batch)):
         body = re.search(r'typedef struct ' + cname + r' \{(.*?)\}', header, re.S).group(1)
         body = re.sub(r'/\*.*?\*/', '', body, flags=re.S)
         fields = re.findall(r'\b(?:int32_t|int64_t|double|uint8_t)\s*\**\s*(\w+)\s*;', body)

@@ -1,0 +1,57 @@
+"""Batched solves: every problem of a batch gets the result of its own
+standalone solve (same arithmetic, per-problem stop rule), and matches the
+oracle."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _problem(seed, n, M=6, nphi=8):
+    from openbte_amd.geometry import random_pore_cell_2d
+    from openbte_amd.material import synthetic_material_2d
+    from openbte_amd.objects import BoundaryConditions, EffectiveThermalConductivity
+    geo = random_pore_cell_2d(seed, n)
+    mat = synthetic_material_2d(M, nphi, 1.0, 1.0e4)
+    bcs = BoundaryConditions(periodic={'Periodic_x': 1.0, 'Periodic_y': 0.0}, diffuse='Boundary')
+    eff = EffectiveThermalConductivity(contact='Periodic_x', normalization=-geo.size[0] / geo.size[1])
+    return dict(geo=geo, mat=mat, bcs=bcs, effective_thermal_conductivity=eff)
+
+
+def test_batch_equals_single_and_oracle():
+    from oracle import bte_oracle as bo, fourier_oracle as fo
+    from openbte_amd.bte import BTE_RTA
+    from openbte_amd.batch import BTE_RTA_batch
+    probs = [_problem(11, 300), _problem(12, 900), _problem(13, 500)]
+    for pr in probs:
+        pr['fourier'] = fo.fourier(pr['geo'], pr['mat'].thermal_conductivity, pr['bcs'],
+                                   pr['effective_thermal_conductivity'])
+    outs = BTE_RTA_batch(probs, bte_max_iter=8)
+    for pr, out in zip(probs, outs):
+        single = BTE_RTA(pr['geo'], pr['mat'], pr['bcs'], fourier=pr['fourier'],
+                         effective_thermal_conductivity=pr['effective_thermal_conductivity'], bte_max_iter=8)
+        assert out.aux['n_iter'] == single.aux['n_iter']
+        assert out.kappa_eff == single.kappa_eff
+        np.testing.assert_array_equal(out.variables['Temperature_BTE']['data'],
+                                      single.variables['Temperature_BTE']['data'])
+        ref = bo.bte_rta(pr['geo'], pr['mat'], pr['bcs'], fourier=pr['fourier'],
+                         effective_thermal_conductivity=pr['effective_thermal_conductivity'], bte_max_iter=8)
+        assert out.aux['n_iter'] == ref.aux['n_iter']
+        assert abs(out.kappa_eff / ref.kappa_eff - 1) < 1e-6
+        assert np.abs(out.variables['Temperature_BTE']['data']
+                      - ref.variables['Temperature_BTE']['data']).max() < 1e-8
+
+
+def test_batch_different_iteration_counts():
+    """Problems stop at their own iteration; the early ones stay frozen."""
+    from openbte_amd.bte import BTE_RTA
+    from openbte_amd.batch import BTE_RTA_batch
+    probs = [_problem(21, 400), _problem(22, 400)]
+    outs = BTE_RTA_batch(probs, bte_max_iter=30, bte_rtol=1e-3)
+    for pr, out in zip(probs, outs):
+        single = BTE_RTA(pr['geo'], pr['mat'], pr['bcs'],
+                         effective_thermal_conductivity=pr['effective_thermal_conductivity'],
+                         bte_max_iter=30, bte_rtol=1e-3)
+        assert out.aux['n_iter'] == single.aux['n_iter']
+        np.testing.assert_array_equal(out.variables['Temperature_BTE']['data'],
+                                      single.variables['Temperature_BTE']['data'])
