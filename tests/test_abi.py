@@ -38,5 +38,5 @@ def test_struct_layout_matches_header():
                       ('obte_bte_batch', _lib.Batch)):
         body = re.search(r'typedef struct ' + cname + r' \{(.*?)\}', header, re.S).group(1)
         body = re.sub(r'/\*.*?\*/', '', body, flags=re.S)
-        fields = re.findall(r'\b(?:int32_t|int64_t|double|uint8_t)\s*\**\s*(\w+)\s*;', body)
+        fields = re.findall(r'^\s*(?:const\s+)?\w+\s*\**\s*(\w+)\s*;', body, re.M)
         assert fields == [f[0] for f in py._fields_], cname
