@@ -33,6 +33,11 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+BATCH_CONFIGS = {
+    # name: (geometries, min cells, max cells, n_mfp, n_dirs, mfp_max nm, seed)
+    'batch64': (64, 1000, 80000, 30, 48, 1.0e4, 2026),
+}
+
 CONFIGS = {
     # name: (L nm, porosity, cells, n_mfp, n_dirs, mfp_max nm)
     'plain2k': (10.0, 0.25, 2000, 30, 48, 1.0e4),
@@ -111,12 +116,100 @@ def cpu_baseline(geo, mat, bcs, eff, fourier_T, budget_s=20.0):
     }
 
 
+def make_batch(name):
+    from openbte_amd.geometry import random_pore_cell_2d
+    from openbte_amd.material import synthetic_material_2d
+    from openbte_amd.objects import BoundaryConditions, EffectiveThermalConductivity
+    G, nmin, nmax, M, nphi, mfp_max, seed = BATCH_CONFIGS[name]
+    rng = np.random.default_rng(seed)
+    ns = np.exp(rng.uniform(np.log(nmin), np.log(nmax), G)).astype(int)
+    mat = synthetic_material_2d(M, nphi, 1.0, mfp_max, seed=seed)
+    bcs = BoundaryConditions(periodic={'Periodic_x': 1.0, 'Periodic_y': 0.0}, diffuse='Boundary')
+    eff = EffectiveThermalConductivity(contact='Periodic_x', normalization=-1.0)
+    geos = [random_pore_cell_2d(seed + 1 + i, int(n)) for i, n in enumerate(ns)]
+    return geos, mat, bcs, eff
+
+
+def run_batch(args):
+    """Batched form vs the plain one-solve-at-a-time form on the same problems."""
+    import torch
+    from openbte_amd.batch import BTEBatch
+    from openbte_amd.bte import BTEDevice, SweepConfig
+    from openbte_amd.fourier import Fourier
+    from openbte_amd.operator import build_host_operator
+    geos, mat, bcs, eff = make_batch(args.config)
+    M, L = mat.grid[0], mat.grid[1]
+    ncells = np.array([g.n_elems for g in geos])
+    print(f'# {args.config}: {len(geos)} geometries, {ncells.sum()} cells ({ncells.min()}-{ncells.max()})',
+          file=sys.stderr, flush=True)
+    t0s = []
+    for g in geos:
+        f = Fourier(g, mat.thermal_conductivity, bcs, eff, verbose=False)
+        t0s.append(f.variables['Temperature_Fourier']['data'])
+    ops = [build_host_operator(g, mat, bcs, eff) for g in geos]
+    cfg = SweepConfig(mc=args.mc or 8, block=args.block or 256)
+    K = args.steps
+    updates = float(ncells.sum()) * M * L * K
+
+    batch = BTEBatch(ops, cfg=cfg)
+    if not args.no_graphs:
+        batch.enable_graphs(True)
+    batch.reset(t0s)
+    for _ in range(args.warmup):
+        batch.outer_iteration()
+    batch.reset(t0s)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    exts = [batch.outer_iteration() for _ in range(K)]
+    torch.cuda.synchronize()
+    dt_batch = time.perf_counter() - t
+    kappas = [float(d.buf['kappa'].sum().item()) for d in batch.devs]
+    del batch
+    torch.cuda.empty_cache()
+
+    # plain form: the same K outer iterations, one geometry at a time
+    dt_plain = 0.0
+    plain_kappa = []
+    for op, t0 in zip(ops, t0s):
+        dev = BTEDevice(op, cfg=cfg)
+        dev.reset(t0)
+        dev.outer_iteration()
+        dev.reset(t0)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(K):
+            dev.outer_iteration()
+        torch.cuda.synchronize()
+        dt_plain += time.perf_counter() - t
+        plain_kappa.append(float(dev.buf['kappa'].sum().item()))
+        del dev
+    torch.cuda.empty_cache()
+    same = max(abs(a - b) / abs(b) for a, b in zip(kappas, plain_kappa))
+    value = updates / dt_batch
+    out = {
+        'metric': 'cell*mode updates/s (batched outer iterations, all systems to direct-solver accuracy)',
+        'value': value, 'unit': 'cell*mode updates/s', 'n_gpus': 1, 'steps': K, 'warmup': args.warmup,
+        'ms_per_step': dt_batch / K * 1e3, 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': 'f64', 'data': 'synthetic (64 seeded multi-pore cells, seeded lognormal Si-like material)',
+        'config': {'workload': f'{args.config}: {len(geos)} periodic multi-pore cells, {int(ncells.sum())} cells '
+                               f'({int(ncells.min())}-{int(ncells.max())}), {M} MFP x {L} directions',
+                   'mc': cfg.mc, 'block': cfg.block, 'hipgraph': not args.no_graphs,
+                   'extensions_per_step': exts,
+                   'plain_value': updates / dt_plain, 'plain_ms_per_step': dt_plain / K * 1e3,
+                   'batched_over_plain': dt_plain / dt_batch,
+                   'max_rel_kappa_diff_batched_vs_plain': same},
+    }
+    print(json.dumps(out))
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=20)
     ap.add_argument('--warmup', type=int, default=2)
-    ap.add_argument('--config', default='plain200k', choices=sorted(CONFIGS))
+    ap.add_argument('--config', default='plain200k', choices=sorted(CONFIGS) + sorted(BATCH_CONFIGS))
+    ap.add_argument('--no-graphs', action='store_true', help='batched configs: launch without hipGraphs')
     ap.add_argument('--mc', type=int, default=None)
     ap.add_argument('--block', type=int, default=None)
     ap.add_argument('--no-cpu', action='store_true')
@@ -130,6 +223,9 @@ def main(argv=None):
         print(json.dumps({'error': 'this build runs on one GPU; multi-GPU runs (mode sharding + one '
                                    'all-reduce per outer iteration) are not part of it', 'n_gpus': args.gpus}))
         return 2
+
+    if args.config in BATCH_CONFIGS:
+        return run_batch(args)
 
     import torch
     from openbte_amd.bte import BTEDevice, SweepConfig

@@ -60,7 +60,7 @@ typedef struct obte_bte_plan {
   int32_t dim;        /* 2 or 3 */
   int32_t mc;         /* MFP bins per chunk: 4, 8, 16, 32 or 64 */
   int32_t n_chunks;   /* ceil(M / mc) */
-  int32_t block;      /* threads per sweep workgroup (multiple of 64, <= 1024) */
+  int32_t block;      /* threads per sweep workgroup: 64, 128 or 256 */
   int32_t n_cut;      /* total cut faces over all directions */
   int32_t n_wall;     /* diffuse wall sides */
   int32_t n_contact;  /* sides of the kappa contact */
@@ -105,11 +105,12 @@ typedef struct obte_bte_plan {
   double nano;             /* 1 for the nanowire perturbation -s_z (3D), else 0 */
 
   /* assembled operator (written by obte_bte_assemble) */
-  double* nb_coef;         /* [L][kin][n] -min(s . k, 0) of each inflow slot */
-  double* diag;            /* [L][n] D_l in position order */
   double* pert;            /* [L][n] perturbation in position order */
-  double* q;               /* [L][n] pert + diffuse-wall source (per outer iteration) */
-  double* b0;              /* [L][n] DeltaT + heat in position order (per outer iteration) */
+  double* rec;             /* [L][n][8] sweep record of each position (64 B):
+                              [0] D_l, [1] b0 = DeltaT + heat, [2] q = pert + diffuse-wall source,
+                              [3..5] -min(s . k, 0) of the inflow slots (0 if unused),
+                              [6] int32 pair (src0, src1), [7] int32 pair (src2, cell);
+                              [1] and [2] are rewritten by obte_bte_source every outer iteration */
 
   /* state */
   const double* delta_t;   /* [n] lattice temperature of the previous iteration */
@@ -141,11 +142,12 @@ typedef struct obte_reduce_io {
 } obte_reduce_io_t;
 
 /* Face-flux assembly of D_l, the inflow coefficients and the perturbation
- * (openbte/bte.py:47-124). */
+ * (openbte/bte.py:47-124) into pert and the static part of rec. */
 int obte_bte_assemble(const obte_bte_plan_t* plan, obte_stream_t stream);
 
-/* Per-iteration sources in position order: q = pert - TB Gbm2 (diffuse
- * walls) and b0 = DeltaT + heat (openbte/bte.py:199-200,211-214). */
+/* Per-iteration sources in position order, into rec[1] and rec[2]:
+ * q = pert - TB Gbm2 (diffuse walls) and b0 = DeltaT + heat
+ * (openbte/bte.py:199-200,211-214). */
 int obte_bte_source(const obte_bte_plan_t* plan, obte_stream_t stream);
 
 /* Level-scheduled upwind sweeps of all (l, m) systems (openbte/bte.py:217-218).
@@ -228,6 +230,9 @@ int obte_batch_sweep(const obte_bte_batch_t* b, int32_t mode, obte_stream_t stre
 /* op 0 project, 1 begin_extend, 2 end_extend (as the obte_gcr_* calls) */
 int obte_batch_gcr(const obte_bte_batch_t* b, int32_t op, obte_stream_t stream);
 int obte_batch_gcr_reset(const obte_bte_batch_t* b, int32_t use_delta_t, obte_stream_t stream);
+/* *out (device int) = number of active GCR tasks over all problems; used to
+ * record the rounds of a graph-captured iteration. */
+int obte_batch_log_active(const obte_bte_batch_t* b, int32_t* out, obte_stream_t stream);
 /* reduce into rio, then DeltaT, TB <- rio.dt_new, rio.tb_new for live problems */
 int obte_batch_reduce(const obte_bte_batch_t* b, obte_stream_t stream);
 

@@ -29,7 +29,7 @@ class BtePlan(C.Structure):
         ('wall_ptr', _p), ('wall_side', _p), ('wall_db', _p), ('wall_temp', _p), ('wall_cell', _p),
         ('wall_rinv', _p), ('wall_dbs', _p), ('ce1', _p), ('ce2', _p), ('c_na', _p), ('mfps', _p),
         ('t_coeff', _p), ('sigma', _p), ('heat', _p), ('nano', _f64),
-        ('nb_coef', _p), ('diag', _p), ('pert', _p), ('q', _p), ('b0', _p),
+        ('pert', _p), ('rec', _p),
         ('delta_t', _p), ('tb', _p), ('t', _p), ('part_t', _p), ('part_j', _p), ('part_tb', _p),
         ('kappa', _p), ('supp', _p), ('scratch', _p),
     ]
@@ -95,6 +95,7 @@ _SIGNATURES = {
     'obte_batch_gcr': [C.POINTER(Batch), _i32, _p],
     'obte_batch_gcr_reset': [C.POINTER(Batch), _i32, _p],
     'obte_batch_reduce': [C.POINTER(Batch), _p],
+    'obte_batch_log_active': [C.POINTER(Batch), _p, _p],
     'obte_bte_reduce': [C.POINTER(BtePlan), _p, _p, _p, _p, _p],
     'obte_fourier_solve': [C.POINTER(FourierPlan), _p, _p, _p, _p, _p, _f64, _i32, _i32,
                            C.POINTER(_i32), C.POINTER(_f64), _p],
@@ -115,6 +116,8 @@ def load() -> C.CDLL:
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f'{LIB_PATH} is missing: run `python -m openbte_amd.build` (no CPU fallback exists)')
+    # load torch's HIP runtime first so that the library binds to the same one
+    import torch  # noqa: F401
     lib = C.CDLL(LIB_PATH)
     for name, args in _SIGNATURES.items():
         fn = getattr(lib, name)

@@ -153,22 +153,86 @@ class BTEBatch:
         self.norm_old = np.array([float(np.linalg.norm(t)) for t in delta_t0])
         _lib.call('obte_batch_gcr_reset', self.b, 1, self.stream)
 
-    def outer_iteration(self, max_extend: Optional[int] = None) -> int:
-        """One outer iteration of every live problem; returns the extensions."""
-        b, st = self.b, self.stream
-        max_extend = self.cfg.max_extend if max_extend is None else max_extend
+    # launch sequences (always on the caller's current stream, so they can be captured)
+    def _head(self, rounds: int, log: bool = False):
+        b, st = self.b, torch.cuda.current_stream(self.dev).cuda_stream
         _lib.call('obte_batch_source', b, st)
         _lib.call('obte_batch_sweep', b, 0, st)
         _lib.call('obte_batch_gcr', b, 0, st)
-        ext = 0
-        while ext < max_extend and int(self.n_active.sum().item()) > 0:
-            _lib.call('obte_batch_gcr', b, 1, st)
-            _lib.call('obte_batch_sweep', b, 1, st)
-            _lib.call('obte_batch_gcr', b, 2, st)
-            ext += 1
+        if log:
+            _lib.call('obte_batch_log_active', b, self.round_log[0:1].data_ptr(), st)
+        for i in range(rounds):
+            self._round()
+            if log:
+                _lib.call('obte_batch_log_active', b, self.round_log[i + 1:i + 2].data_ptr(), st)
+
+    def _round(self):
+        b, st = self.b, torch.cuda.current_stream(self.dev).cuda_stream
+        _lib.call('obte_batch_gcr', b, 1, st)
+        _lib.call('obte_batch_sweep', b, 1, st)
+        _lib.call('obte_batch_gcr', b, 2, st)
+
+    def _tail(self):
+        b, st = self.b, torch.cuda.current_stream(self.dev).cuda_stream
         _lib.call('obte_batch_sweep', b, 2, st)
         _lib.call('obte_batch_reduce', b, st)
+
+    def outer_iteration(self, max_extend: Optional[int] = None) -> int:
+        """One outer iteration of every live problem; returns the extensions."""
+        max_extend = self.cfg.max_extend if max_extend is None else max_extend
+        if self.use_graphs:
+            return self._outer_iteration_graph(max_extend)
+        self._head(0)
+        ext = 0
+        while ext < max_extend and int(self.n_active.sum().item()) > 0:
+            self._round()
+            ext += 1
+        self._tail()
         return ext
+
+    # -- hipGraph-captured outer iteration ---------------------------------
+    use_graphs = False
+
+    def enable_graphs(self, on: bool = True):
+        """Replay the outer iteration from hipGraphs: graph A = source, first
+        sweep, projection and E extension rounds (rounds of converged tasks
+        are no-ops inside the kernels); graph B = final sweep and reduction.
+        E follows the previous iteration's extension count; a residual left
+        after graph A is finished by host-driven rounds before graph B, so the
+        solve stays exact."""
+        self.use_graphs = on
+        self._graphs = {}
+        self._tail_graph = None
+        self._next_rounds = 8
+        self.round_log = torch.zeros(self.cfg.max_extend + 2, dtype=torch.int32, device=self.dev)
+
+    def _capture(self, fn, *args):
+        g = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream(device=self.dev)
+        s.wait_stream(torch.cuda.current_stream(self.dev))
+        with torch.cuda.stream(s):
+            with torch.cuda.graph(g, stream=s):
+                fn(*args)
+        torch.cuda.current_stream(self.dev).wait_stream(s)
+        return g
+
+    def _outer_iteration_graph(self, max_extend: int) -> int:
+        E = max(1, min(self._next_rounds, max_extend))
+        if E not in self._graphs:
+            self._graphs[E] = self._capture(self._head, E, True)
+        if self._tail_graph is None:
+            self._tail_graph = self._capture(self._tail)
+        self._graphs[E].replay()
+        log = self.round_log[:E + 1].cpu().numpy()
+        ext = int((log[:E] > 0).sum())  # rounds that found active tasks
+        extra = 0
+        if log[E] > 0:
+            while ext + extra < max_extend and int(self.n_active.sum().item()) > 0:
+                self._round()
+                extra += 1
+        self._tail_graph.replay()
+        self._next_rounds = max(1, ext + extra)
+        return ext + extra
 
     def solve(self, delta_t0, bte_max_iter: int = 100, bte_rtol: float = 1e-4, on_iteration=None):
         """Iterate every problem to its own stop rule (openbte/bte.py:193-196,251-254)."""

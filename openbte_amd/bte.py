@@ -123,11 +123,8 @@ class BTEDevice:
         b['sigma'] = _t(op.sigma, f64, dev)
         b['heat'] = _t(op.heat, f64, dev)
         # assembled operator
-        b['nb_coef'] = torch.zeros(L * self.kin * n, dtype=f64, device=dev)
-        b['diag'] = torch.zeros(L * n, dtype=f64, device=dev)
         b['pert'] = torch.zeros(L * n, dtype=f64, device=dev)
-        b['q'] = torch.zeros(L * n, dtype=f64, device=dev)
-        b['b0'] = torch.zeros(L * n, dtype=f64, device=dev)
+        b['rec'] = torch.zeros(L * n * 8, dtype=f64, device=dev)
         # state
         b['dt'] = [torch.zeros(n, dtype=f64, device=dev), torch.zeros(n, dtype=f64, device=dev)]
         b['tb'] = [torch.zeros(max(1, n_wall), dtype=f64, device=dev) for _ in range(2)]
@@ -175,7 +172,7 @@ class BTEDevice:
         for name in ('order', 'pos', 'lvl_off', 'lvl_ptr', 'nb_src', 'nb_face', 'cut_off',
                      'cut_exit', 'dirs', 'k_face', 'jump_face', 'cf_ptr', 'cf_face', 'wall_ptr', 'wall_side',
                      'wall_db', 'wall_temp', 'wall_cell', 'wall_rinv', 'wall_dbs', 'ce1', 'ce2', 'c_na', 'mfps',
-                     't_coeff', 'sigma', 'heat', 'nb_coef', 'diag', 'pert', 'q', 'b0', 't', 'part_t', 'part_j',
+                     't_coeff', 'sigma', 'heat', 'pert', 'rec', 't', 'part_t', 'part_j',
                      'part_tb', 'kappa', 'supp', 'scratch'):
             setattr(P, name, b[name].data_ptr())
         P.nano = op.nano

@@ -45,7 +45,8 @@ def build(verbose: bool = False) -> str:
             objs.append(obj)
         for src in CPP_SOURCES:
             obj = os.path.join(tmp, src + '.o')
-            cmd = ['g++', '-c', '-O3', '-fPIC', '-std=c++17', '-fopenmp', '-ffp-contract=off'] + inc + \
+            cmd = ['g++', '-c', '-O3', '-fPIC', '-std=c++17', '-fopenmp', '-ffp-contract=off',
+                   '-I', '/opt/rocm/include', '-D__HIP_PLATFORM_AMD__'] + inc + \
                   [os.path.join(CSRC, src), '-o', obj]
             jobs.append(cmd)
             objs.append(obj)

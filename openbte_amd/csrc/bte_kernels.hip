@@ -23,6 +23,17 @@ namespace {
 
 constexpr int kMaxBlock = 1024;
 
+// Packed per-position record of the sweep operator, kRec doubles (64 B):
+//   [0] D_l  [1] b0 = DeltaT + heat  [2] q  [3..5] inflow coefficients (unused = 0)
+//   [6] (src0, src1)  [7] (src2, cell) as int32 pairs
+constexpr int kRec = 8;
+
+__device__ __forceinline__ double pack_ints(int lo, int hi) { return __hiloint2double(hi, lo); }
+
+// s_waitcnt immediate (gfx9 encoding) that waits for vmcnt <= n only
+constexpr int vmcnt_only(int n) { return (n & 15) | (7 << 4) | (15 << 8) | ((n >> 4) << 14); }
+constexpr int kSweepBlock = 256;  // sweep kernels: at most 256 threads (register budget)
+
 __device__ __forceinline__ double dotv(const double* a, const double* b, int dim) {
   double g = a[0] * b[0] + a[1] * b[1];
   if (dim == 3) g += a[2] * b[2];
@@ -108,18 +119,23 @@ __global__ void k_assemble(S src) {
     else if (P.wall_side[w] < 0) pert -= g * P.wall_temp[w];
   }
   if (P.nano != 0.0 && dim == 3) pert -= s[2];
-  P.diag[gid] = D;
   P.pert[gid] = pert;
   const int64_t base = (int64_t)l * P.kin * P.n + p;
+  double cf[3] = {0.0, 0.0, 0.0};
+  int sr[3] = {0, 0, 0};
   for (int kk = 0; kk < P.kin; ++kk) {
     const int f = P.nb_face[base + (int64_t)kk * P.n];
-    double cf = 0.0;
+    sr[kk] = P.nb_src[base + (int64_t)kk * P.n];
     if (f >= 0) {
       const double g = dotv(s, P.k_face + (int64_t)f * dim, dim);
-      cf = g < 0.0 ? -g : 0.0;
+      cf[kk] = g < 0.0 ? -g : 0.0;
     }
-    P.nb_coef[base + (int64_t)kk * P.n] = cf;
   }
+  double2* r = reinterpret_cast<double2*>(P.rec + gid * kRec);
+  r[0] = make_double2(D, 0.0);
+  r[1] = make_double2(0.0, cf[0]);
+  r[2] = make_double2(cf[1], cf[2]);
+  r[3] = make_double2(pack_ints(sr[0], sr[1]), pack_ints(sr[2], c));
 }
 
 // q = pert + sum over diffuse walls of (-TB_s) min(s . db, 0)   (openbte/bte.py:199-200,214)
@@ -143,121 +159,107 @@ __global__ void k_source(S src) {
     const double g = dotv(s, P.wall_db + (int64_t)w * dim, dim);
     if (g < 0.0) q += -P.tb[sd] * g;
   }
-  P.q[gid] = q;
-  P.b0[gid] = P.delta_t[c] + P.heat[c];
+  P.rec[gid * kRec + 1] = P.delta_t[c] + P.heat[c];
+  P.rec[gid * kRec + 2] = q;
 }
 
 // ---------------------------------------------------------------------------
 // (b) level-scheduled upwind sweep, (c) fused mode sums
 //
-// One workgroup per task = (problem, direction, chunk of MC MFP bins); one
-// lane per cell of the current level, looping over the MC bins of its chunk.
-// A cell's MC values are one contiguous row of T, so every upwind gather is
-// a run of 16-byte vector loads and the mode sums are register accumulations.
+// One workgroup per task = (problem, direction, chunk of MC MFP bins).  A
+// cell's MC values are one contiguous row of T; G = MC/2 consecutive lanes
+// share a cell and each owns a pair of bins (one 16-byte access), so a wave
+// reads or writes 64/G whole rows with one cache line per row.  Each thread
+// holds U cells of the current level (cells slot, slot + CPS, ...), issues all
+// their loads first, then the divisions and stores.
 //
-// Software pipelining across levels: at the start of level k each lane
-// loads the source terms, diagonal, inflow slots and coefficients of its
-// cell of level k+1, so their latency overlaps level k's work and after the
-// barrier only the upwind gathers remain on the critical path.  Levels
-// wider than the block are finished with extra (non-prefetched) passes.
+// Software pipelining across levels: at the start of level k the per-cell
+// data of level k+1 (sources, diagonal, inflow slots and coefficients) is
+// loaded, so after the barrier only the upwind gathers are on the critical
+// path.  Level pointers live in LDS (a scalar load in the loop would make
+// every LDS access wait on it).  Levels wider than U*CPS cells take extra
+// passes.
 // ---------------------------------------------------------------------------
-template <int KIN>
-struct CellRegs {
-  int p;  // position, -1 if this lane has no cell
-  double b0, q, dd;
-  int src[KIN];
-  double cf[KIN];
-  int cell;
-  double dtc;
+// Lanes of one cell that share a quad (2 for MC = 4, else 4) split the 64-byte
+// record: lane jq of the quad loads NP = 4 / GQ 16-byte pieces, starting at
+// piece jq * NP, and the quad exchanges them with DPP (no LDS, no memory).
+template <int U, int NP>
+struct LevelRegs {
+  int p[U];            // position, -1 for an idle slot
+  double2 pc[U][NP];   // this lane's pieces of the record
 };
 
-template <int MODE, int KIN>
-__device__ __forceinline__ void load_cell(CellRegs<KIN>& R, int p, int p1, int n, const int32_t* __restrict__ nsrc,
-                                          const double* __restrict__ ncoef, const double* __restrict__ b0,
-                                          const double* __restrict__ qv, const double* __restrict__ dg,
-                                          const int32_t* __restrict__ order, const double* __restrict__ dt) {
-  const bool ok = p < p1;
-  R.p = ok ? p : -1;
-  if (!ok) return;
-  R.dd = dg[p];
-  R.b0 = 0.0;
-  R.q = 0.0;
-  if (MODE != 1) {
-    R.b0 = b0[p];
-    R.q = qv[p];
-  }
+template <int U, int NP>
+__device__ __forceinline__ void load_level(LevelRegs<U, NP>& R, int base, int p1, int cps, int jq,
+                                           const double* __restrict__ rec) {
 #pragma unroll
-  for (int kk = 0; kk < KIN; ++kk) {
-    R.src[kk] = nsrc[(int64_t)kk * n + p];
-    R.cf[kk] = ncoef[(int64_t)kk * n + p];
-  }
-  if (MODE == 2) {
-    R.cell = order[p];
-    R.dtc = dt[R.cell];
+  for (int u = 0; u < U; ++u) {
+    const int p = base + u * cps;
+    const bool ok = p < p1;
+    R.p[u] = ok ? p : -1;
+    const int pc = ok ? p : 0;  // idle slots read position 0 (always present) and are discarded
+    const double2* r = reinterpret_cast<const double2*>(rec + (int64_t)pc * kRec) + jq * NP;
+#pragma unroll
+    for (int i = 0; i < NP; ++i) R.pc[u][i] = r[i];
   }
 }
 
-constexpr int kSub = 4;  // MFP bins per register block
-
-template <int MC, int MODE, int KIN>
-__device__ __forceinline__ void solve_cell(const CellRegs<KIN>& R, double* __restrict__ T, const double* s_lam,
-                                           const int* s_act, const double* s_tco, const double* s_sig, int dim,
-                                           double& pt, double& pj0, double& pj1, double& pj2) {
-  const double* __restrict__ rows[KIN];
-#pragma unroll
-  for (int kk = 0; kk < KIN; ++kk) rows[kk] = T + (int64_t)R.src[kk] * MC;
-  double* __restrict__ out = T + (int64_t)R.p * MC;
-  // one register block of kSub bins live at a time
-#pragma unroll 1
-  for (int mb = 0; mb < MC; mb += kSub) {
-    if (MODE == 1 && !s_act[mb / kSub]) continue;  // uniform: whole sub-block converged
-    constexpr int W = MC < kSub ? MC : kSub;
-    double acc[W];
-#pragma unroll
-    for (int j = 0; j < W; ++j) acc[j] = 0.0;
-#pragma unroll
-    for (int kk = 0; kk < KIN; ++kk) {
-      const double2* r2 = reinterpret_cast<const double2*>(rows[kk] + mb);
-#pragma unroll
-      for (int j = 0; j < W / 2; ++j) {
-        const double2 v = r2[j];
-        acc[2 * j] += R.cf[kk] * v.x;
-        acc[2 * j + 1] += R.cf[kk] * v.y;
-      }
-    }
-    double tv[W];
-#pragma unroll
-    for (int j = 0; j < W; ++j) {
-      const double lam = s_lam[mb + j];
-      tv[j] = (R.b0 + lam * (R.q + acc[j])) / (1.0 + lam * R.dd);
-    }
-    double2* o2 = reinterpret_cast<double2*>(out + mb);
-#pragma unroll
-    for (int j = 0; j < W / 2; ++j) o2[j] = make_double2(tv[2 * j], tv[2 * j + 1]);
-    if (MODE == 2) {
-#pragma unroll
-      for (int j = 0; j < W; ++j) {
-        const int m = mb + j;
-        pt += s_tco[m] * tv[j];
-        const double d = tv[j] - R.dtc;
-        pj0 += d * s_sig[3 * m + 0];
-        pj1 += d * s_sig[3 * m + 1];
-        if (dim == 3) pj2 += d * s_sig[3 * m + 2];
-      }
-    }
-  }
+// v of lane S of this lane's group of GQ lanes inside its quad
+template <int GQ, int S>
+__device__ __forceinline__ double quad_bcast(double v) {
+  constexpr int ctrl = GQ == 4 ? (S | S << 2 | S << 4 | S << 6) : (S | S << 2 | (2 + S) << 4 | (2 + S) << 6);
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), ctrl, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), ctrl, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
 }
+
+// double D of the record held in pieces pc (all lanes of the wave active)
+template <int GQ, int NP, int D>
+__device__ __forceinline__ double rec_get(const double2 (&pc)[NP]) {
+  constexpr int piece = D / 2;
+  const double2 w = pc[piece % NP];
+  return quad_bcast<GQ, piece / NP>(D % 2 ? w.y : w.x);
+}
+
+// v + v of lane (lane ^ O) within quads
+template <int O>
+__device__ __forceinline__ double quad_xor_add(double v) {
+  constexpr int ctrl = O == 1 ? (1 | 0 << 2 | 3 << 4 | 2 << 6) : (2 | 3 << 2 | 0 << 4 | 1 << 6);
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), ctrl, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), ctrl, 0xF, 0xF, false);
+  return v + __hiloint2double(hi, lo);
+}
+
+// Sum over the G lanes of one cell (fixed xor-tree order; every lane of the
+// cell gets the same bits since the additions commute).
+template <int G>
+__device__ __forceinline__ double cell_sum(double v) {
+#pragma unroll
+  for (int o = G / 2; o >= 4; o >>= 1) v += __shfl_xor(v, o, G);
+  if (G >= 4) v = quad_xor_add<2>(v);
+  if (G >= 2) v = quad_xor_add<1>(v);
+  return v;
+}
+
+constexpr int kLvlStage = 2048;  // level pointers staged in LDS per refill
 
 template <int MC>
 struct SweepShared {
-  double red[kMaxBlock];
-  double lam[MC], tco[MC], sig[3 * MC];
-  int act[(MC + kSub - 1) / kSub];
+  double red[kSweepBlock];
+  double sig[3 * MC];
+  int lvl[kLvlStage + 2];  // staged level starts of the running direction
 };
+
+template <int MC>
+constexpr int units_of() { return MC >= 16 ? 2 : 4; }
 
 // One (problem pid, task lt = l * n_chunks + chunk) sweep by the whole block.
 template <class S, int MC, int MODE, int KIN>
 __device__ void sweep_task(const S& src, int pid, int lt, SweepShared<MC>& sh) {
+  constexpr int G = MC / 2;       // lanes per cell
+  constexpr int U = units_of<MC>();
+  constexpr int GQ = G < 4 ? G : 4;  // lanes of a cell inside one quad
+  constexpr int NP = 4 / GQ;         // record pieces per lane
   const obte_bte_plan_t& P = src.plan(pid);
   const obte_sweep_io_t& io = src.sio(pid);
   const int L = P.n_dirs;
@@ -267,36 +269,54 @@ __device__ void sweep_task(const S& src, int pid, int lt, SweepShared<MC>& sh) {
   const int dim = P.dim;
   const int tid = threadIdx.x;
   const int nthr = blockDim.x;
+  const int jl = tid % G;          // bin pair of this lane
+  const int cslot = tid / G;       // cell slot of this lane
+  const int cps = nthr / G;        // cells per pass
+  const int jq = jl % GQ;          // record part of this lane
 
-  if (tid < MC) {
-    const int mg = chunk * MC + tid;
-    const bool valid = mg < P.n_mfp;
-    sh.lam[tid] = valid ? P.mfps[mg] : 0.0;
-    sh.tco[tid] = (MODE == 2 && valid) ? P.t_coeff[(int64_t)mg * L + l] : 0.0;
-    for (int d = 0; d < 3; ++d)
-      sh.sig[3 * tid + d] = (MODE == 2 && valid && d < dim) ? P.sigma[((int64_t)mg * L + l) * dim + d] : 0.0;
-  }
-  if (tid < (MC + kSub - 1) / kSub) {
-    int a = 1;
-    if (MODE == 1 && io.lane_active != nullptr) {
-      a = 0;
-      for (int j = tid * kSub; j < (tid + 1) * kSub && j < MC; ++j) a |= io.lane_active[(int64_t)lt * MC + j];
+  // per-lane constants of its two bins
+  const int m0 = chunk * MC + 2 * jl;
+  const bool v0 = m0 < P.n_mfp, v1 = m0 + 1 < P.n_mfp;
+  const double lam0 = v0 ? P.mfps[m0] : 0.0, lam1 = v1 ? P.mfps[m0 + 1] : 0.0;
+  double tco0 = 0.0, tco1 = 0.0, sx0 = 0.0, sx1 = 0.0, sy0 = 0.0, sy1 = 0.0, sz0 = 0.0, sz1 = 0.0;
+  if (MODE == 2) {
+    if (v0) {
+      tco0 = P.t_coeff[(int64_t)m0 * L + l];
+      const double* sg = P.sigma + ((int64_t)m0 * L + l) * dim;
+      sx0 = sg[0];
+      sy0 = sg[1];
+      if (dim == 3) sz0 = sg[2];
     }
-    sh.act[tid] = a;
+    if (v1) {
+      tco1 = P.t_coeff[(int64_t)(m0 + 1) * L + l];
+      const double* sg = P.sigma + ((int64_t)(m0 + 1) * L + l) * dim;
+      sx1 = sg[0];
+      sy1 = sg[1];
+      if (dim == 3) sz1 = sg[2];
+    }
+    if (cslot == 0) {
+      sh.sig[3 * 2 * jl + 0] = sx0;
+      sh.sig[3 * 2 * jl + 1] = sy0;
+      sh.sig[3 * 2 * jl + 2] = sz0;
+      sh.sig[3 * (2 * jl + 1) + 0] = sx1;
+      sh.sig[3 * (2 * jl + 1) + 1] = sy1;
+      sh.sig[3 * (2 * jl + 1) + 2] = sz1;
+    }
   }
+  bool act = true;  // transfer sweeps skip converged bin pairs
+  if (MODE == 1 && io.lane_active != nullptr)
+    act = io.lane_active[(int64_t)lt * MC + 2 * jl] != 0 || io.lane_active[(int64_t)lt * MC + 2 * jl + 1] != 0;
 
   double* __restrict__ T = P.t + (int64_t)lt * ((int64_t)n + P.ncut_max) * MC;
-  const int32_t* __restrict__ lvl = P.lvl_ptr + P.lvl_off[l];
+  const int32_t* __restrict__ lvl_g = P.lvl_ptr + P.lvl_off[l];
   const int nlev = P.lvl_off[l + 1] - P.lvl_off[l] - 1;
   const int cbase = P.cut_off[l];
   const int ncut = P.cut_off[l + 1] - cbase;
   const int64_t lo = (int64_t)l * n;
-  const int32_t* __restrict__ nsrc = P.nb_src + lo * KIN;
-  const double* __restrict__ ncoef = P.nb_coef + lo * KIN;
-  const double* __restrict__ b0 = P.b0 + lo;
-  const double* __restrict__ qv = P.q + lo;
-  const double* __restrict__ dg = P.diag + lo;
-  const int32_t* __restrict__ order = P.order + lo;
+  const double* __restrict__ rec = P.rec + lo * kRec;
+  const double* __restrict__ dtp = P.delta_t;
+  double* __restrict__ part_t = P.part_t + (int64_t)lt * n;
+  double* __restrict__ part_j = P.part_j + (int64_t)lt * n * dim;
 
   // interface inflows of the cut faces -> rows n .. n + ncut of T
   if (ncut > 0) {
@@ -304,37 +324,126 @@ __device__ void sweep_task(const S& src, int pid, int lt, SweepShared<MC>& sh) {
     double* __restrict__ ti = T + (int64_t)n * MC;
     for (int i = tid; i < ncut * MC; i += nthr) ti[i] = yv[i];
   }
+  int* const lvl = sh.lvl;
+  // level starts from `from` on; past the last level every level is empty
+  auto stage = [&](int from) {
+    for (int i = tid; i < kLvlStage + 2; i += nthr) lvl[i] = lvl_g[from + i <= nlev ? from + i : nlev];
+  };
+  stage(0);
   __syncthreads();
 
-  // R holds this lane's cell of the current level; Rn is loaded at the start
-  // of a level for the next one, so its latency overlaps the level's work.
-  CellRegs<KIN> R, Rn;
-  if (nlev > 0) load_cell<MODE, KIN>(R, lvl[0] + tid, lvl[1], n, nsrc, ncoef, b0, qv, dg, order, P.delta_t);
-
-  for (int k = 0; k < nlev; ++k) {
-    const int p0 = lvl[k];
-    const int p1 = lvl[k + 1];
-    if (k + 1 < nlev) load_cell<MODE, KIN>(Rn, p1 + tid, lvl[k + 2], n, nsrc, ncoef, b0, qv, dg, order, P.delta_t);
-    for (int base = p0; base < p1; base += nthr) {
-      CellRegs<KIN> Rx;
-      if (base != p0) load_cell<MODE, KIN>(Rx, base + tid, p1, n, nsrc, ncoef, b0, qv, dg, order, P.delta_t);
-      const CellRegs<KIN>& C = base == p0 ? R : Rx;
-      if (C.p >= 0) {
-        double pt = 0.0, pj0 = 0.0, pj1 = 0.0, pj2 = 0.0;
-        solve_cell<MC, MODE, KIN>(C, T, sh.lam, sh.act, sh.tco, sh.sig, dim, pt, pj0, pj1, pj2);
-        if (MODE == 2) {
-          // partial mode sums in sweep order (coalesced); the reduction maps
-          // positions back to cells
-          const int64_t o = (int64_t)lt * n + C.p;
-          P.part_t[o] = pt;
-          P.part_j[o * dim + 0] = pj0;
-          P.part_j[o * dim + 1] = pj1;
-          if (dim == 3) P.part_j[o * dim + 2] = pj2;
-        }
+  using Regs = LevelRegs<U, NP>;
+  double dtc[U];  // DeltaT of each cell (mode sums only)
+  // upwind gathers of the U cells of C (idle slots skip them)
+  auto gather = [&](const Regs& C, double2 (&g)[U][KIN]) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const double s01 = rec_get<GQ, NP, 6>(C.pc[u]);
+      const double s2c = rec_get<GQ, NP, 7>(C.pc[u]);
+      int sr[3] = {__double2loint(s01), __double2hiint(s01), __double2loint(s2c)};
+#pragma unroll
+      for (int kk = 0; kk < KIN; ++kk) {
+        g[u][kk] = make_double2(0.0, 0.0);
+        if (C.p[u] >= 0 && (MODE != 1 || act))
+          g[u][kk] = *reinterpret_cast<const double2*>(T + (int64_t)sr[kk] * MC + 2 * jl);
+      }
+      if (MODE == 2) {
+        dtc[u] = 0.0;
+        if (C.p[u] >= 0) dtc[u] = dtp[__double2hiint(s2c)];
       }
     }
-    R = Rn;
+  };
+  auto finish = [&](const Regs& C, const double2 (&g)[U][KIN]) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const double dd = rec_get<GQ, NP, 0>(C.pc[u]);
+      const double bb = MODE == 1 ? 0.0 : rec_get<GQ, NP, 1>(C.pc[u]);
+      const double qq = MODE == 1 ? 0.0 : rec_get<GQ, NP, 2>(C.pc[u]);
+      const double cf[3] = {rec_get<GQ, NP, 3>(C.pc[u]), rec_get<GQ, NP, 4>(C.pc[u]),
+                            KIN == 3 ? rec_get<GQ, NP, 5>(C.pc[u]) : 0.0};
+      double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+      for (int kk = 0; kk < KIN; ++kk) {
+        a0 += cf[kk] * g[u][kk].x;
+        a1 += cf[kk] * g[u][kk].y;
+      }
+      const double t0 = (bb + lam0 * (qq + a0)) / (1.0 + lam0 * dd);
+      const double t1 = (bb + lam1 * (qq + a1)) / (1.0 + lam1 * dd);
+      if (MODE == 2) {
+        // partial mode sums in sweep order; every lane of the cell holds the
+        // sums, lane v stores the v-th of (T, J_x, J_y[, J_z])
+        const double d0 = t0 - dtc[u], d1 = t1 - dtc[u];
+        const double pt = cell_sum<G>(tco0 * t0 + tco1 * t1);
+        const double px = cell_sum<G>(d0 * sx0 + d1 * sx1);
+        const double py = cell_sum<G>(d0 * sy0 + d1 * sy1);
+        const double pz = dim == 3 ? cell_sum<G>(d0 * sz0 + d1 * sz1) : 0.0;
+        if (C.p[u] >= 0) {
+          const int pp = C.p[u];
+#pragma unroll
+          for (int r = 0; r * G < 4; ++r) {
+            const int v = r * G + jl;
+            if (v <= dim) {
+              const double val = v == 0 ? pt : v == 1 ? px : v == 2 ? py : pz;
+              double* dst = v == 0 ? part_t + pp : part_j + (int64_t)pp * dim + (v - 1);
+              *dst = val;
+            }
+          }
+        }
+      }
+      if (C.p[u] >= 0 && (MODE != 1 || act))
+        *reinterpret_cast<double2*>(T + (int64_t)C.p[u] * MC + 2 * jl) = make_double2(t0, t1);
+    }
+  };
+
+  // One level (kk levels after the staged base) with the cells of C; the
+  // records of the level two further go into N2.  Issue order matters:
+  // vector-memory counters retire in issue order, so the gathers go out
+  // before the prefetch and the arithmetic waits for the gathers only.  The
+  // records of the next level were issued before the gathers of this one, so
+  // no wait on a prefetch is ever exposed.
+  auto body = [&](int kk, Regs& C, Regs& N2) {
+    const int p0 = lvl[kk];
+    const int p1 = lvl[kk + 1];
+    double2 g[U][KIN];
+    gather(C, g);
+    __builtin_amdgcn_sched_barrier(0);
+    load_level<U, NP>(N2, lvl[kk + 2] + cslot, lvl[kk + 3], cps, jq, rec);
+    __builtin_amdgcn_sched_barrier(0);
+    // everything but the U * NP prefetch loads has landed (an unconditional
+    // wait: the waits the compiler places sit in the per-cell branches)
+    __builtin_amdgcn_s_waitcnt(vmcnt_only(U * NP));
+    __builtin_amdgcn_sched_barrier(0);
+    finish(C, g);
+    // levels wider than one pass: C is spent, reload it per extra pass
+    for (int base = p0 + U * cps; base < p1; base += U * cps) {
+      load_level<U, NP>(C, base + cslot, p1, cps, jq, rec);
+      gather(C, g);
+      finish(C, g);
+    }
     __syncthreads();
+  };
+
+  // Three register sets rotate (current, next, prefetch target) without
+  // copies; the level count is padded to a multiple of 3 with empty levels so
+  // the unrolled loop has no exits.  Level starts are staged kSeg at a time.
+  constexpr int kSeg = (kLvlStage - 2) / 3 * 3;
+  const int nlev3 = (nlev + 2) / 3 * 3;
+  Regs A, B, C;
+  load_level<U, NP>(A, lvl[0] + cslot, lvl[1], cps, jq, rec);
+  load_level<U, NP>(B, lvl[1] + cslot, lvl[2], cps, jq, rec);
+  // A and B arrive before the loop, so the loop body never waits on them
+  __builtin_amdgcn_s_waitcnt(0);
+  for (int seg = 0; seg < nlev3; seg += kSeg) {
+    if (seg > 0) {
+      stage(seg);
+      __syncthreads();
+    }
+    const int kend = min(kSeg, nlev3 - seg);
+    for (int kk = 0; kk < kend; kk += 3) {
+      body(kk, A, C);
+      body(kk + 1, B, A);
+      body(kk + 2, C, B);
+    }
   }
 
   // exit values of the cut faces (upstream cells across the cut)
@@ -425,7 +534,7 @@ __device__ __forceinline__ int task_of_block(int b, int ntask) {
 
 // single problem: one task per block
 template <int MC, int MODE, int KIN>
-__global__ __launch_bounds__(kMaxBlock) void k_sweep(One src, int ntask) {
+__global__ __launch_bounds__(kSweepBlock) void k_sweep(One src, int ntask) {
   __shared__ SweepShared<MC> sh;
   const int t = task_of_block(blockIdx.x, ntask);
   if (t < 0) return;
@@ -437,7 +546,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_sweep(One src, int ntask) {
 // batch: persistent workgroups draining 8 XCD-local task queues (tasks of one
 // (problem, direction) share a queue; a drained queue steals from the next)
 template <int MC, int MODE, int KIN>
-__global__ __launch_bounds__(kMaxBlock) void k_sweep_batch(Many src, const int32_t* __restrict__ qoff,
+__global__ __launch_bounds__(kSweepBlock) void k_sweep_batch(Many src, const int32_t* __restrict__ qoff,
                                                            int32_t* __restrict__ qcnt) {
   __shared__ SweepShared<MC> sh;
   __shared__ int s_task;
@@ -835,6 +944,16 @@ __global__ void k_reduce_final(S src) {
   if (threadIdx.x == 0) *src.red(pid).norm2 = red[0];
 }
 
+// total active tasks of all problems -> *out (records a round of a captured iteration)
+__global__ void k_log_active(Many src, int n_problems, int32_t* out) {
+  __shared__ int tot;
+  if (threadIdx.x == 0) tot = 0;
+  __syncthreads();
+  for (int p = threadIdx.x; p < n_problems; p += blockDim.x) atomicAdd(&tot, *src.gcr(p).n_active);
+  __syncthreads();
+  if (threadIdx.x == 0) *out = tot;
+}
+
 // batch: DeltaT, TB <- the reduced values (pointers of the plans stay fixed)
 __global__ void k_commit(Many src) {
   const int pid = blockIdx.y;
@@ -853,8 +972,8 @@ int check_plan(const obte_bte_plan_t* P, const char* who) {
   if (P->mc != 4 && P->mc != 8 && P->mc != 16 && P->mc != 32 && P->mc != 64)
     return obte_fail(OBTE_EINVAL, "plan: mc must be 4, 8, 16, 32 or 64");
   if (P->n_chunks != (P->n_mfp + P->mc - 1) / P->mc) return obte_fail(OBTE_EINVAL, "plan: n_chunks != ceil(M/mc)");
-  if (P->block < 64 || P->block > kMaxBlock || (P->block & (P->block - 1)) != 0)
-    return obte_fail(OBTE_EINVAL, "plan: block must be a power of two in [64, 1024]");
+  if (P->block < 64 || P->block > kSweepBlock || (P->block & (P->block - 1)) != 0)
+    return obte_fail(OBTE_EINVAL, "plan: block must be 64, 128 or 256");
   if (P->kin != 2 && P->kin != 3) return obte_fail(OBTE_EINVAL, "plan: kin must be 2 or 3");
   if (P->ncut_max < 0) return obte_fail(OBTE_EINVAL, "plan: ncut_max < 0");
   if ((P->n + 255) / 256 > 4096) return obte_fail(OBTE_ELIMIT, "plan: more than 4096*256 cells (scratch size)");
@@ -863,7 +982,7 @@ int check_plan(const obte_bte_plan_t* P, const char* who) {
 
 int launch_status(const char* who) {
   hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return obte_fail(OBTE_EHIP, who);
+  if (e != hipSuccess) return obte_fail_hip(who, e);
   return OBTE_OK;
 }
 
@@ -933,8 +1052,8 @@ int check_batch(const obte_bte_batch_t* B, const char* who) {
     return obte_fail(OBTE_EINVAL, who);
   if (B->mc != 4 && B->mc != 8 && B->mc != 16 && B->mc != 32 && B->mc != 64)
     return obte_fail(OBTE_EINVAL, "batch: mc must be 4, 8, 16, 32 or 64");
-  if (B->block < 64 || B->block > kMaxBlock || (B->block & (B->block - 1)) != 0)
-    return obte_fail(OBTE_EINVAL, "batch: block must be a power of two in [64, 1024]");
+  if (B->block < 64 || B->block > kSweepBlock || (B->block & (B->block - 1)) != 0)
+    return obte_fail(OBTE_EINVAL, "batch: block must be 64, 128 or 256");
   if (B->kin != 2 && B->kin != 3) return obte_fail(OBTE_EINVAL, "batch: kin must be 2 or 3");
   if (B->groups <= 0 || B->max_elems <= 0 || B->max_dirs <= 0 || B->max_cells <= 0)
     return obte_fail(OBTE_EINVAL, "batch: invalid launch sizes");
@@ -1090,6 +1209,14 @@ int obte_batch_gcr_reset(const obte_bte_batch_t* b, int32_t use_delta_t, obte_st
   hipLaunchKernelGGL((k_gcr_reset<Many>), dim3(b->max_dirs, b->n_problems), dim3(256), 0, (hipStream_t)stream,
                      make_many(b, 0), (int)use_delta_t);
   return launch_status("obte_batch_gcr_reset: launch failed");
+}
+
+int obte_batch_log_active(const obte_bte_batch_t* b, int32_t* out, obte_stream_t stream) {
+  int rc = check_batch(b, "obte_batch_log_active: invalid batch");
+  if (rc) return rc;
+  if (!out) return obte_fail(OBTE_EINVAL, "obte_batch_log_active: null output");
+  hipLaunchKernelGGL(k_log_active, dim3(1), dim3(256), 0, (hipStream_t)stream, make_many(b, 0), b->n_problems, out);
+  return launch_status("obte_batch_log_active: launch failed");
 }
 
 int obte_batch_reduce(const obte_bte_batch_t* b, obte_stream_t stream) {

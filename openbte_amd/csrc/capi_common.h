@@ -10,3 +10,6 @@
 
 // Record `msg` as the calling thread's last error and return `code`.
 int obte_fail(int code, const char* msg);
+
+// Record `who: <HIP error string>` and return OBTE_EHIP.
+int obte_fail_hip(const char* who, int hip_error);

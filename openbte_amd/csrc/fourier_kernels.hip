@@ -299,7 +299,7 @@ inline unsigned nblocks(int64_t n) { return (unsigned)((n + kB - 1) / kB); }
 
 int status(const char* who) {
   hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return obte_fail(OBTE_EHIP, who);
+  if (e != hipSuccess) return obte_fail_hip(who, e);
   return OBTE_OK;
 }
 

@@ -55,3 +55,23 @@ def test_batch_different_iteration_counts():
         assert out.aux['n_iter'] == single.aux['n_iter']
         np.testing.assert_array_equal(out.variables['Temperature_BTE']['data'],
                                       single.variables['Temperature_BTE']['data'])
+
+
+def test_batch_graph_replay_matches_eager():
+    """hipGraph-replayed outer iterations give the eager results, replay after replay."""
+    from openbte_amd.batch import BTEBatch
+    from openbte_amd.operator import build_host_operator
+    probs = [_problem(31, 500), _problem(32, 700)]
+    ops = [build_host_operator(p['geo'], p['mat'], p['bcs'], p['effective_thermal_conductivity']) for p in probs]
+    t0 = [np.zeros(p['geo'].n_elems) for p in probs]
+    eager = BTEBatch(ops)
+    graph = BTEBatch(ops)
+    graph.enable_graphs(True)
+    eager.reset(t0)
+    graph.reset(t0)
+    for it in range(6):
+        eager.outer_iteration()
+        graph.outer_iteration()
+        for de, dg in zip(eager.devs, graph.devs):
+            np.testing.assert_array_equal(de.buf['dt'][0].cpu().numpy(), dg.buf['dt'][0].cpu().numpy())
+            np.testing.assert_array_equal(de.buf['kappa'].cpu().numpy(), dg.buf['kappa'].cpu().numpy())
