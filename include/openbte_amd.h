@@ -207,7 +207,8 @@ typedef struct obte_bte_batch {
   int32_t n_tasks;       /* sum over problems of n_dirs * n_chunks */
   int32_t mc;            /* shared launch shape (equal to every plan's) */
   int32_t block;
-  int32_t kin;
+  int32_t kin;           /* max inflow slots over the problems (a plan with fewer
+                            slots has zero coefficients in the rest of rec) */
   int32_t groups;        /* persistent workgroups of a sweep launch */
   int32_t max_elems;     /* max n_dirs * n over problems */
   int32_t max_dirs;      /* max n_dirs */

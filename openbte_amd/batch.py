@@ -36,29 +36,46 @@ def _struct_tensor(structs, dev) -> torch.Tensor:
 class BTEBatch:
     """A batch of problems sharing the launch shape (mc, block, kin)."""
 
-    def __init__(self, ops: Sequence, cfg: Optional[SweepConfig] = None, scheds: Optional[Sequence] = None,
-                 groups: Optional[int] = None, device=None):
+    def __init__(self, ops: Sequence = (), cfg: Optional[SweepConfig] = None, scheds: Optional[Sequence] = None,
+                 groups: Optional[int] = None, device=None, devices: Optional[Sequence[BTEDevice]] = None,
+                 n_active: Optional[torch.Tensor] = None, norm2: Optional[torch.Tensor] = None,
+                 slots: Optional[Sequence[int]] = None):
+        """Build the batch from host operators, or (``devices``) over existing
+        ``BTEDevice`` objects whose counters are views ``n_active[slots[p]]``
+        and ``norm2[slots[p]]`` of the given arrays (the serving loop keeps
+        problems resident and rebuilds only this view when its set changes)."""
         if not torch.cuda.is_available():
             raise RuntimeError('openbte_amd.BTEBatch needs a ROCm GPU; there is no CPU fallback')
         self.dev = device or torch.device('cuda', torch.cuda.current_device())
         self.cfg = cfg or SweepConfig()
-        P = len(ops)
+        if devices is None:
+            P = len(ops)
+            self.n_active = torch.zeros(P, dtype=torch.int32, device=self.dev)
+            self.norm2 = torch.zeros(P, dtype=torch.float64, device=self.dev)
+            self.slots = np.arange(P)
+            self.devs: List[BTEDevice] = []
+            for i, op in enumerate(ops):
+                d = BTEDevice(op, sched=None if scheds is None else scheds[i], cfg=SweepConfig(**vars(self.cfg)),
+                              device=self.dev, n_active=self.n_active[i:i + 1], norm2=self.norm2[i:i + 1])
+                self.devs.append(d)
+            assemble = True
+        else:
+            self.devs = list(devices)
+            P = len(self.devs)
+            self.n_active, self.norm2 = n_active, norm2
+            self.slots = np.asarray(slots if slots is not None else np.arange(P), dtype=np.int64)
+            for d in self.devs:
+                if (d.cfg.mc, d.cfg.block) != (self.cfg.mc, self.cfg.block):
+                    raise ValueError('every problem of a batch needs the batch (mc, block)')
+            assemble = False  # each device assembled its operator when it was built
         self.n_problems = P
-        self.n_active = torch.zeros(P, dtype=torch.int32, device=self.dev)
-        self.norm2 = torch.zeros(P, dtype=torch.float64, device=self.dev)
-        self.devs: List[BTEDevice] = []
-        for i, op in enumerate(ops):
-            d = BTEDevice(op, sched=None if scheds is None else scheds[i], cfg=SweepConfig(**vars(self.cfg)),
-                          device=self.dev, n_active=self.n_active[i:i + 1], norm2=self.norm2[i:i + 1])
-            self.devs.append(d)
-        kins = {d.kin for d in self.devs}
-        if len(kins) != 1:
-            raise ValueError('a batch needs one inflow-slot width (do not mix 2D and 3D problems)')
-        self.kin = kins.pop()
+        # a problem with fewer inflow slots carries zero coefficients in the
+        # rest of its records, so the widest launch serves every problem
+        self.kin = max(d.kin for d in self.devs)
         self.stream = torch.cuda.current_stream(self.dev).cuda_stream
         self.live = torch.ones(P, dtype=torch.int32, device=self.dev)
         self._build_tasks()
-        self._build_structs(groups)
+        self._build_structs(groups, assemble)
         self.n_iter = np.zeros(P, dtype=np.int64)
         self.error = np.ones(P)
         self.norm_old = np.zeros(P)
@@ -93,7 +110,7 @@ class BTEBatch:
         self.n_tasks = len(tp)
         self.queue_load = load
 
-    def _build_structs(self, groups):
+    def _build_structs(self, groups, assemble: bool = True):
         devs = self.devs
         P = self.n_problems
         for d in devs:
@@ -138,7 +155,8 @@ class BTEBatch:
         B.live = self.live.data_ptr()
         B.task_pid, B.task_local = self.task_pid.data_ptr(), self.task_local.data_ptr()
         B.queue_off, B.queue_cnt = self.queue_off.data_ptr(), self.queue_cnt.data_ptr()
-        _lib.call('obte_batch_assemble', self.b, self.stream)
+        if assemble:
+            _lib.call('obte_batch_assemble', self.b, self.stream)
 
     # -- iteration ---------------------------------------------------------
     def reset(self, delta_t0: Sequence[np.ndarray]) -> None:
@@ -240,7 +258,7 @@ class BTEBatch:
         live = np.ones(self.n_problems, bool)
         while live.any():
             self.outer_iteration()
-            nrm = np.sqrt(self.norm2.cpu().numpy())
+            nrm = np.sqrt(self.norm2.cpu().numpy()[self.slots])
             for p in np.nonzero(live)[0]:
                 self.n_iter[p] += 1
                 self.error[p] = abs(nrm[p] - self.norm_old[p]) / nrm[p]

@@ -62,6 +62,41 @@ def _t(a, dtype, dev):
     return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype).to(dev)
 
 
+def _kmax(cfg: SweepConfig, ny: int) -> int:
+    """Krylov basis length: cfg.kmax capped so that the two basis blocks
+    (u and c, kmax x ny doubles each) fit in cfg.basis_bytes."""
+    return int(max(8, min(cfg.kmax, cfg.basis_bytes // (16 * ny))))
+
+
+def device_bytes(op: HostOperator, sched: Schedule, cfg: Optional[SweepConfig] = None) -> int:
+    """Device memory (bytes) a ``BTEDevice`` of this operator allocates; the
+    serving loop admits a request only when this fits its HBM budget."""
+    cfg = cfg or SweepConfig()
+    n, L, M, dim = op.n, op.n_dirs, op.n_mfp, op.dim
+    mc = cfg.mc
+    nch = (M + mc - 1) // mc
+    n_cut = int(sched.cut_off[-1])
+    ncut_max = int(np.diff(sched.cut_off).max()) if L > 0 else 0
+    kin = max(2, max_inflow(sched))
+    n_wall = len(op.i_diffuse)
+    one = lambda k: max(1, k)  # noqa: E731  (empty host arrays are stored as one element)
+    i4 = (2 * L * n + len(sched.lvl_off) + len(sched.lvl_ptr) + 2 * L * kin * n + len(sched.cut_off)
+          + len(sched.cut_exit) + len(op.cf_ptr) + len(op.cf_face) + len(op.wall_ptr) + one(len(op.wall_side))
+          + one(n_wall) + one(len(op.ce1)) + one(len(op.ce2)))
+    f8 = (op.dirs.size + op.k_face.size + op.jump_face.size + max(dim, op.wall_db.size) + one(len(op.wall_temp))
+          + one(n_wall) + (op.wall_dbs.size if n_wall else dim) + max(dim, op.c_na.size) + op.mfps.size
+          + op.t_coeff.size + op.sigma.size + op.heat.size)
+    f8 += L * n + L * n * 8                                  # pert, rec
+    f8 += 2 * n + 2 * one(n_wall) + n * dim                  # dt, tb, j
+    f8 += L * nch * (n + ncut_max) * mc                      # t
+    f8 += L * nch * n * (1 + dim) + one(L * nch * n_wall)    # part_t, part_j, part_tb
+    f8 += 2 * M * L + 4096 + 8                               # kappa, supp, scratch
+    ny = max(1, nch * n_cut * mc)
+    f8 += 4 * ny + 2 * _kmax(cfg, ny) * ny + M * L           # y r x w, u c, rnorm
+    i4 += 2 * L * nch + L * nch * mc                         # k, active, lane_active
+    return int(8 * f8 + 4 * i4)
+
+
 class BTEDevice:
     """Device-resident operator, schedule and workspaces of one geometry and
     material.  Reusable across solves (warm state is reset by ``reset``)."""
@@ -139,7 +174,7 @@ class BTEDevice:
         b['scratch'] = torch.zeros(4096 + 8, dtype=f64, device=dev)
         # interface Krylov state
         ny = max(1, nch * self.n_cut * mc)
-        cfg.kmax = int(max(8, min(cfg.kmax, cfg.basis_bytes // (16 * ny))))
+        cfg.kmax = _kmax(cfg, ny)
         g = self.gbuf = {}
         g['y'] = torch.zeros(ny, dtype=f64, device=dev)
         g['r'] = torch.zeros(ny, dtype=f64, device=dev)
@@ -159,6 +194,12 @@ class BTEDevice:
         self.gcr = self._make_gcr()
         self.stream = torch.cuda.current_stream(self.dev).cuda_stream
         _lib.call('obte_bte_assemble', self.plan, self.stream)
+
+    def nbytes(self) -> int:
+        """Bytes of the device tensors this object owns (shared counters excluded)."""
+        ts = [t for v in self.buf.values() for t in (v if isinstance(v, list) else [v])]
+        ts += [t for k, t in self.gbuf.items() if k != 'n_active']
+        return int(sum(t.numel() * t.element_size() for t in ts))
 
     # -- C structs -------------------------------------------------------
     def _make_plan(self) -> _lib.BtePlan:

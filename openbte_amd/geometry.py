@@ -226,3 +226,79 @@ def random_pore_cell_2d(seed: int, n_cells: int, h_ref: float = 0.274, max_pores
         except (RuntimeError, ValueError):
             continue
     raise RuntimeError(f'could not place pores for seed {seed}')
+
+
+def _periodic_canonical(nodes: np.ndarray, L: float) -> np.ndarray:
+    """Index of the node each node maps to when x = +L/2 and y = +L/2 are
+    wrapped onto -L/2 (periodic partners share one canonical node)."""
+    tol = 1e-9 * L
+    wrapped = nodes.copy()
+    wrapped[np.abs(wrapped[:, 0] - L / 2) < tol, 0] -= L
+    wrapped[np.abs(wrapped[:, 1] - L / 2) < tol, 1] -= L
+    key = np.round(wrapped / (1e-7 * L)).astype(np.int64)
+    _, first, inv = np.unique(key, axis=0, return_index=True, return_inverse=True)
+    return first[inv.reshape(-1)]
+
+
+def _prism_tets(v: np.ndarray, key: np.ndarray) -> np.ndarray:
+    """Split prisms (bottom v0 v1 v2, top v3 v4 v5 with v[i+3] above v[i]) into
+    3 tetrahedra each.  Every quad face is cut along the diagonal through its
+    smallest-key vertex, so neighbouring prisms (and periodic partners, whose
+    keys agree) cut shared faces the same way (Dompierre et al., "How to
+    subdivide pyramids, prisms and hexahedra into tetrahedra", 1999)."""
+    # the 6 rotations / reflections that bring each vertex to position 0
+    perms = np.array([[0, 1, 2, 3, 4, 5], [1, 2, 0, 4, 5, 3], [2, 0, 1, 5, 3, 4],
+                      [3, 5, 4, 0, 2, 1], [4, 3, 5, 1, 0, 2], [5, 4, 3, 2, 1, 0]])
+    kv = key[v]
+    w = v[np.arange(len(v))[:, None], perms[np.argmin(kv, axis=1)]]
+    kw = key[w]
+    a = np.minimum(kw[:, 1], kw[:, 5]) < np.minimum(kw[:, 2], kw[:, 4])
+    t = np.empty((len(v), 3, 4), np.int64)
+    c0, c1, c2, c3, c4, c5 = (w[:, i] for i in range(6))
+    t[a, 0] = np.stack([c0, c1, c2, c5], 1)[a]
+    t[a, 1] = np.stack([c0, c1, c5, c4], 1)[a]
+    t[~a, 0] = np.stack([c0, c1, c2, c4], 1)[~a]
+    t[~a, 1] = np.stack([c0, c4, c2, c5], 1)[~a]
+    t[:, 2] = np.stack([c0, c4, c5, c3], 1)
+    return t.reshape(-1, 4)
+
+
+def pore_slab_3d(L: float = 10.0, H: float = 5.0, porosity: float = 0.25, n_cells: int = 20000,
+                 seed: int = 0, nz: Optional[int] = None) -> Mesh:
+    """Periodic porous slab: the ``pore_cell_2d`` cross-section (side ``L``
+    nm, centred cylindrical pore of the given porosity) extruded over a
+    thickness ``H`` in ``nz`` layers, each triangular prism split into 3
+    tetrahedra.  x and y are periodic (``Periodic_x``, ``Periodic_y``); the
+    pore wall and the two slab faces z = 0, z = H form the diffuse
+    ``Boundary``.  ``n_cells`` is the target tetrahedron count; by default the
+    layer thickness follows the in-plane mesh spacing."""
+    solid = L * L * (1.0 - porosity)
+    if nz is None:
+        # 3 n2 nz = n_cells with n2 = 2 solid / h^2 and nz = H / h
+        h = (6.0 * solid * H / n_cells) ** (1.0 / 3.0)
+        nz = max(1, int(round(H / h)))
+    n2 = max(16, int(round(n_cells / (3 * nz))))
+    base = pore_cell_2d(L, porosity, n2, seed=seed)
+    p2, t2 = base.nodes, base.elems
+    m = len(p2)
+    canon = _periodic_canonical(p2, L)
+    zs = np.linspace(0.0, H, nz + 1)
+    nodes = np.concatenate([np.concatenate([p2, np.full((m, 1), z)], 1) for z in zs])
+    key = (np.tile(canon, nz + 1) * (nz + 1) + np.repeat(np.arange(nz + 1), m)).astype(np.int64)
+    k = np.arange(nz)[:, None, None]
+    bottom = t2[None, :, :] + k * m
+    prisms = np.concatenate([bottom, bottom + m], axis=2).reshape(-1, 6)
+    tets = _prism_tets(prisms, key)
+
+    def classify(side_nodes, centroids):
+        tol = 1e-9 * max(L, H)
+        c = centroids
+        tag = np.full(len(c), 4, np.int64)
+        tag[np.abs(c[:, 0] + L / 2) < tol] = 0
+        tag[np.abs(c[:, 0] - L / 2) < tol] = 1
+        tag[np.abs(c[:, 1] + L / 2) < tol] = 2
+        tag[np.abs(c[:, 1] - L / 2) < tol] = 3
+        return tag
+
+    return build_mesh(nodes, tets, classify, _REGIONS,
+                      periodic={'Periodic_x': np.array([L, 0.0, 0.0]), 'Periodic_y': np.array([0.0, L, 0.0])})

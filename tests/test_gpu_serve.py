@@ -1,0 +1,109 @@
+"""Continuous-batching server: every result equals the result of the same
+solve alone (same launch shape), whatever else shares the device; admission
+respects the HBM budget; the footprint estimate is exact."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _problems():
+    from oracle import fourier_oracle as fo
+    from openbte_amd.geometry import pore_cell_2d, pore_slab_3d
+    from openbte_amd.material import synthetic_material_2d, synthetic_material_3d
+    from openbte_amd.objects import BoundaryConditions, EffectiveThermalConductivity
+    bcs = BoundaryConditions(periodic={'Periodic_x': 1.0, 'Periodic_y': 0.0}, diffuse='Boundary')
+    out = []
+    for i, (n, phi, M) in enumerate([(500, 0.2, 6), (900, 0.3, 7), (400, 0.1, 5)]):
+        geo = pore_cell_2d(10.0, phi, n, seed=10 + i)
+        mat = synthetic_material_2d(M, 8, 1.0, 1.0e4)
+        eff = EffectiveThermalConductivity(contact='Periodic_x', normalization=-1.0)
+        out.append((geo, mat, bcs, eff))
+    geo = pore_slab_3d(10.0, 3.0, 0.2, 900, seed=2, nz=3)
+    mat = synthetic_material_3d(5, 2, 4, 1.0, 1e3)
+    eff = EffectiveThermalConductivity(contact='Periodic_x', normalization=-1.0 / 3.0)
+    out.append((geo, mat, bcs, eff))
+    fs = [fo.fourier(g, m.thermal_conductivity, b, e) for g, m, b, e in out]
+    return out, fs
+
+
+@pytest.fixture(scope='module')
+def alone():
+    from openbte_amd.bte import BTE_RTA, SweepConfig
+    probs, fs = _problems()
+    cfg = SweepConfig(mc=8, block=128)
+    res = [BTE_RTA(g, m, b, fourier=f, effective_thermal_conductivity=e, bte_max_iter=12, bte_rtol=1e-6,
+                   config=SweepConfig(**vars(cfg)))
+           for (g, m, b, e), f in zip(probs, fs)]
+    return probs, fs, cfg, res
+
+
+def _same(a, b):
+    assert a.aux['n_iter'] == b.aux['n_iter']
+    assert abs(a.kappa_eff - b.kappa_eff) <= 1e-13 * abs(b.kappa_eff)
+    dT = np.abs(a.variables['Temperature_BTE']['data'] - b.variables['Temperature_BTE']['data']).max()
+    assert dT <= 1e-13
+    dJ = np.abs(a.variables['Flux_BTE']['data'] - b.variables['Flux_BTE']['data']).max()
+    assert dJ <= 1e-12 * np.abs(b.variables['Flux_BTE']['data']).max()
+
+
+def test_served_results_equal_alone(alone):
+    from openbte_amd.bte import SweepConfig
+    from openbte_amd.serve import BTEServer, Request
+    probs, fs, cfg, ref = alone
+    srv = BTEServer(cfg=SweepConfig(**vars(cfg)), max_slots=8)
+    reqs = [Request(g, m, b, e, fourier=f, bte_max_iter=12, bte_rtol=1e-6) for (g, m, b, e), f in zip(probs, fs)]
+    # staggered arrivals: two at once, the others while those run
+    srv.submit(reqs[0])
+    srv.submit(reqs[1])
+    while len(srv.live) < 2:
+        srv.step()
+    srv.step()
+    srv.submit(reqs[2])
+    srv.step()
+    srv.submit(reqs[3])
+    srv.run_until_idle()
+    srv.close()
+    assert srv.stats.peak_live >= 3
+    for r, a in zip(reqs, ref):
+        assert r.error is None
+        print('served', r.id, r.result.kappa_eff, 'alone', a.kappa_eff, 'latency', r.latency)
+        _same(r.result, a)
+
+
+def test_admission_by_footprint(alone):
+    from openbte_amd.bte import SweepConfig, device_bytes
+    from openbte_amd.operator import build_host_operator, build_schedule
+    from openbte_amd.serve import BTEServer, Request
+    probs, fs, cfg, ref = alone
+    sizes = []
+    for g, m, b, e in probs:
+        op = build_host_operator(g, m, b, e)
+        sizes.append(device_bytes(op, build_schedule(op), cfg))
+    budget = max(sizes) + min(sizes) // 2  # never two of the larger problems at once
+    srv = BTEServer(cfg=SweepConfig(**vars(cfg)), hbm_budget=budget, max_slots=8)
+    reqs = [srv.submit(Request(g, m, b, e, fourier=f, bte_max_iter=12, bte_rtol=1e-6))
+            for (g, m, b, e), f in zip(probs, fs)]
+    srv.run_until_idle()
+    srv.close()
+    assert srv.stats.peak_bytes <= budget
+    for r, a in zip(reqs, ref):
+        _same(r.result, a)
+    # a request larger than the whole budget fails alone, the server goes on
+    small = BTEServer(cfg=SweepConfig(**vars(cfg)), hbm_budget=min(sizes) - 1, max_slots=2)
+    g, m, b, e = probs[0]
+    r = small.submit(Request(g, m, b, e, fourier=fs[0]))
+    small.run_until_idle()
+    small.close()
+    assert isinstance(r.error, MemoryError) and r.result is None
+
+
+def test_footprint_estimate_is_exact(alone):
+    from openbte_amd.bte import BTEDevice, SweepConfig, device_bytes
+    from openbte_amd.operator import build_host_operator, build_schedule
+    probs, fs, cfg, ref = alone
+    for g, m, b, e in probs:
+        op = build_host_operator(g, m, b, e)
+        sc = build_schedule(op)
+        d = BTEDevice(op, sched=sc, cfg=SweepConfig(**vars(cfg)))
+        assert device_bytes(op, sc, cfg) == d.nbytes()

@@ -3,7 +3,7 @@ sweep scheduler."""
 import numpy as np
 import pytest
 
-from openbte_amd.geometry import pore_cell_2d, quad_cell_2d, structured_tri_cell_2d
+from openbte_amd.geometry import pore_cell_2d, pore_slab_3d, quad_cell_2d, structured_tri_cell_2d
 from openbte_amd.material import synthetic_material_2d
 from openbte_amd.objects import BoundaryConditions
 from openbte_amd.operator import build_host_operator, build_schedule, fixed_width_inflow, max_inflow
@@ -12,12 +12,13 @@ from openbte_amd.operator import build_host_operator, build_schedule, fixed_widt
 @pytest.mark.parametrize("geo", [pore_cell_2d(10.0, 0.25, 800, seed=1), quad_cell_2d(10.0, 6),
                                  structured_tri_cell_2d(10.0, 8),
                                  pore_cell_2d(20.0, 0.3, 1500, seed=2,
-                                              pores=[('circle', (-4.0, -3.0), 3.0), ('square', (4.0, 4.0), 2.5)])])
+                                              pores=[('circle', (-4.0, -3.0), 3.0), ('square', (4.0, 4.0), 2.5)]),
+                                 pore_slab_3d(10.0, 3.0, 0.2, 900, seed=2, nz=3)])
 def test_mesh_contract(geo):
     sem = geo.side_elem_map
     na = geo.normal_areas
     # closed cells: sum of outward normal areas is zero
-    S = np.zeros((geo.n_elems, 2))
+    S = np.zeros((geo.n_elems, geo.dim))
     act = np.concatenate([geo.internal, geo.boundary_sides])
     np.add.at(S, sem[act, 0], na[act])
     np.add.at(S, sem[geo.internal, 1], -na[geo.internal])
@@ -88,3 +89,21 @@ def test_scheduler_breaks_cycles():
     assert sc.cut_off[-1] == 1
     assert sorted(sc.order.tolist()) == [0, 1, 2]
     assert sc.n_levels[0] == 3
+
+
+def test_slab_3d_volume_and_regions():
+    """Extruded tetrahedral slab: the tets tile the prism layers exactly, so
+    the volume is (L^2 - pore polygon area) H; x and y sides pair up."""
+    L, H, phi = 10.0, 3.0, 0.2
+    geo = pore_slab_3d(L, H, phi, 900, seed=2, nz=3)
+    base = pore_cell_2d(L, phi, int(round(900 / 9)), seed=2)
+    assert geo.n_elems == 3 * 3 * base.n_elems
+    assert abs(geo.elem_volumes.sum() - base.elem_volumes.sum() * H) < 1e-10 * L * L * H
+    assert geo.dim == 3 and geo.elems.shape[1] == 4
+    assert set(geo.periodic_sides) == {'Periodic_x', 'Periodic_y'}
+    wall = geo.side_physical_regions['Boundary']
+    z = geo.side_centroids[wall, 2]
+    faces_z = np.isclose(z, 0.0) | np.isclose(z, H)
+    # top and bottom faces: two triangles per base triangle each
+    assert faces_z.sum() == 2 * base.n_elems
+    assert np.allclose(geo.side_areas[wall][faces_z].sum(), 2 * base.elem_volumes.sum())
