@@ -38,6 +38,11 @@ BATCH_CONFIGS = {
     'batch64': (64, 1000, 80000, 30, 48, 1.0e4, 2026),
 }
 
+SERVE_CONFIGS = {
+    # name: (requests, 2D cells range, 3D tets range, MFP bins range, 2D dirs, 3D (theta, phi), 3D share, seed)
+    'serve': (24, (1000, 200_000), (5000, 150_000), (20, 60), 48, (8, 8), 0.4, 7),
+}
+
 CONFIGS = {
     # name: (L nm, porosity, cells, n_mfp, n_dirs, mfp_max nm)
     'plain2k': (10.0, 0.25, 2000, 30, 48, 1.0e4),
@@ -203,12 +208,151 @@ def run_batch(args):
     return 0
 
 
+def make_requests(name):
+    """Seeded request mix: 2D pore cells and 3D porous slabs (tetrahedra)."""
+    from openbte_amd.geometry import pore_cell_2d, pore_slab_3d
+    from openbte_amd.material import synthetic_material_2d, synthetic_material_3d
+    from openbte_amd.objects import BoundaryConditions, EffectiveThermalConductivity
+    N, r2, r3, rm, nphi, (nth, nph), share3, seed = SERVE_CONFIGS[name]
+    rng = np.random.default_rng(seed)
+    bcs = BoundaryConditions(periodic={'Periodic_x': 1.0, 'Periodic_y': 0.0}, diffuse='Boundary')
+    out = []
+    for i in range(N):
+        M = int(rng.integers(rm[0], rm[1] + 1))
+        phi = float(rng.uniform(0.05, 0.35))
+        if rng.random() < share3:
+            n = int(np.exp(rng.uniform(np.log(r3[0]), np.log(r3[1]))))
+            L = float(np.sqrt(n * 0.45 ** 3 / (1 - phi) / 4.0) ** (2 / 3) * 4.0 ** (1 / 3))
+            H = L / 4.0
+            geo = pore_slab_3d(L, H, phi, n, seed=seed + i)
+            mat = synthetic_material_3d(M, nth, nph, 1.0, 1.0e4, seed=seed)
+            eff = EffectiveThermalConductivity(contact='Periodic_x', normalization=-1.0 / H)
+        else:
+            n = int(np.exp(rng.uniform(np.log(r2[0]), np.log(r2[1]))))
+            L = float(np.sqrt(n * 0.274 ** 2 / (2.0 * (1.0 - phi))))
+            geo = pore_cell_2d(L, phi, n, seed=seed + i)
+            mat = synthetic_material_2d(M, nphi, 1.0, 1.0e4, seed=seed)
+            eff = EffectiveThermalConductivity(contact='Periodic_x', normalization=-1.0)
+        out.append((geo, mat, bcs, eff))
+    return out
+
+
+def run_serve(args):
+    """Continuous-batching server vs the plain one-solve-at-a-time GPU form
+    under the same open-loop Poisson request stream (lambda = 70% of the
+    server's measured saturated capacity)."""
+    import torch
+    from openbte_amd.bte import BTE_RTA, SweepConfig
+    from openbte_amd.fourier import Fourier
+    from openbte_amd.serve import BTEServer, Request
+
+    t0 = time.perf_counter()
+    probs = make_requests(args.config)
+    N = len(probs)
+    sizes = [(g.n_elems, g.dim, m.t_coeff.shape[0], m.t_coeff.shape[1]) for g, m, _, _ in probs]
+    print(f'# {args.config}: {N} requests generated in {time.perf_counter() - t0:.1f}s; '
+          f'cells {min(s[0] for s in sizes)}-{max(s[0] for s in sizes)}, '
+          f'{sum(s[1] == 3 for s in sizes)} 3D', file=sys.stderr, flush=True)
+    cfg = SweepConfig()
+    if args.mc:
+        cfg.mc = args.mc
+    if args.block:
+        cfg.block = args.block
+    rtol, maxit = 1e-4, 100
+
+    # warm-up: one small solve of each kind (kernel loading, allocator)
+    for g, m, b, e in sorted(probs, key=lambda p: p[0].n_elems)[:1] + [p for p in probs if p[0].dim == 3][:1]:
+        f = Fourier(g, m.thermal_conductivity, b, e, verbose=False)
+        BTE_RTA(g, m, b, fourier=f, effective_thermal_conductivity=e, bte_max_iter=2,
+                config=SweepConfig(**vars(cfg)))
+    torch.cuda.synchronize()
+
+    # plain form: each request alone (Fourier guess + BTE solve), one after the other
+    service, plain_res, updates = [], [], []
+    for g, m, b, e in probs:
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        f = Fourier(g, m.thermal_conductivity, b, e, verbose=False)
+        r = BTE_RTA(g, m, b, fourier=f, effective_thermal_conductivity=e, bte_max_iter=maxit, bte_rtol=rtol,
+                    config=SweepConfig(**vars(cfg)))
+        torch.cuda.synchronize()
+        service.append(time.perf_counter() - ts)
+        plain_res.append(r)
+        M, L = m.t_coeff.shape
+        updates.append(g.n_elems * M * L * r.aux['n_iter'])
+    service = np.array(service)
+    total_updates = float(sum(updates))
+    print(f'# plain: {service.sum():.2f}s for {N} requests, {total_updates / service.sum():.3e} updates/s',
+          file=sys.stderr, flush=True)
+
+    def serve(arrivals):
+        srv = BTEServer(cfg=SweepConfig(**vars(cfg)), max_slots=64, prep_workers=4)
+        reqs = [Request(g, m, b, e, fourier='auto', bte_max_iter=maxit, bte_rtol=rtol) for g, m, b, e in probs]
+        order = np.argsort(arrivals, kind='stable')
+        start = time.perf_counter()
+        k = 0
+        while k < N or srv.pending():
+            now = time.perf_counter() - start
+            while k < N and arrivals[order[k]] <= now:
+                srv.submit(reqs[order[k]])
+                k += 1
+            got = srv.step()
+            if not srv.live and not got and k < N:
+                time.sleep(min(0.002, max(0.0, arrivals[order[k]] - (time.perf_counter() - start))))
+        makespan = time.perf_counter() - start
+        srv.close()
+        return reqs, makespan, srv.stats
+
+    # saturated capacity: every request at t = 0
+    reqs_sat, span_sat, st_sat = serve(np.zeros(N))
+    capacity = N / span_sat
+    print(f'# served saturated: {span_sat:.2f}s, capacity {capacity:.2f} req/s, peak live {st_sat.peak_live}',
+          file=sys.stderr, flush=True)
+    # open loop at 70% of that capacity
+    lam = 0.7 * capacity
+    arrivals = np.cumsum(np.random.default_rng(123).exponential(1.0 / lam, N))
+    reqs, span, st = serve(arrivals)
+    lat = np.array([r.latency for r in reqs])
+    # the plain form under the same arrivals: a FIFO queue in front of one solver
+    end, plat = 0.0, []
+    for a, s_ in zip(arrivals, service):
+        end = max(end, a) + s_
+        plat.append(end - a)
+    plat = np.array(plat)
+    kdiff = max(abs(r.result.kappa_eff / p.kappa_eff - 1) for r, p in zip(reqs, plain_res))
+    print(f'# served open loop: lambda {lam:.2f}/s, p50 {np.percentile(lat, 50):.3f}s p99 '
+          f'{np.percentile(lat, 99):.3f}s; plain p50 {np.percentile(plat, 50):.3f}s p99 '
+          f'{np.percentile(plat, 99):.3f}s; max kappa diff {kdiff:.2e}', file=sys.stderr, flush=True)
+    value = total_updates / span_sat
+    out = {
+        'metric': 'cell*mode updates/s (served, saturated) + p50/p99 request latency at 70% load',
+        'value': value, 'unit': 'cell*mode updates/s', 'n_gpus': 1, 'steps': N, 'warmup': 2,
+        'ms_per_step': span_sat / N * 1e3, 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': 'f64', 'data': 'synthetic (seeded 2D pore cells and 3D porous slabs, seeded Si-like material)',
+        'config': {'workload': f'{args.config}: {N} requests, 2D {SERVE_CONFIGS[args.config][1]} cells x 48 '
+                               f'directions, 3D {SERVE_CONFIGS[args.config][2]} tets x 64 directions, '
+                               f'{SERVE_CONFIGS[args.config][3]} MFP bins, bte_rtol {rtol}',
+                   'mc': cfg.mc, 'block': cfg.block,
+                   'plain_value': total_updates / service.sum(),
+                   'served_over_plain': service.sum() / span_sat,
+                   'lambda_per_s': lam,
+                   'served_p50_s': float(np.percentile(lat, 50)), 'served_p99_s': float(np.percentile(lat, 99)),
+                   'plain_p50_s': float(np.percentile(plat, 50)), 'plain_p99_s': float(np.percentile(plat, 99)),
+                   'open_loop_updates_per_s': total_updates / span,
+                   'peak_live': st_sat.peak_live, 'peak_bytes': st_sat.peak_bytes,
+                   'max_rel_kappa_diff_served_vs_alone': kdiff},
+    }
+    print(json.dumps(out))
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=20)
     ap.add_argument('--warmup', type=int, default=2)
-    ap.add_argument('--config', default='plain200k', choices=sorted(CONFIGS) + sorted(BATCH_CONFIGS))
+    ap.add_argument('--config', default='plain200k',
+                    choices=sorted(CONFIGS) + sorted(BATCH_CONFIGS) + sorted(SERVE_CONFIGS))
     ap.add_argument('--no-graphs', action='store_true', help='batched configs: launch without hipGraphs')
     ap.add_argument('--mc', type=int, default=None)
     ap.add_argument('--block', type=int, default=None)
@@ -226,6 +370,8 @@ def main(argv=None):
 
     if args.config in BATCH_CONFIGS:
         return run_batch(args)
+    if args.config in SERVE_CONFIGS:
+        return run_serve(args)
 
     import torch
     from openbte_amd.bte import BTEDevice, SweepConfig
