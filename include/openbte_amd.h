@@ -89,15 +89,23 @@ typedef struct obte_bte_plan {
   const int32_t* cf_ptr;   /* [n+1] directed faces whose row is cell c */
   const int32_t* cf_face;
   const int32_t* wall_ptr; /* [n+1] wall entries of cell c (diffuse and mixed) */
-  const int32_t* wall_side;/* diffuse index (>= 0) or -1 for a mixed wall */
+  const int32_t* wall_side;/* diffuse index (>= 0), -1 for a mixed wall, or -2 - k for the
+                              reflecting side k of a partially transmitting interface
+                              (diffuse index k, no outflow into D_l: the interface face
+                              already counts it; wall_temp then holds 1 - t) */
   const double* wall_db;   /* [n_wallent][dim] n A / V */
-  const double* wall_temp; /* [n_wallent] T_wall of mixed walls (0 for diffuse) */
+  const double* wall_temp; /* [n_wallent] T_wall of mixed walls, 1 - t of interface
+                              sides, 0 for diffuse walls */
   const int32_t* wall_cell;/* [n_wall] cell of each diffuse side */
   const double* wall_rinv; /* [n_wall] 1 / sum_ml min(sigma.db, 0) (0 if the sum is 0) */
   const double* wall_dbs;  /* [n_wall][dim] db of each diffuse side */
   const int32_t* ce1;      /* [n_contact] */
   const int32_t* ce2;      /* [n_contact] (periodic contact only) */
   const double* c_na;      /* [n_contact][dim] normal areas of the contact */
+  const double* c_t;       /* [n_contact] transmission of each contact side (NULL: 1) */
+  const double* t_face;    /* [n_faces] transmission of each directed face (NULL: 1);
+                              inflow coefficients are t * -min(s . k, 0); jump_face is
+                              expected pre-multiplied by t */
   const double* mfps;      /* [M] Lambda_m (nm) */
   const double* t_coeff;   /* [M][L] */
   const double* sigma;     /* [M][L][dim] */
@@ -292,6 +300,21 @@ int obte_fourier_correction(const obte_fourier_plan_t* plan, const double* grad,
 
 /* Diffusive kappa of T through the contact into out (device, 1 double). */
 int obte_fourier_kappa(const obte_fourier_plan_t* plan, const double* t, double* out, obte_stream_t stream);
+
+/* ---- differentiation (interfacial-transmission model) -------------------
+ *
+ * Replaces the reverse-mode gradient of the reference's JAX inverse solver
+ * (openbte/inverse/bte.py:35-76 compute_kappa_and_gradient, via the custom
+ * vjp of openbte/inverse/matinverse.py:647-662).  For every directed face f
+ * (row cell i_face[f], upwind cell j_face[f], reflecting side side_of_face[f]
+ * or -1) writes grad[f] = d kappa_eff / d t_f without the contact term:
+ *   -V_i sum_l min(s_l . k_f, 0) sum_m t_ml Lambda_m mu'[lbar(l)][m][i] (T[l][m][j] - jump0_f - tb_k)
+ * t_fwd / t_adj: solution workspaces (layout of plan->t) of the converged
+ * forward solve and of the adjoint solve; lbar: reversed direction of each l. */
+int obte_tim_face_grad(const obte_bte_plan_t* plan, const double* t_fwd, const double* t_adj, const int32_t* lbar,
+                       const int32_t* i_face, const int32_t* j_face, const int32_t* side_of_face, const double* tb,
+                       const double* jump0, const double* vol, int32_t n_faces, double* grad,
+                       obte_stream_t stream);
 
 #ifdef __cplusplus
 }

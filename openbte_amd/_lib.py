@@ -27,7 +27,8 @@ class BtePlan(C.Structure):
         ('nb_src', _p), ('nb_face', _p), ('cut_off', _p), ('cut_exit', _p),
         ('dirs', _p), ('k_face', _p), ('jump_face', _p), ('cf_ptr', _p), ('cf_face', _p),
         ('wall_ptr', _p), ('wall_side', _p), ('wall_db', _p), ('wall_temp', _p), ('wall_cell', _p),
-        ('wall_rinv', _p), ('wall_dbs', _p), ('ce1', _p), ('ce2', _p), ('c_na', _p), ('mfps', _p),
+        ('wall_rinv', _p), ('wall_dbs', _p), ('ce1', _p), ('ce2', _p), ('c_na', _p), ('c_t', _p), ('t_face', _p),
+        ('mfps', _p),
         ('t_coeff', _p), ('sigma', _p), ('heat', _p), ('nano', _f64),
         ('pert', _p), ('rec', _p),
         ('delta_t', _p), ('tb', _p), ('t', _p), ('part_t', _p), ('part_j', _p), ('part_tb', _p),
@@ -96,6 +97,7 @@ _SIGNATURES = {
     'obte_batch_gcr_reset': [C.POINTER(Batch), _i32, _p],
     'obte_batch_reduce': [C.POINTER(Batch), _p],
     'obte_batch_log_active': [C.POINTER(Batch), _p, _p],
+    'obte_tim_face_grad': [C.POINTER(BtePlan), _p, _p, _p, _p, _p, _p, _p, _p, _p, C.c_int32, _p, _p],
     'obte_bte_reduce': [C.POINTER(BtePlan), _p, _p, _p, _p, _p],
     'obte_fourier_solve': [C.POINTER(FourierPlan), _p, _p, _p, _p, _p, _f64, _i32, _i32,
                            C.POINTER(_i32), C.POINTER(_f64), _p],

@@ -86,6 +86,8 @@ def device_bytes(op: HostOperator, sched: Schedule, cfg: Optional[SweepConfig] =
     f8 = (op.dirs.size + op.k_face.size + op.jump_face.size + max(dim, op.wall_db.size) + one(len(op.wall_temp))
           + one(n_wall) + (op.wall_dbs.size if n_wall else dim) + max(dim, op.c_na.size) + op.mfps.size
           + op.t_coeff.size + op.sigma.size + op.heat.size)
+    if op.t_face is not None:
+        f8 += len(op.t_face) + max(1, len(op.c_t))          # t_face, c_t
     f8 += L * n + L * n * 8                                  # pert, rec
     f8 += 2 * n + 2 * one(n_wall) + n * dim                  # dt, tb, j
     f8 += L * nch * (n + ncut_max) * mc                      # t
@@ -144,7 +146,13 @@ class BTEDevice:
         b['cf_ptr'] = _t(op.cf_ptr, i32, dev)
         b['cf_face'] = _t(op.cf_face, i32, dev)
         b['wall_ptr'] = _t(op.wall_ptr, i32, dev)
-        b['wall_side'] = _t(np.maximum(op.wall_side, -1) if len(op.wall_side) else np.zeros(1), i32, dev)
+        ws = np.asarray(op.wall_side, np.int64)
+        # diffuse index, -1 (isothermal) or -2 - k (reflecting interface side k): the
+        # source kernel indexes TB with these, so check them here
+        k = np.where(ws >= 0, ws, -2 - ws)
+        if len(ws) and (np.any(k[ws != -1] >= max(n_wall, 0)) or np.any(k[ws != -1] < 0)):
+            raise ValueError('wall entry refers to a diffuse side out of range')
+        b['wall_side'] = _t(ws if len(ws) else np.zeros(1), i32, dev)
         b['wall_db'] = _t(op.wall_db if len(op.wall_db) else np.zeros((1, dim)), f64, dev)
         b['wall_temp'] = _t(op.wall_temp if len(op.wall_temp) else np.zeros(1), f64, dev)
         b['wall_cell'] = _t(op.i_diffuse if n_wall else np.zeros(1), i32, dev)
@@ -157,6 +165,9 @@ class BTEDevice:
         b['t_coeff'] = _t(op.t_coeff, f64, dev)
         b['sigma'] = _t(op.sigma, f64, dev)
         b['heat'] = _t(op.heat, f64, dev)
+        if op.t_face is not None:
+            b['t_face'] = _t(op.t_face, f64, dev)
+            b['c_t'] = _t(op.c_t if len(op.c_t) else np.zeros(1), f64, dev)
         # assembled operator
         b['pert'] = torch.zeros(L * n, dtype=f64, device=dev)
         b['rec'] = torch.zeros(L * n * 8, dtype=f64, device=dev)
@@ -217,6 +228,8 @@ class BTEDevice:
                      'part_tb', 'kappa', 'supp', 'scratch'):
             setattr(P, name, b[name].data_ptr())
         P.nano = op.nano
+        if 't_face' in b:
+            P.t_face, P.c_t = b['t_face'].data_ptr(), b['c_t'].data_ptr()
         P.delta_t = b['dt'][self.cur].data_ptr()
         P.tb = b['tb'][self.cur].data_ptr()
         return P

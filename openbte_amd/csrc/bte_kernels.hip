@@ -113,10 +113,14 @@ __global__ void k_assemble(S src) {
     else pert += P.jump_face[f] * g;
   }
   // diffuse and isothermal walls: outflow into D, isothermal inflow into pert
+  // (interface reflecting sides, wall_side <= -2, add neither: their outflow is
+  // the interface face's, their inflow is a source term)
   for (int w = P.wall_ptr[c]; w < P.wall_ptr[c + 1]; ++w) {
+    const int sd = P.wall_side[w];
+    if (sd <= -2) continue;
     const double g = dotv(s, P.wall_db + (int64_t)w * dim, dim);
     if (g > 0.0) D += g;
-    else if (P.wall_side[w] < 0) pert -= g * P.wall_temp[w];
+    else if (sd == -1) pert -= g * P.wall_temp[w];
   }
   if (P.nano != 0.0 && dim == 3) pert -= s[2];
   P.pert[gid] = pert;
@@ -129,6 +133,7 @@ __global__ void k_assemble(S src) {
     if (f >= 0) {
       const double g = dotv(s, P.k_face + (int64_t)f * dim, dim);
       cf[kk] = g < 0.0 ? -g : 0.0;
+      if (P.t_face != nullptr) cf[kk] *= P.t_face[f];
     }
   }
   double2* r = reinterpret_cast<double2*>(P.rec + gid * kRec);
@@ -139,6 +144,7 @@ __global__ void k_assemble(S src) {
 }
 
 // q = pert + sum over diffuse walls of (-TB_s) min(s . db, 0)   (openbte/bte.py:199-200,214)
+//     + sum over interface reflecting sides of (1 - t) (-TB_k) min(s . db, 0)
 // b0 = DeltaT + heat, both in position order
 template <class S>
 __global__ void k_source(S src) {
@@ -155,9 +161,12 @@ __global__ void k_source(S src) {
   double q = P.pert[gid];
   for (int w = P.wall_ptr[c]; w < P.wall_ptr[c + 1]; ++w) {
     const int sd = P.wall_side[w];
-    if (sd < 0) continue;
+    if (sd == -1) continue;
     const double g = dotv(s, P.wall_db + (int64_t)w * dim, dim);
-    if (g < 0.0) q += -P.tb[sd] * g;
+    if (g < 0.0) {
+      if (sd >= 0) q += -P.tb[sd] * g;
+      else q += -P.wall_temp[w] * P.tb[-2 - sd] * g;
+    }
   }
   P.rec[gid * kRec + 1] = P.delta_t[c] + P.heat[c];
   P.rec[gid * kRec + 2] = q;
@@ -489,8 +498,9 @@ __device__ void sweep_task(const S& src, int pid, int lt, SweepShared<MC>& sh) {
           const double t1 = T[(int64_t)pos[P.ce1[sc]] * MC + m];
           const double snp = sn > 0.0 ? sn : 0.0;
           const double snm = sn < 0.0 ? sn : 0.0;
-          kap += t1 * snp + P.contact_jump * snm;
-          if (P.contact_kind == 1) kap += T[(int64_t)pos[P.ce2[sc]] * MC + m] * snm;
+          double kc = t1 * snp + P.contact_jump * snm;
+          if (P.contact_kind == 1) kc += T[(int64_t)pos[P.ce2[sc]] * MC + m] * snm;
+          kap += P.c_t != nullptr ? P.c_t[sc] * kc : kc;
           sup += t1 * sqrt(area2);
         }
         // fixed-order block reductions

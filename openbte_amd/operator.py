@@ -53,6 +53,9 @@ class HostOperator(NamedTuple):
     sigma: np.ndarray
     heat: np.ndarray
     nano: float
+    # partially transmitting interfaces (None: every face transmits fully)
+    t_face: Optional[np.ndarray] = None   # (F,) transmission of each directed face
+    c_t: Optional[np.ndarray] = None      # (n_contact,) transmission of each contact side
 
 
 class Schedule(NamedTuple):

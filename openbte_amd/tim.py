@@ -1,0 +1,295 @@
+"""Differentiable kappa_eff of pixel densities (interfacial-transmission model).
+
+Model (the reference's inverse-design interpolation, openbte/inverse/bte.py:
+150-176): pixel densities rho in [0, 1] map to a transmission per side
+between two different pixels,
+
+    rho~ = k0 + rho (1 - k0),  k0 = 1e-12,   t = 2 rho~_a rho~_b / (rho~_a + rho~_b),
+
+and t = 1 inside a pixel.  Across such a side a cell receives t times the
+upwind value (the periodic jump included) plus (1 - t) times the diffuse
+re-emission of its own outflow through the side: each side of an interface is
+a diffuse wall of weight 1 - t with a lagged wall temperature, exactly like the
+walls of openbte/bte.py:73-78,199-214.  The kappa contact flux through a side
+is scaled by its t.  With t = 1 everywhere the model is the plain solve.
+
+Adjoint.  For upwind finite volumes (V A_s)^T = V A_{-s} (D_l is the same for
+s and -s because every cell is closed), so A_ml^{-T} = V A_{m,lbar}^{-1} V^{-1}
+with lbar the reversed direction.  Writing the adjoint as lambda_ml =
+t_ml V mu'_ml, the transposed coupled system (mode sums DeltaT, TB) becomes the
+forward outer iteration on mu' -- the same operator, the same sweeps in
+reversed level order, the same GCR interface solve and reductions -- with
+one constant source, the kappa functional's derivative, in place of the
+perturbation.  The face derivatives are then one HIP reduction over modes
+(obte_tim_face_grad); the chain rule to pixels is host work.
+
+``KappaTIM`` wraps forward and adjoint in a torch.autograd.Function.
+"""
+from __future__ import annotations
+
+from typing import Optional
+
+import numpy as np
+import torch
+
+from . import _lib
+from .bte import BTEDevice, SweepConfig
+from .operator import HostOperator, build_host_operator, build_schedule
+
+K0 = 1e-12
+
+
+def side_transmission(geo, pixel_of_cell: np.ndarray, rho: np.ndarray):
+    """Transmission t of every internal side (``geo.internal`` order) and its
+    derivatives with respect to the densities of the pixels of
+    ``side_elem_map[:,0]`` (a) and ``[:,1]`` (b) (openbte/inverse/bte.py:153-157)."""
+    inn = np.asarray(geo.internal)
+    sem = np.asarray(geo.side_elem_map)[inn]
+    pa = pixel_of_cell[sem[:, 0]]
+    pb = pixel_of_cell[sem[:, 1]]
+    interface = pa != pb
+    r = K0 + np.asarray(rho, np.float64) * (1.0 - K0)
+    a, b = r[pa], r[pb]
+    t = np.where(interface, 2.0 * a * b / (a + b), 1.0)
+    dt_da = np.where(interface, 2.0 * b * b / (a + b) ** 2 * (1.0 - K0), 0.0)
+    dt_db = np.where(interface, 2.0 * a * a / (a + b) ** 2 * (1.0 - K0), 0.0)
+    return t, dt_da, dt_db, pa, pb, interface
+
+
+def reversed_directions(dirs: np.ndarray) -> np.ndarray:
+    """lbar[l] with s_lbar = -s_l; raises if the direction set is not closed
+    under reversal."""
+    d = np.asarray(dirs, np.float64)
+    dot = d @ d.T
+    lbar = np.argmin(dot, axis=1)
+    if np.abs(d[lbar] + d).max() > 1e-9:
+        raise ValueError('the direction set has no reversed partner for every direction')
+    return lbar.astype(np.int32)
+
+
+def _with_interfaces(op: HostOperator, geo, interface: np.ndarray) -> tuple:
+    """Host operator whose wall list also holds one reflecting side per
+    directed interface face (weights set later), and the face tables."""
+    inn = np.asarray(geo.internal)
+    nint = len(inn)
+    F = 2 * nint
+    face_itf = np.concatenate([interface, interface])
+    faces = np.nonzero(face_itf)[0]                    # directed faces with a reflecting side
+    n_real = len(op.i_diffuse)
+    side_of_face = -np.ones(F, np.int32)
+    side_of_face[faces] = n_real + np.arange(len(faces))
+    cell = op.i_mat[faces].astype(np.int64)
+    db = op.k_face[faces]
+    Gb = np.einsum('mlj,sj->mls', op.sigma, db)
+    tot = np.clip(Gb, None, 0).sum(axis=(0, 1))
+    rinv = np.divide(1.0, tot, out=np.zeros_like(tot), where=tot != 0)
+    # merge the reflecting entries into the per-cell wall lists
+    n = op.n
+    old_cell = np.repeat(np.arange(n), np.diff(op.wall_ptr))
+    all_cell = np.concatenate([old_cell, cell])
+    all_side = np.concatenate([op.wall_side.astype(np.int64), -2 - (n_real + np.arange(len(faces)))])
+    all_db = np.concatenate([op.wall_db.reshape(-1, op.dim), db])
+    all_temp = np.concatenate([op.wall_temp, np.zeros(len(faces))])
+    o = np.argsort(all_cell, kind='stable')
+    wall_ptr = np.zeros(n + 1, np.int64)
+    np.add.at(wall_ptr, all_cell + 1, 1)
+    op2 = op._replace(
+        wall_ptr=np.cumsum(wall_ptr).astype(np.int32), wall_side=all_side[o].astype(np.int32),
+        wall_db=np.ascontiguousarray(all_db[o]), wall_temp=all_temp[o],
+        i_diffuse=np.concatenate([op.i_diffuse, cell]),
+        wall_dbs=np.ascontiguousarray(np.concatenate([op.wall_dbs.reshape(-1, op.dim), db])),
+        wall_rinv=np.concatenate([op.wall_rinv, rinv]),
+        t_face=np.ones(F), c_t=np.ones(len(op.ce1)))
+    # position of each reflecting side's entry in the merged wall list
+    inv = np.empty(len(o), np.int64)
+    inv[o] = np.arange(len(o))
+    entry_of_side = inv[len(old_cell):]
+    return op2, faces, side_of_face, entry_of_side
+
+
+class TIMSolver:
+    """Forward kappa_eff(rho) and its adjoint gradient on one GPU.
+
+    ``pixel_of_cell`` maps cells to pixels (n_pixels = max + 1).  The forward
+    solve iterates the outer loop of openbte/bte.py:193-256 until both the
+    lattice temperature and kappa change by less than ``rtol`` (relative), a
+    tighter rule than the reference's norm test, so that gradients and finite
+    differences are meaningful."""
+
+    def __init__(self, geo, mat, bcs, eff, pixel_of_cell: np.ndarray, cfg: Optional[SweepConfig] = None,
+                 rtol: float = 1e-11, max_iter: int = 2000):
+        if not torch.cuda.is_available():
+            raise RuntimeError('openbte_amd.TIMSolver needs a ROCm GPU; there is no CPU fallback')
+        self.geo = geo
+        self.pixel_of_cell = np.asarray(pixel_of_cell, np.int64)
+        self.n_pixels = int(self.pixel_of_cell.max()) + 1
+        self.rtol, self.max_iter = float(rtol), int(max_iter)
+        op = build_host_operator(geo, mat, bcs, eff)
+        self.c_sides = np.asarray(geo.periodic_sides[eff.contact])
+        if op.contact_kind != 1:
+            raise ValueError('the differentiable kappa needs a periodic kappa contact')
+        _, _, _, _, _, interface = side_transmission(geo, self.pixel_of_cell, np.ones(self.n_pixels))
+        self.interface = interface
+        self.jump0 = op.jump_face.copy()
+        op, faces, side_of_face, entry = _with_interfaces(op, geo, interface)
+        self.op, self.faces, self.entry = op, faces, entry
+        self.lbar = reversed_directions(op.dirs)
+        # adjoint requirements: t and sigma symmetric under reversal, sigma = C t Lambda s
+        if np.abs(op.t_coeff[:, self.lbar] - op.t_coeff).max() > 1e-12 * np.abs(op.t_coeff).max():
+            raise ValueError('t_coeff is not symmetric under direction reversal')
+        tl = op.t_coeff[:, :, None] * op.mfps[:, None, None] * op.dirs[None, :, :]
+        mask = np.abs(tl) > 1e-14 * np.abs(tl).max()
+        ratio = op.sigma[mask] / tl[mask]
+        self.c_sigma = float(np.median(ratio))
+        if np.abs(ratio / self.c_sigma - 1).max() > 1e-9:
+            raise ValueError('sigma_ml is not proportional to t_ml Lambda_m s_l')
+        self.sched = build_schedule(op)
+        self.dev = BTEDevice(op, sched=self.sched, cfg=cfg)
+        d = self.dev
+        dv = d.dev
+        self.vol = torch.as_tensor(np.asarray(geo.elem_volumes, np.float64), device=dv)
+        self.i_face = torch.as_tensor(op.i_mat, dtype=torch.int32, device=dv)
+        self.j_face = torch.as_tensor(op.j_mat, dtype=torch.int32, device=dv)
+        self.side_of_face = torch.as_tensor(side_of_face, dtype=torch.int32, device=dv)
+        self.jump0_d = torch.as_tensor(self.jump0, dtype=torch.float64, device=dv)
+        self.lbar_d = torch.as_tensor(self.lbar, dtype=torch.int32, device=dv)
+        self.heat0 = torch.zeros_like(d.buf['heat'])
+        self.n_iter = 0
+        self.kappa = None
+        self.rho = None
+
+    # -- transmissions ---------------------------------------------------------
+    def set_density(self, rho: np.ndarray) -> None:
+        rho = np.asarray(rho, np.float64).reshape(self.n_pixels)
+        t, da, db, pa, pb, _ = side_transmission(self.geo, self.pixel_of_cell, rho)
+        self.rho, self.t_int, self.dt_da, self.dt_db, self.pa, self.pb = rho, t, da, db, pa, pb
+        d, op = self.dev, self.op
+        t_face = np.concatenate([t, t])
+        d.buf['t_face'].copy_(torch.as_tensor(t_face))
+        d.buf['jump_face'].copy_(torch.as_tensor(self.jump0 * t_face))
+        wt = op.wall_temp.copy()
+        wt[self.entry] = 1.0 - t_face[self.faces]
+        d.buf['wall_temp'].copy_(torch.as_tensor(wt))
+        side_t = np.ones(len(self.geo.sides))
+        side_t[np.asarray(self.geo.internal)] = t
+        d.buf['c_t'].copy_(torch.as_tensor(side_t[self.c_sides]))
+        _lib.call('obte_bte_assemble', d.plan, d.stream)
+
+    # -- solves ----------------------------------------------------------------
+    def _iterate(self, t0: np.ndarray) -> int:
+        d = self.dev
+        d.reset(t0)
+        prev = d.buf['dt'][d.cur].clone()
+        k_old = None
+        for it in range(1, self.max_iter + 1):
+            d.outer_iteration()
+            cur = d.buf['dt'][d.cur]
+            dn = float(torch.linalg.norm(cur - prev).item())
+            nn = float(torch.linalg.norm(cur).item())
+            k = float(d.buf['kappa'].sum().item())
+            prev.copy_(cur)
+            if nn > 0 and dn <= self.rtol * nn and k_old is not None and abs(k - k_old) <= self.rtol * abs(k):
+                return it
+            k_old = k
+        return self.max_iter
+
+    def forward(self, rho: np.ndarray) -> float:
+        self.set_density(rho)
+        d = self.dev
+        self.n_iter = self._iterate(np.zeros(self.op.n))
+        self.kappa = float(d.buf['kappa'].sum().item())
+        self.t_fwd = d.buf['t'].clone()
+        self.tb_fwd = d.buf['tb'][d.cur].clone()
+        return self.kappa
+
+    def _adjoint_source(self) -> torch.Tensor:
+        """Constant adjoint source per (direction d, position): the kappa
+        functional's derivative for the reversed direction, divided by t_ml
+        Lambda_m V (openbte/bte.py:237-243 with the contact transmission)."""
+        op, geo = self.op, self.geo
+        n, L = op.n, op.n_dirs
+        na = np.asarray(geo.normal_areas)[self.c_sides]
+        side_t = np.ones(len(geo.sides))
+        side_t[np.asarray(geo.internal)] = self.t_int
+        ct = side_t[self.c_sides]
+        g = op.dirs @ na.T                                   # (L, nc) s_d . nA
+        src = np.zeros((L, n))
+        scale = op.normalization * self.c_sigma
+        # reversed direction: (s_l . nA)^+ = -(s_d . nA)^-, (s_l . nA)^- = -(s_d . nA)^+
+        np.add.at(src.T, op.ce1, (-np.clip(g, None, 0) * ct * scale).T)
+        np.add.at(src.T, op.ce2, (-np.clip(g, 0, None) * ct * scale).T)
+        src /= np.asarray(geo.elem_volumes)[None, :]
+        pos_src = np.take_along_axis(src, self.sched.order.reshape(L, n).astype(np.int64), axis=1)
+        return torch.as_tensor(pos_src.reshape(-1), device=self.dev.dev)
+
+    def adjoint(self) -> None:
+        d = self.dev
+        P = d.plan
+        d.buf['pert'].copy_(self._adjoint_source())
+        heat_ptr = P.heat
+        P.heat = self.heat0.data_ptr()
+        try:
+            self.n_iter_adj = self._iterate(np.zeros(self.op.n))
+        finally:
+            P.heat = heat_ptr
+            _lib.call('obte_bte_assemble', P, d.stream)     # restores the forward perturbation
+
+    def gradient(self) -> np.ndarray:
+        """d kappa_eff / d rho for the densities of the last ``forward``."""
+        if self.kappa is None:
+            raise RuntimeError('call forward first')
+        self.adjoint()
+        d, op = self.dev, self.op
+        F = len(op.i_mat)
+        grad_f = torch.zeros(F, dtype=torch.float64, device=d.dev)
+        _lib.call('obte_tim_face_grad', d.plan, self.t_fwd.data_ptr(), d.buf['t'].data_ptr(),
+                  self.lbar_d.data_ptr(), self.i_face.data_ptr(), self.j_face.data_ptr(),
+                  self.side_of_face.data_ptr(), self.tb_fwd.data_ptr(), self.jump0_d.data_ptr(),
+                  self.vol.data_ptr(), F, grad_f.data_ptr(), d.stream)
+        gf = grad_f.cpu().numpy()
+        nint = F // 2
+        g_side = gf[:nint] + gf[nint:]
+        g_side += self._contact_term()
+        grad = np.zeros(self.n_pixels)
+        np.add.at(grad, self.pa, g_side * self.dt_da)
+        np.add.at(grad, self.pb, g_side * self.dt_db)
+        return grad
+
+    def _contact_term(self) -> np.ndarray:
+        """d kappa / d t of the contact sides through the flux formula itself."""
+        op, geo, d = self.op, self.geo, self.dev
+        n, L, M, mc, nch = op.n, op.n_dirs, op.n_mfp, d.cfg.mc, d.n_chunks
+        rows = n + d.ncut_max
+        T = self.t_fwd.view(L, nch, rows, mc)
+        pos = torch.as_tensor(self.sched.pos.reshape(L, n).astype(np.int64), device=d.dev)
+        e1 = torch.as_tensor(op.ce1.astype(np.int64), device=d.dev)
+        e2 = torch.as_tensor(op.ce2.astype(np.int64), device=d.dev)
+        li = torch.arange(L, device=d.dev)[:, None]
+        T1 = T[li, :, pos[:, e1], :]                    # (L, nc, nch, mc)
+        T2 = T[li, :, pos[:, e2], :]
+        T1 = T1.reshape(L, len(op.ce1), nch * mc)[:, :, :M].cpu().numpy()
+        T2 = T2.reshape(L, len(op.ce1), nch * mc)[:, :, :M].cpu().numpy()
+        na = np.asarray(geo.normal_areas)[self.c_sides]
+        sn = np.einsum('mlj,cj->lcm', op.sigma, na)      # (L, nc, M)
+        per = (np.clip(sn, 0, None) * T1 + np.clip(sn, None, 0) * (T2 + op.contact_jump)).sum(axis=(0, 2))
+        out = np.zeros(len(op.i_mat) // 2)
+        side_index = {s: k for k, s in enumerate(np.asarray(geo.internal))}
+        for c, s in enumerate(self.c_sides):
+            out[side_index[int(s)]] += op.normalization * per[c]
+        return out
+
+
+class KappaTIM(torch.autograd.Function):
+    """kappa = KappaTIM.apply(rho, solver): forward solve on the GPU, backward
+    by the adjoint solve (``TIMSolver.gradient``)."""
+
+    @staticmethod
+    def forward(ctx, rho: torch.Tensor, solver: TIMSolver):
+        k = solver.forward(rho.detach().cpu().numpy().astype(np.float64))
+        ctx.solver = solver
+        return torch.tensor(k, dtype=torch.float64, device=rho.device)
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        g = ctx.solver.gradient()
+        return grad_out * torch.as_tensor(g, dtype=torch.float64, device=grad_out.device), None
