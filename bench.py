@@ -43,6 +43,11 @@ SERVE_CONFIGS = {
     'serve': (24, (1000, 200_000), (5000, 150_000), (20, 60), 48, (8, 8), 0.4, 7),
 }
 
+INVERSE_CONFIGS = {
+    # name: (pixels per side, L nm, n_mfp, n_dirs, mfp_max nm, smoothing (pixels), seed)
+    'inverse40': (40, 100.0, 30, 48, 1.0e4, 3.0, 11),
+}
+
 CONFIGS = {
     # name: (L nm, porosity, cells, n_mfp, n_dirs, mfp_max nm)
     'plain2k': (10.0, 0.25, 2000, 30, 48, 1.0e4),
@@ -346,13 +351,94 @@ def run_serve(args):
     return 0
 
 
+def density_field(N, smooth, seed):
+    """Seeded smooth density on an N x N periodic pixel grid, in [0.05, 1]."""
+    z = np.random.default_rng(seed).standard_normal((N, N))
+    k = np.fft.fftfreq(N)
+    g = np.exp(-2 * (np.pi * smooth) ** 2 * (k[:, None] ** 2 + k[None, :] ** 2))
+    z = np.real(np.fft.ifft2(np.fft.fft2(z) * g))
+    z = (z - z.mean()) / z.std()
+    # smooth map into [0.05, 1]: every interface keeps some transmission
+    return (0.05 + 0.95 / (1.0 + np.exp(-2.0 * z))).reshape(-1)
+
+
+def run_inverse(args):
+    """Differentiation workload: kappa_eff(rho) forward + adjoint gradient on
+    a pixel grid (4 triangles per pixel), checked by central differences."""
+    import torch
+    from openbte_amd.bte import SweepConfig
+    from openbte_amd.geometry import structured_tri_cell_2d
+    from openbte_amd.material import synthetic_material_2d
+    from openbte_amd.objects import BoundaryConditions, EffectiveThermalConductivity
+    from openbte_amd.tim import KappaTIM, TIMSolver
+    N, L, M, nphi, mfp_max, smooth, seed = INVERSE_CONFIGS[args.config]
+    geo = structured_tri_cell_2d(L, N)
+    mat = synthetic_material_2d(M, nphi, 1.0, mfp_max, seed=seed)
+    bcs = BoundaryConditions(periodic={'Periodic_x': 1.0, 'Periodic_y': 0.0})
+    eff = EffectiveThermalConductivity(contact='Periodic_x', normalization=-1.0)
+    rho = density_field(N, smooth, seed)
+    cfg = SweepConfig()
+    if args.mc:
+        cfg.mc = args.mc
+    if args.block:
+        cfg.block = args.block
+    t0 = time.perf_counter()
+    solver = TIMSolver(geo, mat, bcs, eff, np.arange(geo.n_elems) // 4, cfg=cfg, rtol=1e-13)
+    setup = time.perf_counter() - t0
+    solver.forward(np.ones_like(rho))  # warm-up (kernels, allocator)
+    solver.x_fwd = solver.x_adj = None  # timed solves start cold
+    torch.cuda.synchronize()
+    r = torch.tensor(rho, dtype=torch.float64, requires_grad=True)
+    t0 = time.perf_counter()
+    k = KappaTIM.apply(r, solver)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    it_fwd = solver.n_iter
+    t0 = time.perf_counter()
+    k.backward()
+    torch.cuda.synchronize()
+    t_adj = time.perf_counter() - t0
+    it_adj = solver.n_iter_adj
+    g = r.grad.numpy()
+    # central differences (h = 1e-6) at the pixels of largest |gradient| and a random one
+    picks = list(np.argsort(-np.abs(g))[:2]) + [int(np.random.default_rng(seed).integers(0, N * N))]
+    h = 1e-6
+    fd_err = []
+    for p in picks:
+        e = np.zeros_like(rho)
+        e[p] = h
+        fd = (solver.forward(rho + e) - solver.forward(rho - e)) / (2 * h)
+        fd_err.append(abs(g[p] - fd) / np.abs(g).max())
+    n = geo.n_elems
+    updates = n * M * nphi * it_fwd
+    value = updates / t_fwd
+    print(f'# {args.config}: kappa {float(k.detach()):.6f}, forward {it_fwd} iterations {t_fwd:.2f}s, adjoint '
+          f'{it_adj} iterations {t_adj:.2f}s, fd rel err {max(fd_err):.2e}', file=sys.stderr, flush=True)
+    out = {
+        'metric': 'cell*mode updates/s (forward TIM solve) + adjoint d kappa / d rho',
+        'value': value, 'unit': 'cell*mode updates/s', 'n_gpus': 1, 'steps': it_fwd, 'warmup': 1,
+        'ms_per_step': t_fwd / it_fwd * 1e3, 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': 'f64', 'data': 'synthetic (seeded smoothed random density, seeded Si-like material)',
+        'config': {'workload': f'{args.config}: {N}x{N} pixels, {n} triangles, {M} MFP x {nphi} directions, '
+                               f'interfacial transmission t = 2 rho_a rho_b / (rho_a + rho_b)',
+                   'mc': cfg.mc, 'block': cfg.block, 'kappa_eff': float(k.detach()), 'setup_s': setup,
+                   'forward_s': t_fwd, 'forward_iterations': it_fwd, 'adjoint_s': t_adj,
+                   'adjoint_iterations': it_adj, 'gradient_entries': int(N * N),
+                   'fd_pixels': [int(p) for p in picks], 'fd_h': h,
+                   'max_fd_rel_err': float(max(fd_err))},
+    }
+    print(json.dumps(out))
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=20)
     ap.add_argument('--warmup', type=int, default=2)
     ap.add_argument('--config', default='plain200k',
-                    choices=sorted(CONFIGS) + sorted(BATCH_CONFIGS) + sorted(SERVE_CONFIGS))
+                    choices=sorted(CONFIGS) + sorted(BATCH_CONFIGS) + sorted(SERVE_CONFIGS)
+                    + sorted(INVERSE_CONFIGS))
     ap.add_argument('--no-graphs', action='store_true', help='batched configs: launch without hipGraphs')
     ap.add_argument('--mc', type=int, default=None)
     ap.add_argument('--block', type=int, default=None)
@@ -372,6 +458,8 @@ def main(argv=None):
         return run_batch(args)
     if args.config in SERVE_CONFIGS:
         return run_serve(args)
+    if args.config in INVERSE_CONFIGS:
+        return run_inverse(args)
 
     import torch
     from openbte_amd.bte import BTEDevice, SweepConfig
