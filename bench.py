@@ -146,7 +146,7 @@ def run_batch(args):
     from openbte_amd.batch import BTEBatch
     from openbte_amd.bte import BTEDevice, SweepConfig
     from openbte_amd.fourier import Fourier
-    from openbte_amd.operator import build_host_operator
+    from openbte_amd.operator import build_host_operator, build_schedule
     geos, mat, bcs, eff = make_batch(args.config)
     M, L = mat.grid[0], mat.grid[1]
     ncells = np.array([g.n_elems for g in geos])
@@ -157,11 +157,19 @@ def run_batch(args):
         f = Fourier(g, mat.thermal_conductivity, bcs, eff, verbose=False)
         t0s.append(f.variables['Temperature_Fourier']['data'])
     ops = [build_host_operator(g, mat, bcs, eff) for g in geos]
-    cfg = SweepConfig(mc=args.mc or 8, block=args.block or 256)
+    scheds = [build_schedule(o) for o in ops]
+    tuned = None
+    if args.mc or args.block:
+        cfg = SweepConfig(mc=args.mc or 8, block=args.block or 256)
+    else:
+        from openbte_amd import tune
+        cfg = tune.tune(ops, scheds)
+        tuned = tune.last_results(ops)
+        print(f'# autotuned mc={cfg.mc} block={cfg.block}', file=sys.stderr, flush=True)
     K = args.steps
     updates = float(ncells.sum()) * M * L * K
 
-    batch = BTEBatch(ops, cfg=cfg)
+    batch = BTEBatch(ops, cfg=SweepConfig(**vars(cfg)), scheds=scheds)
     if not args.no_graphs:
         batch.enable_graphs(True)
     batch.reset(t0s)
@@ -177,23 +185,31 @@ def run_batch(args):
     del batch
     torch.cuda.empty_cache()
 
-    # plain form: the same K outer iterations, one geometry at a time
-    dt_plain = 0.0
-    plain_kappa = []
-    for op, t0 in zip(ops, t0s):
-        dev = BTEDevice(op, cfg=cfg)
-        dev.reset(t0)
-        dev.outer_iteration()
-        dev.reset(t0)
-        torch.cuda.synchronize()
-        t = time.perf_counter()
-        for _ in range(K):
+    # plain form: the same K outer iterations, one geometry at a time, at the
+    # plain path's default shape and at the batch's shape; the faster counts
+    def plain(pcfg):
+        dt, ks = 0.0, []
+        for op, sc, t0 in zip(ops, scheds, t0s):
+            dev = BTEDevice(op, sched=sc, cfg=SweepConfig(**vars(pcfg)))
+            dev.reset(t0)
             dev.outer_iteration()
-        torch.cuda.synchronize()
-        dt_plain += time.perf_counter() - t
-        plain_kappa.append(float(dev.buf['kappa'].sum().item()))
-        del dev
-    torch.cuda.empty_cache()
+            dev.reset(t0)
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            for _ in range(K):
+                dev.outer_iteration()
+            torch.cuda.synchronize()
+            dt += time.perf_counter() - t
+            ks.append(float(dev.buf['kappa'].sum().item()))
+            del dev
+        torch.cuda.empty_cache()
+        return dt, ks
+    plain_runs = {(8, 256): plain(SweepConfig())}
+    if (cfg.mc, cfg.block) not in plain_runs:
+        plain_runs[(cfg.mc, cfg.block)] = plain(cfg)
+    best_plain = min(plain_runs, key=lambda k: plain_runs[k][0])
+    dt_plain = plain_runs[best_plain][0]
+    plain_kappa = plain_runs[(cfg.mc, cfg.block)][1]
     same = max(abs(a - b) / abs(b) for a, b in zip(kappas, plain_kappa))
     value = updates / dt_batch
     out = {
@@ -205,7 +221,10 @@ def run_batch(args):
                                f'({int(ncells.min())}-{int(ncells.max())}), {M} MFP x {L} directions',
                    'mc': cfg.mc, 'block': cfg.block, 'hipgraph': not args.no_graphs,
                    'extensions_per_step': exts,
+                   'autotune': tuned,
                    'plain_value': updates / dt_plain, 'plain_ms_per_step': dt_plain / K * 1e3,
+                   'plain_shape': list(best_plain),
+                   'plain_values_by_shape': {f'mc{k[0]}_block{k[1]}': updates / v[0] for k, v in plain_runs.items()},
                    'batched_over_plain': dt_plain / dt_batch,
                    'max_rel_kappa_diff_batched_vs_plain': same},
     }

@@ -125,8 +125,8 @@ typedef struct obte_bte_plan {
   const double* tb;        /* [n_wall] wall temperatures of the previous iteration */
   double* t;               /* [L][n_chunks][n + ncut_max][mc] solution workspace; rows >= n
                               hold the interface inflows of the running sweep */
-  double* part_t;          /* [L][n_chunks][n] partial lattice temperature, sweep order */
-  double* part_j;          /* [L][n_chunks][n][dim] partial flux, sweep order */
+  double* part_t;          /* [L][n] per-direction lattice temperature, sweep order */
+  double* part_j;          /* [L][n][dim] per-direction flux, sweep order */
   double* part_tb;         /* [L][n_chunks][n_wall] */
   double* kappa;           /* [M][L] */
   double* supp;            /* [M][L] */
@@ -161,8 +161,9 @@ int obte_bte_source(const obte_bte_plan_t* plan, obte_stream_t stream);
 /* Level-scheduled upwind sweeps of all (l, m) systems (openbte/bte.py:217-218).
  * mode 0: source + interface inflow y -> exit values x
  * mode 1: zero source, interface inflow y -> exit values x (transfer operator)
- * mode 2: as 0, plus the mode sums of openbte/bte.py:222-243 into the
- *         partial buffers and kappa / supp.
+ * mode 2: as 0, plus the wall temperatures (openbte/bte.py:227) into
+ *         part_tb and kappa / supp (openbte/bte.py:232-243); the solutions
+ *         stay in plan->t for obte_bte_reduce.
  * y, x: [n_chunks][n_cut][mc].  active: [L][n_chunks] (or NULL = all);
  * lane_active: [L][n_chunks][mc] per-system flags used by mode 1 to skip
  * converged MFP bins (or NULL). */
@@ -199,8 +200,10 @@ int obte_gcr_end_extend(const obte_bte_plan_t* plan, const obte_gcr_t* g, obte_s
  * cell (use_delta_t != 0) or 0. */
 int obte_gcr_reset(const obte_bte_plan_t* plan, const obte_gcr_t* g, int32_t use_delta_t, obte_stream_t stream);
 
-/* Fixed-order reduction of the partial buffers into DeltaT, J, TB
- * (openbte/bte.py:245-249 -> openbte/objects.py:86-101); norm2 = ||DeltaT||^2. */
+/* After a mode-2 sweep: mode sums DeltaT = sum t T and J = sum sigma (T - DeltaT_old)
+ * (openbte/bte.py:222-230) from plan->t, then the fixed-order reduction over
+ * directions into DeltaT, J, TB (openbte/bte.py:245-249 -> openbte/objects.py:86-101);
+ * norm2 = ||DeltaT||^2.  plan->delta_t must still be the sweep's DeltaT_old. */
 int obte_bte_reduce(const obte_bte_plan_t* plan, double* delta_t_new, double* j_new, double* tb_new,
                     double* norm2, obte_stream_t stream);
 

@@ -91,7 +91,7 @@ def device_bytes(op: HostOperator, sched: Schedule, cfg: Optional[SweepConfig] =
     f8 += L * n + L * n * 8                                  # pert, rec
     f8 += 2 * n + 2 * one(n_wall) + n * dim                  # dt, tb, j
     f8 += L * nch * (n + ncut_max) * mc                      # t
-    f8 += L * nch * n * (1 + dim) + one(L * nch * n_wall)    # part_t, part_j, part_tb
+    f8 += L * n * (1 + dim) + one(L * nch * n_wall)          # part_t, part_j, part_tb
     f8 += 2 * M * L + 4096 + 8                               # kappa, supp, scratch
     ny = max(1, nch * n_cut * mc)
     f8 += 4 * ny + 2 * _kmax(cfg, ny) * ny + M * L           # y r x w, u c, rnorm
@@ -177,8 +177,8 @@ class BTEDevice:
         b['j'] = torch.zeros(n * dim, dtype=f64, device=dev)
         # zero-initialised: unused inflow slots read a finite value times 0
         b['t'] = torch.zeros(L * nch * (n + self.ncut_max) * mc, dtype=f64, device=dev)
-        b['part_t'] = torch.empty(L * nch * n, dtype=f64, device=dev)
-        b['part_j'] = torch.empty(L * nch * n * dim, dtype=f64, device=dev)
+        b['part_t'] = torch.empty(L * n, dtype=f64, device=dev)
+        b['part_j'] = torch.empty(L * n * dim, dtype=f64, device=dev)
         b['part_tb'] = torch.empty(max(1, L * nch * n_wall), dtype=f64, device=dev)
         b['kappa'] = torch.zeros(M * L, dtype=f64, device=dev)
         b['supp'] = torch.zeros(M * L, dtype=f64, device=dev)

@@ -230,26 +230,6 @@ __device__ __forceinline__ double rec_get(const double2 (&pc)[NP]) {
   return quad_bcast<GQ, piece / NP>(D % 2 ? w.y : w.x);
 }
 
-// v + v of lane (lane ^ O) within quads
-template <int O>
-__device__ __forceinline__ double quad_xor_add(double v) {
-  constexpr int ctrl = O == 1 ? (1 | 0 << 2 | 3 << 4 | 2 << 6) : (2 | 3 << 2 | 0 << 4 | 1 << 6);
-  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), ctrl, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), ctrl, 0xF, 0xF, false);
-  return v + __hiloint2double(hi, lo);
-}
-
-// Sum over the G lanes of one cell (fixed xor-tree order; every lane of the
-// cell gets the same bits since the additions commute).
-template <int G>
-__device__ __forceinline__ double cell_sum(double v) {
-#pragma unroll
-  for (int o = G / 2; o >= 4; o >>= 1) v += __shfl_xor(v, o, G);
-  if (G >= 4) v = quad_xor_add<2>(v);
-  if (G >= 2) v = quad_xor_add<1>(v);
-  return v;
-}
-
 constexpr int kLvlStage = 2048;  // level pointers staged in LDS per refill
 
 template <int MC>
@@ -287,29 +267,16 @@ __device__ void sweep_task(const S& src, int pid, int lt, SweepShared<MC>& sh) {
   const int m0 = chunk * MC + 2 * jl;
   const bool v0 = m0 < P.n_mfp, v1 = m0 + 1 < P.n_mfp;
   const double lam0 = v0 ? P.mfps[m0] : 0.0, lam1 = v1 ? P.mfps[m0 + 1] : 0.0;
-  double tco0 = 0.0, tco1 = 0.0, sx0 = 0.0, sx1 = 0.0, sy0 = 0.0, sy1 = 0.0, sz0 = 0.0, sz1 = 0.0;
-  if (MODE == 2) {
-    if (v0) {
-      tco0 = P.t_coeff[(int64_t)m0 * L + l];
-      const double* sg = P.sigma + ((int64_t)m0 * L + l) * dim;
-      sx0 = sg[0];
-      sy0 = sg[1];
-      if (dim == 3) sz0 = sg[2];
-    }
-    if (v1) {
-      tco1 = P.t_coeff[(int64_t)(m0 + 1) * L + l];
-      const double* sg = P.sigma + ((int64_t)(m0 + 1) * L + l) * dim;
-      sx1 = sg[0];
-      sy1 = sg[1];
-      if (dim == 3) sz1 = sg[2];
-    }
-    if (cslot == 0) {
-      sh.sig[3 * 2 * jl + 0] = sx0;
-      sh.sig[3 * 2 * jl + 1] = sy0;
-      sh.sig[3 * 2 * jl + 2] = sz0;
-      sh.sig[3 * (2 * jl + 1) + 0] = sx1;
-      sh.sig[3 * (2 * jl + 1) + 1] = sy1;
-      sh.sig[3 * (2 * jl + 1) + 2] = sz1;
+  if (MODE == 2 && cslot == 0) {
+    // group velocities of the chunk's bins for the wall and contact sums
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int m = m0 + h;
+      const double* sg = P.sigma + ((int64_t)(m < P.n_mfp ? m : 0) * L + l) * dim;
+      const bool ok = m < P.n_mfp;
+      sh.sig[3 * (2 * jl + h) + 0] = ok ? sg[0] : 0.0;
+      sh.sig[3 * (2 * jl + h) + 1] = ok ? sg[1] : 0.0;
+      sh.sig[3 * (2 * jl + h) + 2] = ok && dim == 3 ? sg[2] : 0.0;
     }
   }
   bool act = true;  // transfer sweeps skip converged bin pairs
@@ -323,9 +290,6 @@ __device__ void sweep_task(const S& src, int pid, int lt, SweepShared<MC>& sh) {
   const int ncut = P.cut_off[l + 1] - cbase;
   const int64_t lo = (int64_t)l * n;
   const double* __restrict__ rec = P.rec + lo * kRec;
-  const double* __restrict__ dtp = P.delta_t;
-  double* __restrict__ part_t = P.part_t + (int64_t)lt * n;
-  double* __restrict__ part_j = P.part_j + (int64_t)lt * n * dim;
 
   // interface inflows of the cut faces -> rows n .. n + ncut of T
   if (ncut > 0) {
@@ -342,7 +306,6 @@ __device__ void sweep_task(const S& src, int pid, int lt, SweepShared<MC>& sh) {
   __syncthreads();
 
   using Regs = LevelRegs<U, NP>;
-  double dtc[U];  // DeltaT of each cell (mode sums only)
   // upwind gathers of the U cells of C (idle slots skip them)
   auto gather = [&](const Regs& C, double2 (&g)[U][KIN]) {
 #pragma unroll
@@ -355,10 +318,6 @@ __device__ void sweep_task(const S& src, int pid, int lt, SweepShared<MC>& sh) {
         g[u][kk] = make_double2(0.0, 0.0);
         if (C.p[u] >= 0 && (MODE != 1 || act))
           g[u][kk] = *reinterpret_cast<const double2*>(T + (int64_t)sr[kk] * MC + 2 * jl);
-      }
-      if (MODE == 2) {
-        dtc[u] = 0.0;
-        if (C.p[u] >= 0) dtc[u] = dtp[__double2hiint(s2c)];
       }
     }
   };
@@ -378,27 +337,6 @@ __device__ void sweep_task(const S& src, int pid, int lt, SweepShared<MC>& sh) {
       }
       const double t0 = (bb + lam0 * (qq + a0)) / (1.0 + lam0 * dd);
       const double t1 = (bb + lam1 * (qq + a1)) / (1.0 + lam1 * dd);
-      if (MODE == 2) {
-        // partial mode sums in sweep order; every lane of the cell holds the
-        // sums, lane v stores the v-th of (T, J_x, J_y[, J_z])
-        const double d0 = t0 - dtc[u], d1 = t1 - dtc[u];
-        const double pt = cell_sum<G>(tco0 * t0 + tco1 * t1);
-        const double px = cell_sum<G>(d0 * sx0 + d1 * sx1);
-        const double py = cell_sum<G>(d0 * sy0 + d1 * sy1);
-        const double pz = dim == 3 ? cell_sum<G>(d0 * sz0 + d1 * sz1) : 0.0;
-        if (C.p[u] >= 0) {
-          const int pp = C.p[u];
-#pragma unroll
-          for (int r = 0; r * G < 4; ++r) {
-            const int v = r * G + jl;
-            if (v <= dim) {
-              const double val = v == 0 ? pt : v == 1 ? px : v == 2 ? py : pz;
-              double* dst = v == 0 ? part_t + pp : part_j + (int64_t)pp * dim + (v - 1);
-              *dst = val;
-            }
-          }
-        }
-      }
       if (C.p[u] >= 0 && (MODE != 1 || act))
         *reinterpret_cast<double2*>(T + (int64_t)C.p[u] * MC + 2 * jl) = make_double2(t0, t1);
     }
@@ -629,6 +567,67 @@ __device__ double block_lane_max(double v, double* red, double* bc) {
   return r;
 }
 
+// Block-wide per-lane sums of JB values at once (one reduction for JB dot
+// products: 2 barriers instead of 3 per value); v[j] is replaced by the sum
+// over the slots of the block, in a fixed order.  red: JB * blockDim doubles.
+template <int MC, int JB>
+__device__ __forceinline__ void block_lane_sums(double (&v)[JB], double* red, double* bc) {
+  const int lane = threadIdx.x % MC;
+  const int cpp = blockDim.x / MC;
+#pragma unroll
+  for (int j = 0; j < JB; ++j) red[j * blockDim.x + threadIdx.x] = v[j];
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < JB * MC; idx += blockDim.x) {
+    const int j = idx / MC, ln = idx - j * MC;
+    double acc = 0.0;
+    for (int q = 0; q < cpp; ++q) acc += red[j * blockDim.x + q * MC + ln];
+    bc[idx] = acc;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < JB; ++j) v[j] = bc[j * MC + lane];
+  __syncthreads();
+}
+
+constexpr int kGsBlock = 8;  // basis vectors per block of the Gram-Schmidt passes
+
+// cn -= sum_j (c_j . cn) c_j and un -= the same combination of u_j over the
+// first kb basis vectors, JB vectors at a time (dot products of a block
+// together, blocks in sequence).
+template <int MC>
+__device__ void gs_pass(const obte_gcr_t& G, int64_t stride, int64_t base, int ncut, int kb, double* cn,
+                        double* un, double* red, double* bc) {
+  constexpr int JB = kGsBlock;
+  const int lane = threadIdx.x % MC;
+  const int slot = threadIdx.x / MC;
+  const int cpp = blockDim.x / MC;
+  for (int j0 = 0; j0 < kb; j0 += JB) {
+    double d[JB];
+#pragma unroll
+    for (int jj = 0; jj < JB; ++jj) d[jj] = 0.0;
+    for (int f = slot; f < ncut; f += cpp) {
+      const int64_t o = (int64_t)f * MC + lane;
+      const double c0 = cn[o];
+#pragma unroll
+      for (int jj = 0; jj < JB; ++jj)
+        if (j0 + jj < kb) d[jj] += G.c[(j0 + jj) * stride + base + o] * c0;
+    }
+    block_lane_sums<MC, JB>(d, red, bc);
+    for (int f = slot; f < ncut; f += cpp) {
+      const int64_t o = (int64_t)f * MC + lane;
+      double cv = cn[o], uv = un[o];
+#pragma unroll
+      for (int jj = 0; jj < JB; ++jj)
+        if (j0 + jj < kb) {
+          cv -= d[jj] * G.c[(j0 + jj) * stride + base + o];
+          uv -= d[jj] * G.u[(j0 + jj) * stride + base + o];
+        }
+      cn[o] = cv;
+      un[o] = uv;
+    }
+  }
+}
+
 struct GcrView {
   int64_t stride;  // elements of one interface vector set = n_chunks * n_cut * MC
   int64_t base;    // offset of this (l, chunk) block
@@ -672,9 +671,9 @@ __device__ void gcr_flags(const obte_bte_plan_t& P, const obte_gcr_t& G, int tas
 
 // r = x - y; MGS projection of r on the recycled basis; y += U a.
 template <class S, int MC>
-__global__ __launch_bounds__(kMaxBlock) void k_gcr_project(S src) {
-  __shared__ double red[kMaxBlock];
-  __shared__ double bc[64];
+__global__ __launch_bounds__(kSweepBlock) void k_gcr_project(S src) {
+  __shared__ double red[kGsBlock * kSweepBlock];
+  __shared__ double bc[kGsBlock * 64];
   const int pid = src.pid_of(blockIdx.x);
   if (!src.live(pid)) return;
   const int task = src.local_of(blockIdx.x);
@@ -694,20 +693,15 @@ __global__ __launch_bounds__(kMaxBlock) void k_gcr_project(S src) {
     r[o] = x[o] - y[o];
   }
   const int kb = G.k[task];
-  for (int j = 0; j < kb; ++j) {
-    const double* cj = G.c + j * v.stride + v.base;
-    const double* uj = G.u + j * v.stride + v.base;
-    double part = 0.0;
-    for (int f = slot; f < v.ncut; f += cpp) {
-      const int64_t o = (int64_t)f * MC + lane;
-      part += cj[o] * r[o];
-    }
-    const double a = block_lane_sum<MC>(part, red, bc);
-    for (int f = slot; f < v.ncut; f += cpp) {
-      const int64_t o = (int64_t)f * MC + lane;
-      r[o] -= a * cj[o];
-      y[o] += a * uj[o];
-    }
+  // r -= C a, y -= U (-a) with a = C^T r: gs_pass with (cn, un) = (r, -y)
+  for (int f = slot; f < v.ncut; f += cpp) {
+    const int64_t o = (int64_t)f * MC + lane;
+    y[o] = -y[o];
+  }
+  gs_pass<MC>(G, v.stride, v.base, v.ncut, kb, r, y, red, bc);
+  for (int f = slot; f < v.ncut; f += cpp) {
+    const int64_t o = (int64_t)f * MC + lane;
+    y[o] = -y[o];
   }
   double rp = 0.0, ym = 0.0;
   for (int f = slot; f < v.ncut; f += cpp) {
@@ -722,7 +716,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_gcr_project(S src) {
 
 // w = r on unconverged lanes, 0 elsewhere (inflow of the transfer sweep).
 template <class S, int MC>
-__global__ __launch_bounds__(kMaxBlock) void k_gcr_begin(S src) {
+__global__ __launch_bounds__(kSweepBlock) void k_gcr_begin(S src) {
   const int pid = src.pid_of(blockIdx.x);
   if (!src.live(pid)) return;
   const int task = src.local_of(blockIdx.x);
@@ -747,9 +741,9 @@ __global__ __launch_bounds__(kMaxBlock) void k_gcr_begin(S src) {
 // c = w - x (= (I - T_op) w); orthogonalise against the basis, normalise,
 // append, then y += a u, r -= a c with a = c . r.
 template <class S, int MC>
-__global__ __launch_bounds__(kMaxBlock) void k_gcr_end(S src) {
-  __shared__ double red[kMaxBlock];
-  __shared__ double bc[64];
+__global__ __launch_bounds__(kSweepBlock) void k_gcr_end(S src) {
+  __shared__ double red[kGsBlock * kSweepBlock];
+  __shared__ double bc[kGsBlock * 64];
   const int pid = src.pid_of(blockIdx.x);
   if (!src.live(pid)) return;
   const int task = src.local_of(blockIdx.x);
@@ -776,23 +770,9 @@ __global__ __launch_bounds__(kMaxBlock) void k_gcr_end(S src) {
     un[o] = act ? w[o] : 0.0;
     cn[o] = act ? w[o] - x[o] : 0.0;
   }
-  for (int pass = 0; pass < 2; ++pass) {  // two MGS passes for orthogonality
-    for (int j = 0; j < kb; ++j) {
-      const double* cj = G.c + j * v.stride + v.base;
-      const double* uj = G.u + j * v.stride + v.base;
-      double part = 0.0;
-      for (int f = slot; f < v.ncut; f += cpp) {
-        const int64_t o = (int64_t)f * MC + lane;
-        part += cj[o] * cn[o];
-      }
-      const double b = block_lane_sum<MC>(part, red, bc);
-      for (int f = slot; f < v.ncut; f += cpp) {
-        const int64_t o = (int64_t)f * MC + lane;
-        cn[o] -= b * cj[o];
-        un[o] -= b * uj[o];
-      }
-    }
-  }
+  // two block Gram-Schmidt passes ("twice is enough") for orthogonality
+  gs_pass<MC>(G, v.stride, v.base, v.ncut, kb, cn, un, red, bc);
+  gs_pass<MC>(G, v.stride, v.base, v.ncut, kb, cn, un, red, bc);
   double part = 0.0;
   for (int f = slot; f < v.ncut; f += cpp) {
     const int64_t o = (int64_t)f * MC + lane;
@@ -856,29 +836,40 @@ __global__ void k_gcr_reset(S src, int use_dt) {
   for (int c = threadIdx.x; c < P.n_chunks; c += blockDim.x) G.k[(int64_t)l * P.n_chunks + c] = 0;
 }
 
-// Stage 1: sum the chunks of each direction in sweep order (coalesced),
-// in place into chunk 0 of the direction.
+// Stage 1 (openbte/bte.py:222-230): mode sums of each direction in sweep
+// order, straight from the solution workspace of the last full sweep:
+//   part_t[l][p] = sum_m t_ml T_ml,   part_j[l][p] = sum_m sigma_ml (T_ml - DeltaT_old)
+// (chunks then bins, a fixed order), in [L][n].  One thread per
+// (direction, position): neighbouring threads read neighbouring rows.
 template <class S>
-__global__ void k_reduce_chunks(S src) {
+__global__ void k_modesum(S src) {
   const int pid = blockIdx.y;
   if (!src.live(pid)) return;
   const obte_bte_plan_t& P = src.plan(pid);
   const int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int64_t total = (int64_t)P.n_dirs * P.n;
-  if (gid >= total || P.n_chunks == 1) return;
+  if (gid >= total) return;
   const int l = (int)(gid / P.n);
   const int p = (int)(gid - (int64_t)l * P.n);
-  const int dim = P.dim;
-  const int64_t o0 = (int64_t)l * P.n_chunks * P.n + p;
-  double s = P.part_t[o0], j0 = P.part_j[o0 * dim], j1 = P.part_j[o0 * dim + 1];
-  double j2 = dim == 3 ? P.part_j[o0 * dim + 2] : 0.0;
-  for (int c = 1; c < P.n_chunks; ++c) {
-    const int64_t o = o0 + (int64_t)c * P.n;
-    s += P.part_t[o];
-    j0 += P.part_j[o * dim];
-    j1 += P.part_j[o * dim + 1];
-    if (dim == 3) j2 += P.part_j[o * dim + 2];
+  const int dim = P.dim, L = P.n_dirs, M = P.n_mfp, mc = P.mc, nch = P.n_chunks;
+  const double dt = P.delta_t[P.order[gid]];
+  const int64_t rows = (int64_t)P.n + P.ncut_max;
+  double s = 0.0, j0 = 0.0, j1 = 0.0, j2 = 0.0;
+  for (int c = 0; c < nch; ++c) {
+    const double* row = P.t + (((int64_t)l * nch + c) * rows + p) * mc;
+    for (int b = 0; b < mc; ++b) {
+      const int m = c * mc + b;
+      if (m >= M) break;
+      const double v = row[b];
+      const double d = v - dt;
+      const double* sg = P.sigma + ((int64_t)m * L + l) * dim;
+      s += P.t_coeff[(int64_t)m * L + l] * v;
+      j0 += sg[0] * d;
+      j1 += sg[1] * d;
+      if (dim == 3) j2 += sg[2] * d;
+    }
   }
+  const int64_t o0 = gid;  // [L][n]
   P.part_t[o0] = s;
   P.part_j[o0 * dim] = j0;
   P.part_j[o0 * dim + 1] = j1;
@@ -901,7 +892,7 @@ __global__ void k_reduce_cells(S src) {
   if (c < P.n) {
     double j0 = 0.0, j1 = 0.0, j2 = 0.0;
     for (int l = 0; l < P.n_dirs; ++l) {
-      const int64_t o = (int64_t)l * P.n_chunks * P.n + P.pos[(int64_t)l * P.n + c];
+      const int64_t o = (int64_t)l * P.n + P.pos[(int64_t)l * P.n + c];
       s += P.part_t[o];
       j0 += P.part_j[o * dim + 0];
       j1 += P.part_j[o * dim + 1];
@@ -1165,7 +1156,7 @@ int obte_bte_reduce(const obte_bte_plan_t* plan, double* delta_t_new, double* j_
   src.rio.tb_new = tb_new;
   src.rio.norm2 = norm2;
   const int64_t total = (int64_t)plan->n_dirs * plan->n;
-  hipLaunchKernelGGL((k_reduce_chunks<One>), dim3(blocks_for(total, 256), 1), dim3(256), 0, st, src);
+  hipLaunchKernelGGL((k_modesum<One>), dim3(blocks_for(total, 256), 1), dim3(256), 0, st, src);
   hipLaunchKernelGGL((k_reduce_cells<One>), dim3(blocks_for(plan->n, 256), 1), dim3(256), 0, st, src);
   if (plan->n_wall > 0)
     hipLaunchKernelGGL((k_reduce_walls<One>), dim3(blocks_for(plan->n_wall, 256), 1), dim3(256), 0, st, src);
@@ -1234,7 +1225,7 @@ int obte_batch_reduce(const obte_bte_batch_t* b, obte_stream_t stream) {
   if (rc) return rc;
   hipStream_t st = (hipStream_t)stream;
   Many src = make_many(b, 0);
-  hipLaunchKernelGGL((k_reduce_chunks<Many>), dim3(blocks_for(b->max_elems, 256), b->n_problems), dim3(256), 0, st,
+  hipLaunchKernelGGL((k_modesum<Many>), dim3(blocks_for(b->max_elems, 256), b->n_problems), dim3(256), 0, st,
                      src);
   hipLaunchKernelGGL((k_reduce_cells<Many>), dim3(blocks_for(b->max_cells, 256), b->n_problems), dim3(256), 0, st,
                      src);

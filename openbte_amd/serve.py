@@ -279,6 +279,13 @@ class BTEServer:
         self.stats.completed += 1
         self.stats.latencies.append(x.req.latency)
 
+    def wait_prepared(self, poll_s: float = 0.001) -> None:
+        """Block until every submitted request has finished host preparation."""
+        while self.preparing:
+            self._collect_prepared()
+            if self.preparing:
+                time.sleep(poll_s)
+
     def pending(self) -> int:
         return len(self.preparing) + len(self.ready) + len(self.live)
 

@@ -56,12 +56,15 @@ def test_served_results_equal_alone(alone):
     # staggered arrivals: two at once, the others while those run
     srv.submit(reqs[0])
     srv.submit(reqs[1])
-    while len(srv.live) < 2:
-        srv.step()
+    srv.wait_prepared()
+    srv.step()
+    assert len(srv.live) == 2
     srv.step()
     srv.submit(reqs[2])
+    srv.wait_prepared()
     srv.step()
     srv.submit(reqs[3])
+    srv.wait_prepared()
     srv.run_until_idle()
     srv.close()
     assert srv.stats.peak_live >= 3
