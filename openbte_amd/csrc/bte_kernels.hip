@@ -839,41 +839,64 @@ __global__ void k_gcr_reset(S src, int use_dt) {
 // Stage 1 (openbte/bte.py:222-230): mode sums of each direction in sweep
 // order, straight from the solution workspace of the last full sweep:
 //   part_t[l][p] = sum_m t_ml T_ml,   part_j[l][p] = sum_m sigma_ml (T_ml - DeltaT_old)
-// (chunks then bins, a fixed order), in [L][n].  One thread per
-// (direction, position): neighbouring threads read neighbouring rows.
-template <class S>
-__global__ void k_modesum(S src) {
-  const int pid = blockIdx.y;
+// (chunks then bins, a fixed order), in [L][n].  Grid: x = 256 positions,
+// y = direction, z = problem; the direction's weights are staged in LDS and
+// each thread reads its rows as 16-byte vectors (neighbouring threads read
+// neighbouring rows).
+constexpr int kModesumMaxM = 1024;
+
+template <class S, int MC>
+__global__ __launch_bounds__(256) void k_modesum(S src) {
+  __shared__ double w[4 * kModesumMaxM];  // t, sigma_x, sigma_y, sigma_z per bin
+  const int pid = blockIdx.z;
   if (!src.live(pid)) return;
   const obte_bte_plan_t& P = src.plan(pid);
-  const int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  const int64_t total = (int64_t)P.n_dirs * P.n;
-  if (gid >= total) return;
-  const int l = (int)(gid / P.n);
-  const int p = (int)(gid - (int64_t)l * P.n);
-  const int dim = P.dim, L = P.n_dirs, M = P.n_mfp, mc = P.mc, nch = P.n_chunks;
+  const int l = blockIdx.y;
+  const int L = P.n_dirs, M = P.n_mfp, dim = P.dim, nch = P.n_chunks;
+  if (l >= L || (int64_t)blockIdx.x * blockDim.x >= P.n) return;  // uniform over the block
+  for (int m = threadIdx.x; m < M; m += blockDim.x) {
+    const double* sg = P.sigma + ((int64_t)m * L + l) * dim;
+    w[4 * m + 0] = P.t_coeff[(int64_t)m * L + l];
+    w[4 * m + 1] = sg[0];
+    w[4 * m + 2] = sg[1];
+    w[4 * m + 3] = dim == 3 ? sg[2] : 0.0;
+  }
+  __syncthreads();
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P.n) return;
+  const int64_t gid = (int64_t)l * P.n + p;
   const double dt = P.delta_t[P.order[gid]];
   const int64_t rows = (int64_t)P.n + P.ncut_max;
   double s = 0.0, j0 = 0.0, j1 = 0.0, j2 = 0.0;
+  constexpr int H = MC / 2 < 8 ? MC / 2 : 8;  // 16-byte loads in flight per thread
   for (int c = 0; c < nch; ++c) {
-    const double* row = P.t + (((int64_t)l * nch + c) * rows + p) * mc;
-    for (int b = 0; b < mc; ++b) {
-      const int m = c * mc + b;
-      if (m >= M) break;
-      const double v = row[b];
-      const double d = v - dt;
-      const double* sg = P.sigma + ((int64_t)m * L + l) * dim;
-      s += P.t_coeff[(int64_t)m * L + l] * v;
-      j0 += sg[0] * d;
-      j1 += sg[1] * d;
-      if (dim == 3) j2 += sg[2] * d;
+    const double2* row = reinterpret_cast<const double2*>(P.t + (((int64_t)l * nch + c) * rows + p) * MC);
+    for (int h0 = 0; h0 < MC / 2; h0 += H) {
+      double2 v[H];
+#pragma unroll
+      for (int h = 0; h < H; ++h) v[h] = row[h0 + h];
+#pragma unroll
+      for (int h = 0; h < H; ++h) {
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+          const int m = c * MC + 2 * (h0 + h) + e;
+          if (m < M) {
+            const double x = e ? v[h].y : v[h].x;
+            const double d = x - dt;
+            const double* wm = w + 4 * m;
+            s += wm[0] * x;
+            j0 += wm[1] * d;
+            j1 += wm[2] * d;
+            j2 += wm[3] * d;
+          }
+        }
+      }
     }
   }
-  const int64_t o0 = gid;  // [L][n]
-  P.part_t[o0] = s;
-  P.part_j[o0 * dim] = j0;
-  P.part_j[o0 * dim + 1] = j1;
-  if (dim == 3) P.part_j[o0 * dim + 2] = j2;
+  P.part_t[gid] = s;
+  P.part_j[gid * dim] = j0;
+  P.part_j[gid * dim + 1] = j1;
+  if (dim == 3) P.part_j[gid * dim + 2] = j2;
 }
 
 // Stage 2: fixed-order sum over directions, gathering each direction's
@@ -976,6 +999,7 @@ int check_plan(const obte_bte_plan_t* P, const char* who) {
   if (P->block < 64 || P->block > kSweepBlock || (P->block & (P->block - 1)) != 0)
     return obte_fail(OBTE_EINVAL, "plan: block must be 64, 128 or 256");
   if (P->kin != 2 && P->kin != 3) return obte_fail(OBTE_EINVAL, "plan: kin must be 2 or 3");
+  if (P->n_mfp > kModesumMaxM) return obte_fail(OBTE_ELIMIT, "plan: more than 1024 MFP bins");
   if (P->ncut_max < 0) return obte_fail(OBTE_EINVAL, "plan: ncut_max < 0");
   if ((P->n + 255) / 256 > 4096) return obte_fail(OBTE_ELIMIT, "plan: more than 4096*256 cells (scratch size)");
   return OBTE_OK;
@@ -1019,6 +1043,14 @@ struct SweepMany {
     if (mode == 0) hipLaunchKernelGGL((k_sweep_batch<MC, 0, KIN>), grid, blk, 0, st, src, qoff, qcnt);
     else if (mode == 1) hipLaunchKernelGGL((k_sweep_batch<MC, 1, KIN>), grid, blk, 0, st, src, qoff, qcnt);
     else hipLaunchKernelGGL((k_sweep_batch<MC, 2, KIN>), grid, blk, 0, st, src, qoff, qcnt);
+  }
+};
+
+template <int MC, int KIN>
+struct ModesumL {
+  template <class S>
+  static void run(const S& src, dim3 grid, hipStream_t st) {
+    hipLaunchKernelGGL((k_modesum<S, MC>), grid, dim3(256), 0, st, src);
   }
 };
 
@@ -1155,8 +1187,7 @@ int obte_bte_reduce(const obte_bte_plan_t* plan, double* delta_t_new, double* j_
   src.rio.j_new = j_new;
   src.rio.tb_new = tb_new;
   src.rio.norm2 = norm2;
-  const int64_t total = (int64_t)plan->n_dirs * plan->n;
-  hipLaunchKernelGGL((k_modesum<One>), dim3(blocks_for(total, 256), 1), dim3(256), 0, st, src);
+  dispatch<ModesumL>(plan->mc, 2, src, dim3(blocks_for(plan->n, 256), plan->n_dirs, 1), st);
   hipLaunchKernelGGL((k_reduce_cells<One>), dim3(blocks_for(plan->n, 256), 1), dim3(256), 0, st, src);
   if (plan->n_wall > 0)
     hipLaunchKernelGGL((k_reduce_walls<One>), dim3(blocks_for(plan->n_wall, 256), 1), dim3(256), 0, st, src);
@@ -1225,8 +1256,7 @@ int obte_batch_reduce(const obte_bte_batch_t* b, obte_stream_t stream) {
   if (rc) return rc;
   hipStream_t st = (hipStream_t)stream;
   Many src = make_many(b, 0);
-  hipLaunchKernelGGL((k_modesum<Many>), dim3(blocks_for(b->max_elems, 256), b->n_problems), dim3(256), 0, st,
-                     src);
+  dispatch<ModesumL>(b->mc, 2, src, dim3(blocks_for(b->max_cells, 256), b->max_dirs, b->n_problems), st);
   hipLaunchKernelGGL((k_reduce_cells<Many>), dim3(blocks_for(b->max_cells, 256), b->n_problems), dim3(256), 0, st,
                      src);
   if (b->max_walls > 0)
