@@ -628,6 +628,11 @@ __device__ void gs_pass(const obte_gcr_t& G, int64_t stride, int64_t base, int n
   }
 }
 
+// true on every thread if pred holds on any thread of the block
+__device__ __forceinline__ bool block_any(bool pred) {
+  return __syncthreads_or(pred ? 1 : 0) != 0;
+}
+
 struct GcrView {
   int64_t stride;  // elements of one interface vector set = n_chunks * n_cut * MC
   int64_t base;    // offset of this (l, chunk) block
@@ -770,15 +775,31 @@ __global__ __launch_bounds__(kSweepBlock) void k_gcr_end(S src) {
     un[o] = act ? w[o] : 0.0;
     cn[o] = act ? w[o] - x[o] : 0.0;
   }
-  // two block Gram-Schmidt passes ("twice is enough") for orthogonality
-  gs_pass<MC>(G, v.stride, v.base, v.ncut, kb, cn, un, red, bc);
-  gs_pass<MC>(G, v.stride, v.base, v.ncut, kb, cn, un, red, bc);
+  // block Gram-Schmidt; a second pass only where the first one cancelled
+  // more than half of the vector's norm ("twice is enough", Kahan-Parlett)
   double part = 0.0;
   for (int f = slot; f < v.ncut; f += cpp) {
     const int64_t o = (int64_t)f * MC + lane;
     part += cn[o] * cn[o];
   }
-  const double nu = sqrt(block_lane_sum<MC>(part, red, bc));
+  const double n_before = block_lane_sum<MC>(part, red, bc);
+  gs_pass<MC>(G, v.stride, v.base, v.ncut, kb, cn, un, red, bc);
+  part = 0.0;
+  for (int f = slot; f < v.ncut; f += cpp) {
+    const int64_t o = (int64_t)f * MC + lane;
+    part += cn[o] * cn[o];
+  }
+  double n_after = block_lane_sum<MC>(part, red, bc);
+  if (block_any(n_after < 0.25 * n_before)) {  // uniform over the block (it holds barriers)
+    gs_pass<MC>(G, v.stride, v.base, v.ncut, kb, cn, un, red, bc);
+    part = 0.0;
+    for (int f = slot; f < v.ncut; f += cpp) {
+      const int64_t o = (int64_t)f * MC + lane;
+      part += cn[o] * cn[o];
+    }
+    n_after = block_lane_sum<MC>(part, red, bc);
+  }
+  const double nu = sqrt(n_after);
   const double inv = nu > 0.0 ? 1.0 / nu : 0.0;
   double* y = G.y + v.base;
   double* r = G.r + v.base;
