@@ -13,6 +13,7 @@ of that problem alone produces.
 from __future__ import annotations
 
 import ctypes as C
+import threading
 import time
 from typing import List, Optional, Sequence
 
@@ -25,6 +26,7 @@ from .objects import SolverResults
 from .operator import build_host_operator
 
 N_QUEUES = 8
+_CAPTURE_LOCK = threading.Lock()
 
 
 def _struct_tensor(structs, dev) -> torch.Tensor:
@@ -225,11 +227,17 @@ class BTEBatch:
         self.round_log = torch.zeros(self.cfg.max_extend + 2, dtype=torch.int32, device=self.dev)
 
     def _capture(self, fn, *args):
+        # one capture at a time in the process: capture begins with a device
+        # synchronisation, which is not allowed while another thread captures
+        with _CAPTURE_LOCK:
+            return self._capture_locked(fn, *args)
+
+    def _capture_locked(self, fn, *args):
         g = torch.cuda.CUDAGraph()
         s = torch.cuda.Stream(device=self.dev)
         s.wait_stream(torch.cuda.current_stream(self.dev))
         with torch.cuda.stream(s):
-            with torch.cuda.graph(g, stream=s):
+            with torch.cuda.graph(g, stream=s, capture_error_mode='thread_local'):
                 fn(*args)
         torch.cuda.current_stream(self.dev).wait_stream(s)
         return g
