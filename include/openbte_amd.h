@@ -319,6 +319,20 @@ int obte_tim_face_grad(const obte_bte_plan_t* plan, const double* t_fwd, const d
                        const double* jump0, const double* vol, int32_t n_faces, double* grad,
                        obte_stream_t stream);
 
+/* ---- material interpolation ------------------------------------------------
+ *
+ * Splits n_modes phonon modes onto the (MFP bin, direction) grid and adds
+ * them into w_out [n_mfp][n_ang] and sigma_out [n_mfp][n_ang][dim]
+ * (zero-initialised by the caller; n_ang = n_phi in 2D, n_theta * n_phi in
+ * 3D, direction index theta * n_phi + phi): replaces the np.add.at scatter of
+ * openbte/material.py:59-77 (RTA3D) and 164-173 (RTA2DSym) with the index
+ * rule of openbte/utils.py:281-320.  Per mode: weight W = C / tau, sigma = C v
+ * and the vectorial MFP v tau ([n_modes][3], z ignored in 2D); grids
+ * increasing, phi periodic in [0, 2 pi). */
+int obte_rta_scatter(int32_t n_modes, int32_t dim, const double* weight, const double* sigma, const double* mfp,
+                     int32_t n_mfp, const double* mfp_grid, int32_t n_phi, const double* phi_grid, int32_t n_theta,
+                     const double* theta_grid, double* w_out, double* sigma_out, obte_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

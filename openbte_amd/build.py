@@ -1,6 +1,7 @@
 """Build libopenbte_amd.so for gfx950 in-tree (hipcc, no JIT cache).
 
-The library holds the HIP kernels (bte_kernels.hip, fourier_kernels.hip, tim_kernels.hip),
+The library holds the HIP kernels (bte_kernels.hip, fourier_kernels.hip, tim_kernels.hip,
+material_kernels.hip),
 the native host scheduler (schedule.cpp) and the C-ABI error state
 (capi.cpp).  All HIP code is compiled with packed-FP32 instruction
 generation disabled.
@@ -19,7 +20,7 @@ CSRC = os.path.join(HERE, 'csrc')
 LIB = os.path.join(HERE, 'libopenbte_amd.so')
 ARCH = os.environ.get('OBTE_OFFLOAD_ARCH', 'gfx950')
 
-HIP_SOURCES = ['bte_kernels.hip', 'fourier_kernels.hip', 'tim_kernels.hip']
+HIP_SOURCES = ['bte_kernels.hip', 'fourier_kernels.hip', 'tim_kernels.hip', 'material_kernels.hip']
 CPP_SOURCES = ['schedule.cpp', 'capi.cpp']
 
 

@@ -77,6 +77,18 @@ class Material(NamedTuple):
     heat_source_coeff: float
 
 
+class MaterialRTA(NamedTuple):
+    """Phonon modes of a relaxation-time material (openbte/objects.py:114-120):
+    per mode heat capacity (W s/m^3/K), scattering time (s), group velocity
+    (m/s, 3 components) and frequency; the bulk conductivity tensor (W/m/K)."""
+
+    heat_capacity: np.ndarray
+    scattering_time: np.ndarray
+    group_velocity: np.ndarray
+    frequency: np.ndarray = np.zeros(0)
+    thermal_conductivity: np.ndarray = np.zeros(0)
+
+
 class BoundaryConditions(NamedTuple):
     """Boundary conditions (openbte/objects.py:220-241).
 
