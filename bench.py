@@ -160,10 +160,11 @@ def run_batch(args):
     scheds = [build_schedule(o) for o in ops]
     tuned = None
     if args.mc or args.block:
-        cfg = SweepConfig(mc=args.mc or 8, block=args.block or 256)
+        cfg = SweepConfig(mc=args.mc or 8, block=args.block or 256, window=not args.no_window)
     else:
         from openbte_amd import tune
         cfg = tune.tune(ops, scheds)
+        cfg.window = not args.no_window
         tuned = tune.last_results(ops)
         print(f'# autotuned mc={cfg.mc} block={cfg.block}', file=sys.stderr, flush=True)
     K = args.steps
@@ -204,7 +205,7 @@ def run_batch(args):
             del dev
         torch.cuda.empty_cache()
         return dt, ks
-    plain_runs = {(8, 256): plain(SweepConfig())}
+    plain_runs = {(8, 256): plain(SweepConfig(window=not args.no_window))}
     if (cfg.mc, cfg.block) not in plain_runs:
         plain_runs[(cfg.mc, cfg.block)] = plain(cfg)
     best_plain = min(plain_runs, key=lambda k: plain_runs[k][0])
@@ -459,6 +460,7 @@ def main(argv=None):
                     choices=sorted(CONFIGS) + sorted(BATCH_CONFIGS) + sorted(SERVE_CONFIGS)
                     + sorted(INVERSE_CONFIGS))
     ap.add_argument('--no-graphs', action='store_true', help='batched configs: launch without hipGraphs')
+    ap.add_argument('--no-window', action='store_true', help='sweeps read every upwind value from HBM')
     ap.add_argument('--mc', type=int, default=None)
     ap.add_argument('--block', type=int, default=None)
     ap.add_argument('--no-cpu', action='store_true')
@@ -509,7 +511,7 @@ def main(argv=None):
                         'cg_iterations': [c[0] for c in f.aux['cg']],
                         'kappa_diffusive': f.aux['kappa_diffusive']}
         print(f'# fourier done ({f.aux["n_iter"]} iterations, {tf:.2f}s)', file=sys.stderr, flush=True)
-    cfg = SweepConfig()
+    cfg = SweepConfig(window=not args.no_window)
     if args.mc:
         cfg.mc = args.mc
     if args.block:
@@ -584,7 +586,8 @@ def main(argv=None):
         'data': 'synthetic (seeded jittered-Delaunay pore cell, seeded lognormal Si-like material)',
         'config': {'workload': f'{args.config}: L={CONFIGS[args.config][0]:g}nm porosity 0.25, n={n} cells, '
                                f'{M} MFP x {L} directions = {M * L} systems',
-                   'mc': cfg.mc, 'block': cfg.block, 'kappa_eff_after_steps': kappa,
+                   'mc': cfg.mc, 'block': cfg.block, 'lds_window_positions': dev.window_positions(),
+                   'kappa_eff_after_steps': kappa,
                    'extensions_per_step': exts, 'n_cut_faces': dev.n_cut,
                    'outer_error_per_step': errors, 'kmax': cfg.kmax, 'gcr_rtol': cfg.rtol,
                    'fourier': fourier_info, 'setup_s': t_setup},

@@ -40,10 +40,11 @@ const char* obte_last_error(void);
 
 /* Topological sweep order of every direction's upwind graph with the
  * periodic inflow faces (cut_candidate[f] != 0) cut.  i_mat/j_mat/k_mat are
- * the directed internal faces of openbte/bte.py:48-54 (k = n A / V_row). */
+ * the directed internal faces of openbte/bte.py:48-54 (k = n A / V_row).
+ * max_width > 0 caps the cells of one level (0: longest-path levels). */
 int obte_sched_create(int32_t n, int64_t n_faces, int32_t dim, const int32_t* i_mat,
                       const int32_t* j_mat, const double* k_mat, const uint8_t* cut_candidate,
-                      int32_t n_dirs, const double* dirs, void** handle);
+                      int32_t n_dirs, const double* dirs, int32_t max_width, void** handle);
 /* sizes[3*l + {0,1,2}] = levels, inflow entries, cut faces of direction l. */
 int obte_sched_sizes(void* handle, int64_t* sizes);
 int obte_sched_export(void* handle, int32_t* order, int32_t* pos, int32_t* lvl_ptr, int32_t* in_ptr,
@@ -76,6 +77,9 @@ typedef struct obte_bte_plan {
   const int32_t* lvl_ptr;  /* level start positions */
   int32_t kin;             /* inflow slots per position (nv - 1) */
   int32_t ncut_max;        /* max cut faces of one direction */
+  int32_t lvl_pair_max;    /* max over directions and k of |level k| + |level k+1| (0: unknown) */
+  int32_t sweep_window;    /* 1: sweeps may keep recent levels in an LDS window (needs
+                              lvl_pair_max), 0: every upwind value is read from t */
   const int32_t* nb_src;   /* [L][kin][n] upstream row: position (< n), interface row n + cut,
                               or the position itself with a zero coefficient for unused slots */
   const int32_t* nb_face;  /* [L][kin][n] directed face of the slot, -1 if unused */
@@ -117,14 +121,18 @@ typedef struct obte_bte_plan {
   double* rec;             /* [L][n][8] sweep record of each position (64 B):
                               [0] D_l, [1] b0 = DeltaT + heat, [2] q = pert + diffuse-wall source,
                               [3..5] -min(s . k, 0) of the inflow slots (0 if unused),
-                              [6] int32 pair (src0, src1), [7] int32 pair (src2, cell);
+                              [6] int32 pair (src0, src1), [7] int32 pair (src2, keep): keep = 1
+                              when a later position reads the value from t (outside the LDS
+                              window of the smallest window a sweep may use) or it is a cut
+                              exit; modes 0 and 1 of a windowed sweep store only those;
                               [1] and [2] are rewritten by obte_bte_source every outer iteration */
 
   /* state */
   const double* delta_t;   /* [n] lattice temperature of the previous iteration */
   const double* tb;        /* [n_wall] wall temperatures of the previous iteration */
-  double* t;               /* [L][n_chunks][n + ncut_max][mc] solution workspace; rows >= n
-                              hold the interface inflows of the running sweep */
+  double* t;               /* [L][n_chunks][n + ncut_max + 1][mc] solution workspace; rows
+                              n .. n + ncut_max - 1 hold the interface inflows of the running
+                              sweep, the last row is a store target for values nothing reads */
   double* part_t;          /* [L][n] per-direction lattice temperature, sweep order */
   double* part_j;          /* [L][n][dim] per-direction flux, sweep order */
   double* part_tb;         /* [L][n_chunks][n_wall] */
@@ -169,6 +177,12 @@ int obte_bte_source(const obte_bte_plan_t* plan, obte_stream_t stream);
  * converged MFP bins (or NULL). */
 int obte_bte_sweep(const obte_bte_plan_t* plan, int32_t mode, const double* y, double* x,
                    const int32_t* active, const int32_t* lane_active, obte_stream_t stream);
+
+/* Positions of the LDS window that obte_bte_sweep / obte_batch_sweep use for
+ * a plan with these fields (a power of two >= lvl_pair_max, at most 48 KB of
+ * LDS), or 0 when sweeps read every upwind value from t.  Host-only query
+ * (no counterpart in the reference: a launch-shape detail). */
+int32_t obte_sweep_window(int32_t sweep_window, int32_t lvl_pair_max, int32_t mc);
 
 /* Interface Krylov layer: GCR with a recycled basis on the cut-face
  * equations y = R S(B, y).  The transfer operator does not depend on the
@@ -225,6 +239,8 @@ typedef struct obte_bte_batch {
   int32_t max_dirs;      /* max n_dirs */
   int32_t max_cells;     /* max n */
   int32_t max_walls;     /* max n_wall */
+  int32_t lvl_pair_max;  /* max of the plans' lvl_pair_max (0: unknown) */
+  int32_t sweep_window;  /* as in the plan */
   const obte_bte_plan_t* plans;  /* [n_problems] */
   const obte_gcr_t* gcrs;        /* [n_problems] */
   const obte_sweep_io_t* io;     /* [3][n_problems]: sweep buffers of modes 0, 1, 2 */

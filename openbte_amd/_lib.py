@@ -24,6 +24,7 @@ class BtePlan(C.Structure):
         ('block', _i32), ('n_cut', _i32), ('n_wall', _i32), ('n_contact', _i32), ('n_faces', _i32),
         ('contact_kind', _i32), ('normalization', _f64), ('contact_jump', _f64),
         ('order', _p), ('pos', _p), ('lvl_off', _p), ('lvl_ptr', _p), ('kin', _i32), ('ncut_max', _i32),
+        ('lvl_pair_max', _i32), ('sweep_window', _i32),
         ('nb_src', _p), ('nb_face', _p), ('cut_off', _p), ('cut_exit', _p),
         ('dirs', _p), ('k_face', _p), ('jump_face', _p), ('cf_ptr', _p), ('cf_face', _p),
         ('wall_ptr', _p), ('wall_side', _p), ('wall_db', _p), ('wall_temp', _p), ('wall_cell', _p),
@@ -59,6 +60,7 @@ class Batch(C.Structure):
     _fields_ = [
         ('n_problems', _i32), ('n_tasks', _i32), ('mc', _i32), ('block', _i32), ('kin', _i32), ('groups', _i32),
         ('max_elems', _i32), ('max_dirs', _i32), ('max_cells', _i32), ('max_walls', _i32),
+        ('lvl_pair_max', _i32), ('sweep_window', _i32),
         ('plans', _p), ('gcrs', _p), ('io', _p), ('rio', _p), ('live', _p), ('task_pid', _p), ('task_local', _p),
         ('queue_off', _p), ('queue_cnt', _p),
     ]
@@ -79,7 +81,7 @@ class FourierPlan(C.Structure):
 # name -> (argtypes)
 _SIGNATURES = {
     'obte_last_error': [],
-    'obte_sched_create': [_i32, _i64, _i32, _p, _p, _p, _p, _i32, _p, C.POINTER(_p)],
+    'obte_sched_create': [_i32, _i64, _i32, _p, _p, _p, _p, _i32, _p, _i32, C.POINTER(_p)],
     'obte_sched_sizes': [_p, _p],
     'obte_sched_export': [_p, _p, _p, _p, _p, _p, _p, _p, _p],
     'obte_sched_destroy': [_p],
@@ -90,6 +92,7 @@ _SIGNATURES = {
     'obte_gcr_begin_extend': [C.POINTER(BtePlan), C.POINTER(Gcr), _p],
     'obte_gcr_end_extend': [C.POINTER(BtePlan), C.POINTER(Gcr), _p],
     'obte_gcr_reset': [C.POINTER(BtePlan), C.POINTER(Gcr), _i32, _p],
+    'obte_sweep_window': [_i32, _i32, _i32],
     'obte_batch_assemble': [C.POINTER(Batch), _p],
     'obte_batch_source': [C.POINTER(Batch), _p],
     'obte_batch_sweep': [C.POINTER(Batch), _i32, _p],

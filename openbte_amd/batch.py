@@ -152,6 +152,8 @@ class BTEBatch:
         B.max_dirs = max(d.op.n_dirs for d in devs)
         B.max_cells = max(d.op.n for d in devs)
         B.max_walls = max(len(d.op.i_diffuse) for d in devs)
+        B.lvl_pair_max = max(d.lvl_pair_max for d in devs)
+        B.sweep_window = int(bool(self.cfg.window))
         B.plans, B.gcrs, B.io, B.rio = (self.plans_t.data_ptr(), self.gcrs_t.data_ptr(), self.io_t.data_ptr(),
                                         self.rio_t.data_ptr())
         B.live = self.live.data_ptr()

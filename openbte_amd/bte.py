@@ -28,7 +28,8 @@ import torch
 
 from . import _lib
 from .objects import SolverResults
-from .operator import HostOperator, Schedule, build_host_operator, build_schedule, fixed_width_inflow, max_inflow
+from .operator import (HostOperator, Schedule, build_host_operator, build_schedule, fixed_width_inflow,
+                       level_pair_max, max_inflow)
 
 
 @dataclass
@@ -40,6 +41,7 @@ class SweepConfig:
     rtol: float = 1e-10      # interface residual tolerance
     max_extend: int = 64     # cap on basis extensions per outer iteration
     basis_bytes: float = 24e9  # HBM budget of the recycled basis (u and c)
+    window: bool = True      # keep recent levels of a sweep in LDS where they fit
 
 
 @dataclass
@@ -90,7 +92,7 @@ def device_bytes(op: HostOperator, sched: Schedule, cfg: Optional[SweepConfig] =
         f8 += len(op.t_face) + max(1, len(op.c_t))          # t_face, c_t
     f8 += L * n + L * n * 8                                  # pert, rec
     f8 += 2 * n + 2 * one(n_wall) + n * dim                  # dt, tb, j
-    f8 += L * nch * (n + ncut_max) * mc                      # t
+    f8 += L * nch * (n + ncut_max + 1) * mc                  # t
     f8 += L * n * (1 + dim) + one(L * nch * n_wall)          # part_t, part_j, part_tb
     f8 += 2 * M * L + 4096 + 8                               # kappa, supp, scratch
     ny = max(1, nch * n_cut * mc)
@@ -117,6 +119,7 @@ class BTEDevice:
         self.n_chunks = nch
         self.n_cut = int(sc.cut_off[-1])
         self.ncut_max = int(np.diff(sc.cut_off).max()) if L > 0 else 0
+        self.lvl_pair_max = level_pair_max(sc)
         n_wall = len(op.i_diffuse)
         self.kin = max(2, max_inflow(sc))
         if self.kin > 3:
@@ -176,7 +179,7 @@ class BTEDevice:
         b['tb'] = [torch.zeros(max(1, n_wall), dtype=f64, device=dev) for _ in range(2)]
         b['j'] = torch.zeros(n * dim, dtype=f64, device=dev)
         # zero-initialised: unused inflow slots read a finite value times 0
-        b['t'] = torch.zeros(L * nch * (n + self.ncut_max) * mc, dtype=f64, device=dev)
+        b['t'] = torch.zeros(L * nch * (n + self.ncut_max + 1) * mc, dtype=f64, device=dev)
         b['part_t'] = torch.empty(L * n, dtype=f64, device=dev)
         b['part_j'] = torch.empty(L * n * dim, dtype=f64, device=dev)
         b['part_tb'] = torch.empty(max(1, L * nch * n_wall), dtype=f64, device=dev)
@@ -212,6 +215,10 @@ class BTEDevice:
         ts += [t for k, t in self.gbuf.items() if k != 'n_active']
         return int(sum(t.numel() * t.element_size() for t in ts))
 
+    def window_positions(self) -> int:
+        """Positions of the LDS window the sweeps of this device use (0: none)."""
+        return int(_lib.load().obte_sweep_window(int(bool(self.cfg.window)), self.lvl_pair_max, self.cfg.mc))
+
     # -- C structs -------------------------------------------------------
     def _make_plan(self) -> _lib.BtePlan:
         op, b, cfg = self.op, self.buf, self.cfg
@@ -221,6 +228,7 @@ class BTEDevice:
         P.n_cut, P.n_wall, P.n_contact, P.n_faces = self.n_cut, len(op.i_diffuse), len(op.ce1), len(op.i_mat)
         P.contact_kind, P.normalization, P.contact_jump = op.contact_kind, op.normalization, op.contact_jump
         P.kin, P.ncut_max = self.kin, self.ncut_max
+        P.lvl_pair_max, P.sweep_window = self.lvl_pair_max, int(bool(cfg.window))
         for name in ('order', 'pos', 'lvl_off', 'lvl_ptr', 'nb_src', 'nb_face', 'cut_off',
                      'cut_exit', 'dirs', 'k_face', 'jump_face', 'cf_ptr', 'cf_face', 'wall_ptr', 'wall_side',
                      'wall_db', 'wall_temp', 'wall_cell', 'wall_rinv', 'wall_dbs', 'ce1', 'ce2', 'c_na', 'mfps',

@@ -140,7 +140,66 @@ __global__ void k_assemble(S src) {
   r[0] = make_double2(D, 0.0);
   r[1] = make_double2(0.0, cf[0]);
   r[2] = make_double2(cf[1], cf[2]);
-  r[3] = make_double2(pack_ints(sr[0], sr[1]), pack_ints(sr[2], c));
+  r[3] = make_double2(pack_ints(sr[0], sr[1]), pack_ints(sr[2], 0));  // keep flags: k_mark_keep
+}
+
+// Smallest LDS window (positions) any sweep of a plan with these level sizes
+// uses; 0 when the sizes are unknown (no windowed sweep then).
+__host__ __device__ inline int min_window(int pair_max) {
+  if (pair_max <= 0) return 0;
+  int cap = 64;
+  while (cap < pair_max) cap <<= 1;
+  return cap;
+}
+
+// keep flag (high word of rec[7]) of every position whose value a later
+// position reads from t rather than from the smallest LDS window: the
+// upwind source s of position p is outside the window when s < h - cap, h
+// the end of p's level.  One thread per (direction, position), blockIdx.y =
+// problem; runs after k_assemble.
+template <class S>
+__global__ void k_mark_keep(S src) {
+  const int pid = blockIdx.y;
+  const obte_bte_plan_t& P = src.plan(pid);
+  const int cap = min_window(P.lvl_pair_max);
+  const int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (cap == 0 || gid >= (int64_t)P.n_dirs * P.n) return;
+  const int n = P.n;
+  const int l = (int)(gid / n);
+  const int p = (int)(gid - (int64_t)l * n);
+  const int32_t* lp = P.lvl_ptr + P.lvl_off[l];
+  int lo = 1, hi = P.lvl_off[l + 1] - P.lvl_off[l] - 1;  // first level start > p
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (lp[mid] > p) hi = mid;
+    else lo = mid + 1;
+  }
+  const int h = lp[lo];
+  int32_t* words = reinterpret_cast<int32_t*>(P.rec);
+  for (int kk = 0; kk < P.kin; ++kk) {
+    const int64_t e = ((int64_t)l * P.kin + kk) * n + p;
+    const int s = P.nb_src[e];
+    if (P.nb_face[e] >= 0 && s < n && s < h - cap) atomicOr(&words[(((int64_t)l * n + s) * kRec + 7) * 2 + 1], 1);
+  }
+}
+
+// keep flag of the exit position of every cut face (its value leaves the
+// sweep as x); grid-stride over the cuts, blockIdx.y = problem
+template <class S>
+__global__ void k_mark_exits(S src) {
+  const int pid = blockIdx.y;
+  const obte_bte_plan_t& P = src.plan(pid);
+  if (min_window(P.lvl_pair_max) == 0) return;
+  int32_t* words = reinterpret_cast<int32_t*>(P.rec);
+  for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < P.n_cut; f += gridDim.x * blockDim.x) {
+    int lo = 0, hi = P.n_dirs;  // direction l with cut_off[l] <= f < cut_off[l + 1]
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (P.cut_off[mid] <= f) lo = mid;
+      else hi = mid;
+    }
+    atomicOr(&words[(((int64_t)lo * P.n + P.cut_exit[f]) * kRec + 7) * 2 + 1], 1);
+  }
 }
 
 // q = pert + sum over diffuse walls of (-TB_s) min(s . db, 0)   (openbte/bte.py:199-200,214)
@@ -239,14 +298,61 @@ struct SweepShared {
   int lvl[kLvlStage + 2];  // staged level starts of the running direction
 };
 
-template <int MC>
-constexpr int units_of() { return MC >= 16 ? 2 : 4; }
+// cells per thread and level pass; the windowed sweep also holds the far
+// upwind values of two levels, so it takes fewer (one 256-thread pass covers
+// 192 cells at MC = 8 and 256 at MC = 4)
+template <int MC, bool WIN>
+constexpr int units_of() { return WIN ? (MC == 8 ? 3 : 2) : (MC >= 16 ? 2 : 4); }
 
-// One (problem pid, task lt = l * n_chunks + chunk) sweep by the whole block.
-template <class S, int MC, int MODE, int KIN>
-__device__ void sweep_task(const S& src, int pid, int lt, SweepShared<MC>& sh) {
+// ---------------------------------------------------------------------------
+// Windowed sweep.  The values of the last `cap` positions (cap a power of two
+// holding any two adjacent levels) are also kept in LDS, slot
+// (position & (cap - 1)) * (MC / 2) + lane.  An upwind value inside that
+// window is read from LDS; one outside it belongs to a level at least two back
+// and is loaded from T one level ahead (its store went out before the barrier
+// that precedes the load; the waves of a workgroup share the CU's vector L1,
+// so a store issued before s_barrier is seen by loads issued after it).  Per
+// level the critical path is then LDS reads, the arithmetic and the barrier:
+// the global loads of a level are issued a level (records: two) ahead and the
+// only wait is for operations of the previous level.
+//
+// Every global load and store of the level loop is unconditional, so the
+// number of vector-memory operations a wave has in flight is static: slots
+// past the end of a level recompute the level's last cell (same inputs, same
+// value, same address), and near or converged lanes load row 0 instead of a
+// far row.  The wait at the top of a level, vmcnt <= U, leaves exactly the U
+// stores of the previous level in flight (vector-memory operations complete
+// in issue order).
+// ---------------------------------------------------------------------------
+extern __shared__ double2 win_lds[];
+
+template <int U, int NP, int KIN>
+struct WinRegs {
+  int p[U];               // position (slots past the level: its last cell)
+  double2 pc[U][NP];      // this lane's pieces of the record
+  double2 far[U][KIN];    // upwind values loaded from T (outside the window)
+};
+
+template <int U, int NP, int KIN>
+__device__ __forceinline__ void load_win(WinRegs<U, NP, KIN>& R, int base, int p1, int cps, int jq,
+                                         const double* __restrict__ rec) {
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    int p = base + u * cps;
+    p = p < p1 ? p : p1 - 1;
+    R.p[u] = p;
+    const double2* r = reinterpret_cast<const double2*>(rec + (int64_t)p * kRec) + jq * NP;
+#pragma unroll
+    for (int i = 0; i < NP; ++i) R.pc[u][i] = r[i];
+  }
+}
+
+// One (problem pid, task lt = l * n_chunks + chunk) sweep by the whole block;
+// WIN: with an LDS window of cap positions (dynamic shared memory).
+template <class S, int MC, int MODE, int KIN, bool WIN>
+__device__ void sweep_task(const S& src, int pid, int lt, SweepShared<MC>& sh, int cap) {
   constexpr int G = MC / 2;       // lanes per cell
-  constexpr int U = units_of<MC>();
+  constexpr int U = units_of<MC, WIN>();
   constexpr int GQ = G < 4 ? G : 4;  // lanes of a cell inside one quad
   constexpr int NP = 4 / GQ;         // record pieces per lane
   const obte_bte_plan_t& P = src.plan(pid);
@@ -283,7 +389,10 @@ __device__ void sweep_task(const S& src, int pid, int lt, SweepShared<MC>& sh) {
   if (MODE == 1 && io.lane_active != nullptr)
     act = io.lane_active[(int64_t)lt * MC + 2 * jl] != 0 || io.lane_active[(int64_t)lt * MC + 2 * jl + 1] != 0;
 
-  double* __restrict__ T = P.t + (int64_t)lt * ((int64_t)n + P.ncut_max) * MC;
+  // rows: n positions, ncut_max interface inflows, one spare row (store target
+  // of values that no later position reads)
+  double* __restrict__ T = P.t + (int64_t)lt * ((int64_t)n + P.ncut_max + 1) * MC;
+  const int spare = n + P.ncut_max;
   const int32_t* __restrict__ lvl_g = P.lvl_ptr + P.lvl_off[l];
   const int nlev = P.lvl_off[l + 1] - P.lvl_off[l] - 1;
   const int cbase = P.cut_off[l];
@@ -302,8 +411,111 @@ __device__ void sweep_task(const S& src, int pid, int lt, SweepShared<MC>& sh) {
   auto stage = [&](int from) {
     for (int i = tid; i < kLvlStage + 2; i += nthr) lvl[i] = lvl_g[from + i <= nlev ? from + i : nlev];
   };
+  if constexpr (WIN) {
+    // unused inflow slots read the window with a zero coefficient: no
+    // uninitialised (possibly NaN) patterns in it
+    for (int i = tid; i < cap * G; i += nthr) win_lds[i] = make_double2(0.0, 0.0);
+  }
   stage(0);
   __syncthreads();
+  // Three register sets rotate (current, next, prefetch target) without
+  // copies; the level count is padded to a multiple of 3 with empty levels so
+  // the unrolled loop has no exits.  Level starts are staged kSeg at a time.
+  constexpr int kSeg = (kLvlStage - 2) / 3 * 3;
+  const int nlev3 = (nlev + 2) / 3 * 3;
+
+  if constexpr (WIN) {
+    using WR = WinRegs<U, NP, KIN>;
+    const int cmask = cap - 1;
+    // loads of the upwind values of N (a level ending at h) that lie outside
+    // the window; near and converged lanes load row 0 (value unused)
+    auto issue_far = [&](WR& N, int h) {
+      const int lo_w = h - cap;
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const double s01 = rec_get<GQ, NP, 6>(N.pc[u]);
+        const double s2c = rec_get<GQ, NP, 7>(N.pc[u]);
+        const int sr[3] = {__double2loint(s01), __double2hiint(s01), __double2loint(s2c)};
+#pragma unroll
+        for (int kk = 0; kk < KIN; ++kk) {
+          const bool inwin = (unsigned)(sr[kk] - lo_w) < (unsigned)cap;
+          const int row = (inwin || (MODE == 1 && !act)) ? 0 : sr[kk];
+          N.far[u][kk] = *reinterpret_cast<const double2*>(T + (int64_t)row * MC + 2 * jl);
+        }
+      }
+    };
+    // the cells of C (a level ending at h): upwind values from the window or
+    // C.far, then the store to T and to the window
+    auto finish_win = [&](const WR& C, int h) {
+      const int lo_w = h - cap;
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const double dd = rec_get<GQ, NP, 0>(C.pc[u]);
+        const double bb = MODE == 1 ? 0.0 : rec_get<GQ, NP, 1>(C.pc[u]);
+        const double qq = MODE == 1 ? 0.0 : rec_get<GQ, NP, 2>(C.pc[u]);
+        const double cf[3] = {rec_get<GQ, NP, 3>(C.pc[u]), rec_get<GQ, NP, 4>(C.pc[u]),
+                              KIN == 3 ? rec_get<GQ, NP, 5>(C.pc[u]) : 0.0};
+        const double s01 = rec_get<GQ, NP, 6>(C.pc[u]);
+        const double s2c = rec_get<GQ, NP, 7>(C.pc[u]);
+        const int sr[3] = {__double2loint(s01), __double2hiint(s01), __double2loint(s2c)};
+        double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+        for (int kk = 0; kk < KIN; ++kk) {
+          const bool inwin = (unsigned)(sr[kk] - lo_w) < (unsigned)cap;
+          const double2 w = win_lds[(sr[kk] & cmask) * G + jl];
+          double2 v = inwin ? w : C.far[u][kk];
+          if (MODE == 1 && !act) v = make_double2(0.0, 0.0);
+          a0 += cf[kk] * v.x;
+          a1 += cf[kk] * v.y;
+        }
+        const double t0 = (bb + lam0 * (qq + a0)) / (1.0 + lam0 * dd);
+        const double t1 = (bb + lam1 * (qq + a1)) / (1.0 + lam1 * dd);
+        const double2 tv = make_double2(t0, t1);
+        // modes 0 and 1: only values read later from t (or as exits) go to HBM
+        const int row = (MODE == 2 || __double2hiint(s2c) != 0) ? C.p[u] : spare;
+        *reinterpret_cast<double2*>(T + (int64_t)row * MC + 2 * jl) = tv;
+        win_lds[(C.p[u] & cmask) * G + jl] = tv;
+      }
+    };
+    // level kk (cells in C); issues the far loads of level kk + 1 (N) and
+    // the records of level kk + 2 (into N2)
+    auto body = [&](int kk, WR& C, WR& N, WR& N2) {
+      const int p0 = lvl[kk];
+      const int p1 = lvl[kk + 1];
+      __builtin_amdgcn_s_waitcnt(vmcnt_only(U));
+      __builtin_amdgcn_sched_barrier(0);
+      issue_far(N, lvl[kk + 2]);
+      __builtin_amdgcn_sched_barrier(0);
+      load_win<U, NP, KIN>(N2, lvl[kk + 2] + cslot, lvl[kk + 3], cps, jq, rec);
+      __builtin_amdgcn_sched_barrier(0);
+      finish_win(C, p1);
+      // levels wider than one pass (not produced by the default schedule)
+      for (int base = p0 + U * cps; base < p1; base += U * cps) {
+        load_win<U, NP, KIN>(C, base + cslot, p1, cps, jq, rec);
+        issue_far(C, p1);
+        finish_win(C, p1);
+      }
+      __syncthreads();
+    };
+    WR A, B, C;
+    load_win<U, NP, KIN>(A, lvl[0] + cslot, lvl[1], cps, jq, rec);
+    load_win<U, NP, KIN>(B, lvl[1] + cslot, lvl[2], cps, jq, rec);
+    __builtin_amdgcn_s_waitcnt(0);
+    issue_far(A, lvl[1]);
+    __builtin_amdgcn_s_waitcnt(0);
+    for (int seg = 0; seg < nlev3; seg += kSeg) {
+      if (seg > 0) {
+        stage(seg);
+        __syncthreads();
+      }
+      const int kend = min(kSeg, nlev3 - seg);
+      for (int kk = 0; kk < kend; kk += 3) {
+        body(kk, A, B, C);
+        body(kk + 1, B, C, A);
+        body(kk + 2, C, A, B);
+      }
+    }
+  } else {
 
   using Regs = LevelRegs<U, NP>;
   // upwind gathers of the U cells of C (idle slots skip them)
@@ -370,11 +582,6 @@ __device__ void sweep_task(const S& src, int pid, int lt, SweepShared<MC>& sh) {
     __syncthreads();
   };
 
-  // Three register sets rotate (current, next, prefetch target) without
-  // copies; the level count is padded to a multiple of 3 with empty levels so
-  // the unrolled loop has no exits.  Level starts are staged kSeg at a time.
-  constexpr int kSeg = (kLvlStage - 2) / 3 * 3;
-  const int nlev3 = (nlev + 2) / 3 * 3;
   Regs A, B, C;
   load_level<U, NP>(A, lvl[0] + cslot, lvl[1], cps, jq, rec);
   load_level<U, NP>(B, lvl[1] + cslot, lvl[2], cps, jq, rec);
@@ -391,6 +598,7 @@ __device__ void sweep_task(const S& src, int pid, int lt, SweepShared<MC>& sh) {
       body(kk + 1, B, A);
       body(kk + 2, C, B);
     }
+  }
   }
 
   // exit values of the cut faces (upstream cells across the cut)
@@ -481,21 +689,21 @@ __device__ __forceinline__ int task_of_block(int b, int ntask) {
 }
 
 // single problem: one task per block
-template <int MC, int MODE, int KIN>
-__global__ __launch_bounds__(kSweepBlock) void k_sweep(One src, int ntask) {
+template <int MC, int MODE, int KIN, bool WIN>
+__global__ __launch_bounds__(kSweepBlock) void k_sweep(One src, int ntask, int cap) {
   __shared__ SweepShared<MC> sh;
   const int t = task_of_block(blockIdx.x, ntask);
   if (t < 0) return;
   const obte_sweep_io_t& io = src.io;
   if (io.active != nullptr && io.active[t] == 0) return;  // uniform over the block
-  sweep_task<One, MC, MODE, KIN>(src, 0, t, sh);
+  sweep_task<One, MC, MODE, KIN, WIN>(src, 0, t, sh, cap);
 }
 
 // batch: persistent workgroups draining 8 XCD-local task queues (tasks of one
 // (problem, direction) share a queue; a drained queue steals from the next)
-template <int MC, int MODE, int KIN>
+template <int MC, int MODE, int KIN, bool WIN>
 __global__ __launch_bounds__(kSweepBlock) void k_sweep_batch(Many src, const int32_t* __restrict__ qoff,
-                                                           int32_t* __restrict__ qcnt) {
+                                                           int32_t* __restrict__ qcnt, int cap) {
   __shared__ SweepShared<MC> sh;
   __shared__ int s_task;
   const int xcd = blockIdx.x % 8;
@@ -522,7 +730,7 @@ __global__ __launch_bounds__(kSweepBlock) void k_sweep_batch(Many src, const int
     if (!src.live(pid)) continue;
     const obte_sweep_io_t& io = src.sio(pid);
     if (io.active != nullptr && io.active[lt] == 0) continue;
-    sweep_task<Many, MC, MODE, KIN>(src, pid, lt, sh);
+    sweep_task<Many, MC, MODE, KIN, WIN>(src, pid, lt, sh, cap);
   }
 }
 
@@ -887,7 +1095,7 @@ __global__ __launch_bounds__(256) void k_modesum(S src) {
   if (p >= P.n) return;
   const int64_t gid = (int64_t)l * P.n + p;
   const double dt = P.delta_t[P.order[gid]];
-  const int64_t rows = (int64_t)P.n + P.ncut_max;
+  const int64_t rows = (int64_t)P.n + P.ncut_max + 1;
   double s = 0.0, j0 = 0.0, j1 = 0.0, j2 = 0.0;
   constexpr int H = MC / 2 < 8 ? MC / 2 : 8;  // 16-byte loads in flight per thread
   for (int c = 0; c < nch; ++c) {
@@ -1046,24 +1254,53 @@ void dispatch(int mc, int kin, A&&... a) {
   }
 }
 
+// Positions of the LDS window of a sweep with MC bins whose adjacent levels
+// hold at most pair_max cells: the smallest power of two >= pair_max (at least
+// 64), or 0 (no window) when disabled, unknown or over kWindowMaxBytes.
+constexpr int64_t kWindowMaxBytes = 48 * 1024;
+
+inline int window_cap(int enable, int pair_max, int mc) {
+  if (!enable || pair_max <= 0 || pair_max > (1 << 24) || mc > 16) return 0;
+  const int64_t cap = min_window(pair_max);
+  return cap * mc * (int64_t)sizeof(double) <= kWindowMaxBytes ? (int)cap : 0;
+}
+
 template <int MC, int KIN>
 struct SweepOne {
-  static void run(const One& src, int mode, int ntask, int block, hipStream_t st) {
+  template <bool WIN>
+  static void go(const One& src, int mode, int ntask, int block, int cap, hipStream_t st) {
     dim3 grid(((ntask + 7) / 8) * 8), blk(block);
-    if (mode == 0) hipLaunchKernelGGL((k_sweep<MC, 0, KIN>), grid, blk, 0, st, src, ntask);
-    else if (mode == 1) hipLaunchKernelGGL((k_sweep<MC, 1, KIN>), grid, blk, 0, st, src, ntask);
-    else hipLaunchKernelGGL((k_sweep<MC, 2, KIN>), grid, blk, 0, st, src, ntask);
+    const size_t lds = WIN ? (size_t)cap * MC * sizeof(double) : 0;
+    if (mode == 0) hipLaunchKernelGGL((k_sweep<MC, 0, KIN, WIN>), grid, blk, lds, st, src, ntask, cap);
+    else if (mode == 1) hipLaunchKernelGGL((k_sweep<MC, 1, KIN, WIN>), grid, blk, lds, st, src, ntask, cap);
+    else hipLaunchKernelGGL((k_sweep<MC, 2, KIN, WIN>), grid, blk, lds, st, src, ntask, cap);
+  }
+  static void run(const One& src, int mode, int ntask, int block, int cap, hipStream_t st) {
+    if constexpr (MC <= 16) {
+      if (cap > 0) return go<true>(src, mode, ntask, block, cap, st);
+    }
+    go<false>(src, mode, ntask, block, 0, st);
   }
 };
 
 template <int MC, int KIN>
 struct SweepMany {
-  static void run(const Many& src, int mode, int groups, int block, const int32_t* qoff, int32_t* qcnt,
-                  hipStream_t st) {
+  template <bool WIN>
+  static void go(const Many& src, int mode, int groups, int block, const int32_t* qoff, int32_t* qcnt, int cap,
+                 hipStream_t st) {
     dim3 grid(groups), blk(block);
-    if (mode == 0) hipLaunchKernelGGL((k_sweep_batch<MC, 0, KIN>), grid, blk, 0, st, src, qoff, qcnt);
-    else if (mode == 1) hipLaunchKernelGGL((k_sweep_batch<MC, 1, KIN>), grid, blk, 0, st, src, qoff, qcnt);
-    else hipLaunchKernelGGL((k_sweep_batch<MC, 2, KIN>), grid, blk, 0, st, src, qoff, qcnt);
+    const size_t lds = WIN ? (size_t)cap * MC * sizeof(double) : 0;
+    if (mode == 0) hipLaunchKernelGGL((k_sweep_batch<MC, 0, KIN, WIN>), grid, blk, lds, st, src, qoff, qcnt, cap);
+    else if (mode == 1)
+      hipLaunchKernelGGL((k_sweep_batch<MC, 1, KIN, WIN>), grid, blk, lds, st, src, qoff, qcnt, cap);
+    else hipLaunchKernelGGL((k_sweep_batch<MC, 2, KIN, WIN>), grid, blk, lds, st, src, qoff, qcnt, cap);
+  }
+  static void run(const Many& src, int mode, int groups, int block, const int32_t* qoff, int32_t* qcnt, int cap,
+                  hipStream_t st) {
+    if constexpr (MC <= 16) {
+      if (cap > 0) return go<true>(src, mode, groups, block, qoff, qcnt, cap, st);
+    }
+    go<false>(src, mode, groups, block, qoff, qcnt, 0, st);
   }
 };
 
@@ -1138,6 +1375,9 @@ int obte_bte_assemble(const obte_bte_plan_t* plan, obte_stream_t stream) {
   const int64_t total = (int64_t)plan->n_dirs * plan->n;
   hipLaunchKernelGGL((k_assemble<One>), dim3(blocks_for(total, 256), 1), dim3(256), 0, (hipStream_t)stream,
                      make_one(plan, nullptr));
+  hipLaunchKernelGGL((k_mark_keep<One>), dim3(blocks_for(total, 256), 1), dim3(256), 0, (hipStream_t)stream,
+                     make_one(plan, nullptr));
+  hipLaunchKernelGGL((k_mark_exits<One>), dim3(64, 1), dim3(256), 0, (hipStream_t)stream, make_one(plan, nullptr));
   return launch_status("obte_bte_assemble: launch failed");
 }
 
@@ -1148,6 +1388,10 @@ int obte_bte_source(const obte_bte_plan_t* plan, obte_stream_t stream) {
   hipLaunchKernelGGL((k_source<One>), dim3(blocks_for(total, 256), 1), dim3(256), 0, (hipStream_t)stream,
                      make_one(plan, nullptr));
   return launch_status("obte_bte_source: launch failed");
+}
+
+int32_t obte_sweep_window(int32_t sweep_window, int32_t lvl_pair_max, int32_t mc) {
+  return window_cap(sweep_window, lvl_pair_max, mc);
 }
 
 int obte_bte_sweep(const obte_bte_plan_t* plan, int32_t mode, const double* y, double* x, const int32_t* active,
@@ -1162,6 +1406,7 @@ int obte_bte_sweep(const obte_bte_plan_t* plan, int32_t mode, const double* y, d
   src.io.active = active;
   src.io.lane_active = lane_active;
   dispatch<SweepOne>(plan->mc, plan->kin, src, (int)mode, plan->n_dirs * plan->n_chunks, plan->block,
+                     window_cap(plan->sweep_window, plan->lvl_pair_max, plan->mc),
                      (hipStream_t)stream);
   return launch_status("obte_bte_sweep: launch failed");
 }
@@ -1223,6 +1468,10 @@ int obte_batch_assemble(const obte_bte_batch_t* b, obte_stream_t stream) {
   if (rc) return rc;
   hipLaunchKernelGGL((k_assemble<Many>), dim3(blocks_for(b->max_elems, 256), b->n_problems), dim3(256), 0,
                      (hipStream_t)stream, make_many(b, 0));
+  hipLaunchKernelGGL((k_mark_keep<Many>), dim3(blocks_for(b->max_elems, 256), b->n_problems), dim3(256), 0,
+                     (hipStream_t)stream, make_many(b, 0));
+  hipLaunchKernelGGL((k_mark_exits<Many>), dim3(64, b->n_problems), dim3(256), 0, (hipStream_t)stream,
+                     make_many(b, 0));
   return launch_status("obte_batch_assemble: launch failed");
 }
 
@@ -1241,6 +1490,7 @@ int obte_batch_sweep(const obte_bte_batch_t* b, int32_t mode, obte_stream_t stre
   hipStream_t st = (hipStream_t)stream;
   hipLaunchKernelGGL(k_zero_queues, dim3(1), dim3(64), 0, st, b->queue_cnt);
   dispatch<SweepMany>(b->mc, b->kin, make_many(b, mode), (int)mode, b->groups, b->block, b->queue_off, b->queue_cnt,
+                      window_cap(b->sweep_window, b->lvl_pair_max, b->mc),
                       st);
   return launch_status("obte_batch_sweep: launch failed");
 }

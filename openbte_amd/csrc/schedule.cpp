@@ -48,7 +48,7 @@ inline double face_dot(const double* s, const double* k, int dim) {
 }
 
 void build_direction(int32_t n, int64_t n_faces, int dim, const int32_t* i_mat, const int32_t* j_mat,
-                     const double* k_mat, const uint8_t* cut_candidate, const double* s,
+                     const double* k_mat, const uint8_t* cut_candidate, const double* s, int32_t max_width,
                      DirSchedule& out) {
   // inflow faces grouped by row cell
   std::vector<int32_t> cnt(n + 1, 0);
@@ -87,9 +87,12 @@ void build_direction(int32_t n, int64_t n_faces, int dim, const int32_t* i_mat, 
     for (int32_t e = cnt[c]; e < cnt[c + 1]; ++e)
       if (!is_cut[row_faces[e]]) indeg[c]++;
 
-  // Kahn's algorithm by levels (longest-path level of each cell)
+  // Kahn's algorithm by levels (longest-path level of each cell).  With
+  // max_width > 0 a level takes at most max_width ready cells; the rest wait
+  // for the next level ahead of the newly ready ones.  Any such layering is a
+  // valid sweep order and gives the same values.
   std::vector<int32_t> level(n, -1);
-  std::vector<int32_t> front, next;
+  std::vector<int32_t> front, next, carry, cur;
   for (int32_t c = 0; c < n; ++c)
     if (indeg[c] == 0) front.push_back(c);
   int32_t done = 0, lev = 0;
@@ -97,7 +100,7 @@ void build_direction(int32_t n, int64_t n_faces, int dim, const int32_t* i_mat, 
   out.order.reserve(n);
   out.lvl_ptr.clear();
   while (done < n) {
-    if (front.empty()) {
+    if (front.empty() && carry.empty()) {
       // cycle: make the pending cell with the fewest remaining inflows a
       // source by cutting its remaining inflow faces
       int32_t best = -1, bestdeg = 1 << 30;
@@ -117,14 +120,21 @@ void build_direction(int32_t n, int64_t n_faces, int dim, const int32_t* i_mat, 
       front.push_back(best);
     }
     std::sort(front.begin(), front.end());
+    cur.assign(carry.begin(), carry.end());
+    cur.insert(cur.end(), front.begin(), front.end());
+    size_t take = cur.size();
+    if (max_width > 0 && take > (size_t)max_width) take = (size_t)max_width;
+    carry.assign(cur.begin() + take, cur.end());
+    cur.resize(take);
+    std::sort(cur.begin(), cur.end());
     out.lvl_ptr.push_back(done);
     next.clear();
-    for (int32_t c : front) {
+    for (int32_t c : cur) {
       level[c] = lev;
       out.order.push_back(c);
     }
-    done += (int32_t)front.size();
-    for (int32_t c : front) {
+    done += (int32_t)cur.size();
+    for (int32_t c : cur) {
       for (int32_t q = dcnt[c]; q < dcnt[c + 1]; ++q) {
         int32_t f = down_face[q];
         if (f < 0) continue;
@@ -170,9 +180,9 @@ extern "C" {
 
 int obte_sched_create(int32_t n, int64_t n_faces, int32_t dim, const int32_t* i_mat, const int32_t* j_mat,
                       const double* k_mat, const uint8_t* cut_candidate, int32_t n_dirs,
-                      const double* dirs, void** handle) {
+                      const double* dirs, int32_t max_width, void** handle) {
   try {
-    if (!handle || n <= 0 || n_faces < 0 || (dim != 2 && dim != 3) || n_dirs <= 0)
+    if (!handle || n <= 0 || n_faces < 0 || (dim != 2 && dim != 3) || n_dirs <= 0 || max_width < 0)
       return obte_fail(OBTE_EINVAL, "obte_sched_create: invalid arguments");
     for (int64_t f = 0; f < n_faces; ++f)
       if (i_mat[f] < 0 || i_mat[f] >= n || j_mat[f] < 0 || j_mat[f] >= n)
@@ -183,7 +193,8 @@ int obte_sched_create(int32_t n, int64_t n_faces, int32_t dim, const int32_t* i_
     s->dirs.resize(n_dirs);
 #pragma omp parallel for schedule(dynamic)
     for (int32_t l = 0; l < n_dirs; ++l)
-      build_direction(n, n_faces, dim, i_mat, j_mat, k_mat, cut_candidate, dirs + (int64_t)l * dim, s->dirs[l]);
+      build_direction(n, n_faces, dim, i_mat, j_mat, k_mat, cut_candidate, dirs + (int64_t)l * dim, max_width,
+                      s->dirs[l]);
     *handle = s;
     return OBTE_OK;
   } catch (const std::bad_alloc&) {

@@ -188,8 +188,16 @@ def build_host_operator(geo, mat, bcs, effective_thermal_conductivity=None, heat
         heat=heat, nano=1.0 if bcs.is_nanowire else 0.0)
 
 
-def build_schedule(op: HostOperator, dirs: Optional[np.ndarray] = None, cut_candidate: Optional[np.ndarray] = None) -> Schedule:
-    """Run the native level scheduler (libopenbte_amd obte_sched_*)."""
+# Cells per level of the default schedule: with at most 192 cells a level
+# fits one pass of a 256-thread windowed sweep workgroup at mc = 8 (3 cells
+# per thread) and two adjacent levels fit a 512-position LDS window.
+DEFAULT_MAX_WIDTH = 192
+
+
+def build_schedule(op: HostOperator, dirs: Optional[np.ndarray] = None, cut_candidate: Optional[np.ndarray] = None,
+                   max_width: int = DEFAULT_MAX_WIDTH) -> Schedule:
+    """Run the native level scheduler (libopenbte_amd obte_sched_*).
+    ``max_width`` caps the cells of one level (0: longest-path levels)."""
     dirs = op.dirs if dirs is None else np.ascontiguousarray(dirs, dtype=np.float64)
     cut = op.cut_candidate if cut_candidate is None else np.ascontiguousarray(cut_candidate, dtype=np.uint8)
     L = len(dirs)
@@ -199,7 +207,7 @@ def build_schedule(op: HostOperator, dirs: Optional[np.ndarray] = None, cut_cand
     k = np.ascontiguousarray(op.k_face, np.float64)
     h = C.c_void_p()
     _lib.call('obte_sched_create', n, len(i_mat), op.dim, _lib.ptr(i_mat), _lib.ptr(j_mat), _lib.ptr(k),
-              _lib.ptr(cut), L, _lib.ptr(dirs), C.byref(h))
+              _lib.ptr(cut), L, _lib.ptr(dirs), int(max_width), C.byref(h))
     try:
         sizes = np.zeros(3 * L, np.int64)
         _lib.call('obte_sched_sizes', h, _lib.ptr(sizes))
@@ -223,6 +231,19 @@ def build_schedule(op: HostOperator, dirs: Optional[np.ndarray] = None, cut_cand
     return Schedule(order=order, pos=pos, lvl_off=lvl_off, lvl_ptr=lvl_ptr, in_ptr=in_ptr, in_off=in_off,
                     in_src=in_src, in_face=in_face, cut_off=cut_off, cut_exit=cut_exit, cut_face=cut_face,
                     n_levels=nlev.astype(np.int64))
+
+
+def level_pair_max(sched: Schedule) -> int:
+    """Largest |level k| + |level k+1| over all directions (a one-level
+    direction counts its single level)."""
+    best = 0
+    for l in range(len(sched.n_levels)):
+        lp = sched.lvl_ptr[sched.lvl_off[l]:sched.lvl_off[l] + int(sched.n_levels[l]) + 1]
+        w = np.diff(lp)
+        if len(w) == 0:
+            continue
+        best = max(best, int((w[:-1] + w[1:]).max()) if len(w) > 1 else int(w[0]))
+    return best
 
 
 def max_inflow(sched: Schedule) -> int:

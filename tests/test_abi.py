@@ -25,7 +25,7 @@ def test_error_convention_without_gpu():
     assert rc < 0
     assert b'plan' in lib.obte_last_error()
     h = ctypes.c_void_p()
-    rc = lib.obte_sched_create(0, 0, 2, None, None, None, None, 1, None, ctypes.byref(h))
+    rc = lib.obte_sched_create(0, 0, 2, None, None, None, None, 1, None, 0, ctypes.byref(h))
     assert rc == -1
 
 
@@ -40,3 +40,17 @@ def test_struct_layout_matches_header():
         body = re.sub(r'/\*.*?\*/', '', body, flags=re.S)
         fields = re.findall(r'^\s*(?:const\s+)?\w+\s*\**\s*(\w+)\s*;', body, re.M)
         assert fields == [f[0] for f in py._fields_], cname
+
+
+def test_sweep_window_rule():
+    """The LDS window is the smallest power of two (>= 64) holding two adjacent
+    levels, within 48 KB at the chunk's bin count, and only when enabled."""
+    from openbte_amd import _lib
+    lib = _lib.load()
+    assert lib.obte_sweep_window(1, 384, 8) == 512
+    assert lib.obte_sweep_window(1, 10, 8) == 64
+    assert lib.obte_sweep_window(1, 700, 4) == 1024
+    assert lib.obte_sweep_window(1, 513, 8) == 0        # 1024 x 64 B > 48 KB
+    assert lib.obte_sweep_window(1, 100, 32) == 0       # wide chunks keep the global path
+    assert lib.obte_sweep_window(0, 100, 8) == 0        # disabled
+    assert lib.obte_sweep_window(1, 0, 8) == 0          # level sizes unknown

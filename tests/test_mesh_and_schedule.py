@@ -6,7 +6,7 @@ import pytest
 from openbte_amd.geometry import pore_cell_2d, pore_slab_3d, quad_cell_2d, structured_tri_cell_2d
 from openbte_amd.material import synthetic_material_2d
 from openbte_amd.objects import BoundaryConditions
-from openbte_amd.operator import build_host_operator, build_schedule, fixed_width_inflow, max_inflow
+from openbte_amd.operator import build_host_operator, build_schedule, fixed_width_inflow, level_pair_max, max_inflow
 
 
 @pytest.mark.parametrize("geo", [pore_cell_2d(10.0, 0.25, 800, seed=1), quad_cell_2d(10.0, 6),
@@ -38,12 +38,26 @@ def test_mesh_contract(geo):
     assert w.min() > 0 and w.max() < 1
 
 
-def test_scheduler_topological_order():
+@pytest.mark.parametrize('max_width', [0, 7, 192])
+def test_scheduler_topological_order(max_width):
     geo = pore_cell_2d(10.0, 0.25, 900, seed=3)
     mat = synthetic_material_2d(2, 24, 1.0, 1e4)
     bcs = BoundaryConditions(periodic={'Periodic_x': 1.0, 'Periodic_y': 0.0}, diffuse='Boundary')
     op = build_host_operator(geo, mat, bcs)
-    sc = build_schedule(op)
+    sc = build_schedule(op, max_width=max_width)
+    free = build_schedule(op, max_width=0)
+    widths = np.diff(sc.lvl_ptr)
+    if max_width:
+        # no level wider than the cap (padding entries between directions are negative)
+        assert widths.max() <= max_width
+    # capping the width only re-layers the cells: same cut faces per direction
+    assert np.array_equal(np.diff(sc.cut_off), np.diff(free.cut_off))
+    assert np.all(sc.n_levels >= free.n_levels)
+    pair = level_pair_max(sc)
+    lp0 = np.diff(sc.lvl_ptr[sc.lvl_off[0]:sc.lvl_off[0] + sc.n_levels[0] + 1])
+    assert pair >= (lp0[:-1] + lp0[1:]).max()
+    if max_width:
+        assert pair <= 2 * max_width
     n = op.n
     kin = max(2, max_inflow(sc))
     src, face = fixed_width_inflow(sc, n, kin)
