@@ -130,9 +130,8 @@ typedef struct obte_bte_plan {
   /* state */
   const double* delta_t;   /* [n] lattice temperature of the previous iteration */
   const double* tb;        /* [n_wall] wall temperatures of the previous iteration */
-  double* t;               /* [L][n_chunks][n + ncut_max + 1][mc] solution workspace; rows
-                              n .. n + ncut_max - 1 hold the interface inflows of the running
-                              sweep, the last row is a store target for values nothing reads */
+  double* t;               /* [L][n_chunks][n + ncut_max][mc] solution workspace; rows >= n
+                              hold the interface inflows of the running sweep */
   double* part_t;          /* [L][n] per-direction lattice temperature, sweep order */
   double* part_j;          /* [L][n][dim] per-direction flux, sweep order */
   double* part_tb;         /* [L][n_chunks][n_wall] */

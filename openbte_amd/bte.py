@@ -92,7 +92,7 @@ def device_bytes(op: HostOperator, sched: Schedule, cfg: Optional[SweepConfig] =
         f8 += len(op.t_face) + max(1, len(op.c_t))          # t_face, c_t
     f8 += L * n + L * n * 8                                  # pert, rec
     f8 += 2 * n + 2 * one(n_wall) + n * dim                  # dt, tb, j
-    f8 += L * nch * (n + ncut_max + 1) * mc                  # t
+    f8 += L * nch * (n + ncut_max) * mc                      # t
     f8 += L * n * (1 + dim) + one(L * nch * n_wall)          # part_t, part_j, part_tb
     f8 += 2 * M * L + 4096 + 8                               # kappa, supp, scratch
     ny = max(1, nch * n_cut * mc)
@@ -179,7 +179,7 @@ class BTEDevice:
         b['tb'] = [torch.zeros(max(1, n_wall), dtype=f64, device=dev) for _ in range(2)]
         b['j'] = torch.zeros(n * dim, dtype=f64, device=dev)
         # zero-initialised: unused inflow slots read a finite value times 0
-        b['t'] = torch.zeros(L * nch * (n + self.ncut_max + 1) * mc, dtype=f64, device=dev)
+        b['t'] = torch.zeros(L * nch * (n + self.ncut_max) * mc, dtype=f64, device=dev)
         b['part_t'] = torch.empty(L * n, dtype=f64, device=dev)
         b['part_j'] = torch.empty(L * n * dim, dtype=f64, device=dev)
         b['part_tb'] = torch.empty(max(1, L * nch * n_wall), dtype=f64, device=dev)

@@ -272,6 +272,18 @@ __device__ __forceinline__ void load_level(LevelRegs<U, NP>& R, int base, int p1
   }
 }
 
+// num / den for den = 1 + Lambda D >= 1 (no scaling or special cases):
+// reciprocal from v_rcp_f64 and two Newton steps, then one residual
+// correction of the quotient (both sweeps use it, so their values agree bit
+// for bit)
+__device__ __forceinline__ double upwind_value(double num, double den) {
+  double r = __builtin_amdgcn_rcp(den);
+  r = fma(fma(-den, r, 1.0), r, r);
+  r = fma(fma(-den, r, 1.0), r, r);
+  const double q = num * r;
+  return fma(fma(-den, q, num), r, q);
+}
+
 // v of lane S of this lane's group of GQ lanes inside its quad
 template <int GQ, int S>
 __device__ __forceinline__ double quad_bcast(double v) {
@@ -326,24 +338,43 @@ constexpr int units_of() { return WIN ? (MC == 8 ? 3 : 2) : (MC >= 16 ? 2 : 4); 
 // ---------------------------------------------------------------------------
 extern __shared__ double2 win_lds[];
 
+// Raw buffer access (gfx9 descriptor word 3): 32-bit offsets, and lanes past
+// the buffer's size load zeros and drop their stores without a memory access.
+constexpr int kBufWord3 = 0x00020000;
+constexpr int kBufOob = 0x7FFFFFF0;  // an offset past every buffer used here
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* base, int64_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)bytes, kBufWord3);
+}
+__device__ __forceinline__ double2 buf_load2(__amdgpu_buffer_rsrc_t rs, int off) {
+  const f64x2 v = __builtin_bit_cast(f64x2, __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0));
+  return make_double2(v.x, v.y);
+}
+__device__ __forceinline__ void buf_store2(__amdgpu_buffer_rsrc_t rs, int off, double2 t) {
+  const f64x2 v = {t.x, t.y};
+  __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, v), rs, off, 0, 0);
+}
+
 template <int U, int NP, int KIN>
 struct WinRegs {
   int p[U];               // position (slots past the level: its last cell)
   double2 pc[U][NP];      // this lane's pieces of the record
-  double2 far[U][KIN];    // upwind values loaded from T (outside the window)
+  double2 far[U][KIN];    // upwind value loaded from T (0 when it is in the window)
+  int woff[U][KIN];       // window slot of the upwind value (the zero slot when not)
 };
 
 template <int U, int NP, int KIN>
 __device__ __forceinline__ void load_win(WinRegs<U, NP, KIN>& R, int base, int p1, int cps, int jq,
-                                         const double* __restrict__ rec) {
+                                         __amdgpu_buffer_rsrc_t rec_rs) {
 #pragma unroll
   for (int u = 0; u < U; ++u) {
     int p = base + u * cps;
     p = p < p1 ? p : p1 - 1;
     R.p[u] = p;
-    const double2* r = reinterpret_cast<const double2*>(rec + (int64_t)p * kRec) + jq * NP;
 #pragma unroll
-    for (int i = 0; i < NP; ++i) R.pc[u][i] = r[i];
+    for (int i = 0; i < NP; ++i) R.pc[u][i] = buf_load2(rec_rs, p * (kRec * 8) + (jq * NP + i) * 16);
   }
 }
 
@@ -389,10 +420,7 @@ __device__ void sweep_task(const S& src, int pid, int lt, SweepShared<MC>& sh, i
   if (MODE == 1 && io.lane_active != nullptr)
     act = io.lane_active[(int64_t)lt * MC + 2 * jl] != 0 || io.lane_active[(int64_t)lt * MC + 2 * jl + 1] != 0;
 
-  // rows: n positions, ncut_max interface inflows, one spare row (store target
-  // of values that no later position reads)
-  double* __restrict__ T = P.t + (int64_t)lt * ((int64_t)n + P.ncut_max + 1) * MC;
-  const int spare = n + P.ncut_max;
+  double* __restrict__ T = P.t + (int64_t)lt * ((int64_t)n + P.ncut_max) * MC;
   const int32_t* __restrict__ lvl_g = P.lvl_ptr + P.lvl_off[l];
   const int nlev = P.lvl_off[l + 1] - P.lvl_off[l] - 1;
   const int cbase = P.cut_off[l];
@@ -414,7 +442,7 @@ __device__ void sweep_task(const S& src, int pid, int lt, SweepShared<MC>& sh, i
   if constexpr (WIN) {
     // unused inflow slots read the window with a zero coefficient: no
     // uninitialised (possibly NaN) patterns in it
-    for (int i = tid; i < cap * G; i += nthr) win_lds[i] = make_double2(0.0, 0.0);
+    for (int i = tid; i <= cap * G; i += nthr) win_lds[i] = make_double2(0.0, 0.0);
   }
   stage(0);
   __syncthreads();
@@ -427,8 +455,12 @@ __device__ void sweep_task(const S& src, int pid, int lt, SweepShared<MC>& sh, i
   if constexpr (WIN) {
     using WR = WinRegs<U, NP, KIN>;
     const int cmask = cap - 1;
-    // loads of the upwind values of N (a level ending at h) that lie outside
-    // the window; near and converged lanes load row 0 (value unused)
+    const int zero_slot = cap * G;  // a zero double2 past the window, never written
+    const __amdgpu_buffer_rsrc_t t_rs = make_rsrc(T, ((int64_t)n + P.ncut_max) * MC * 8);
+    const __amdgpu_buffer_rsrc_t rec_rs = make_rsrc(rec, (int64_t)n * kRec * 8);
+    // the upwind values of N (a level ending at h): window slot, or a load
+    // from T when outside the window (lanes with nothing to load read past
+    // the buffer: no memory access, a zero value)
     auto issue_far = [&](WR& N, int h) {
       const int lo_w = h - cap;
 #pragma unroll
@@ -439,15 +471,15 @@ __device__ void sweep_task(const S& src, int pid, int lt, SweepShared<MC>& sh, i
 #pragma unroll
         for (int kk = 0; kk < KIN; ++kk) {
           const bool inwin = (unsigned)(sr[kk] - lo_w) < (unsigned)cap;
-          const int row = (inwin || (MODE == 1 && !act)) ? 0 : sr[kk];
-          N.far[u][kk] = *reinterpret_cast<const double2*>(T + (int64_t)row * MC + 2 * jl);
+          const bool dead = MODE == 1 && !act;
+          N.woff[u][kk] = (inwin && !dead) ? (sr[kk] & cmask) * G + jl : zero_slot;
+          N.far[u][kk] = buf_load2(t_rs, (inwin || dead) ? kBufOob : sr[kk] * (MC * 8) + jl * 16);
         }
       }
     };
-    // the cells of C (a level ending at h): upwind values from the window or
-    // C.far, then the store to T and to the window
-    auto finish_win = [&](const WR& C, int h) {
-      const int lo_w = h - cap;
+    // the cells of C (a level ending at h): upwind values from the window plus
+    // C.far (one of the two is zero), then the store to T and to the window
+    auto finish_win = [&](const WR& C) {
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const double dd = rec_get<GQ, NP, 0>(C.pc[u]);
@@ -455,25 +487,18 @@ __device__ void sweep_task(const S& src, int pid, int lt, SweepShared<MC>& sh, i
         const double qq = MODE == 1 ? 0.0 : rec_get<GQ, NP, 2>(C.pc[u]);
         const double cf[3] = {rec_get<GQ, NP, 3>(C.pc[u]), rec_get<GQ, NP, 4>(C.pc[u]),
                               KIN == 3 ? rec_get<GQ, NP, 5>(C.pc[u]) : 0.0};
-        const double s01 = rec_get<GQ, NP, 6>(C.pc[u]);
-        const double s2c = rec_get<GQ, NP, 7>(C.pc[u]);
-        const int sr[3] = {__double2loint(s01), __double2hiint(s01), __double2loint(s2c)};
         double a0 = 0.0, a1 = 0.0;
 #pragma unroll
         for (int kk = 0; kk < KIN; ++kk) {
-          const bool inwin = (unsigned)(sr[kk] - lo_w) < (unsigned)cap;
-          const double2 w = win_lds[(sr[kk] & cmask) * G + jl];
-          double2 v = inwin ? w : C.far[u][kk];
-          if (MODE == 1 && !act) v = make_double2(0.0, 0.0);
-          a0 += cf[kk] * v.x;
-          a1 += cf[kk] * v.y;
+          const double2 w = win_lds[C.woff[u][kk]];
+          a0 += cf[kk] * (w.x + C.far[u][kk].x);
+          a1 += cf[kk] * (w.y + C.far[u][kk].y);
         }
-        const double t0 = (bb + lam0 * (qq + a0)) / (1.0 + lam0 * dd);
-        const double t1 = (bb + lam1 * (qq + a1)) / (1.0 + lam1 * dd);
-        const double2 tv = make_double2(t0, t1);
+        const double2 tv = make_double2(upwind_value(bb + lam0 * (qq + a0), 1.0 + lam0 * dd),
+                                        upwind_value(bb + lam1 * (qq + a1), 1.0 + lam1 * dd));
         // modes 0 and 1: only values read later from t (or as exits) go to HBM
-        const int row = (MODE == 2 || __double2hiint(s2c) != 0) ? C.p[u] : spare;
-        *reinterpret_cast<double2*>(T + (int64_t)row * MC + 2 * jl) = tv;
+        const bool keep = MODE == 2 || __double2hiint(rec_get<GQ, NP, 7>(C.pc[u])) != 0;
+        buf_store2(t_rs, keep ? C.p[u] * (MC * 8) + jl * 16 : kBufOob, tv);
         win_lds[(C.p[u] & cmask) * G + jl] = tv;
       }
     };
@@ -486,20 +511,20 @@ __device__ void sweep_task(const S& src, int pid, int lt, SweepShared<MC>& sh, i
       __builtin_amdgcn_sched_barrier(0);
       issue_far(N, lvl[kk + 2]);
       __builtin_amdgcn_sched_barrier(0);
-      load_win<U, NP, KIN>(N2, lvl[kk + 2] + cslot, lvl[kk + 3], cps, jq, rec);
+      load_win<U, NP, KIN>(N2, lvl[kk + 2] + cslot, lvl[kk + 3], cps, jq, rec_rs);
       __builtin_amdgcn_sched_barrier(0);
-      finish_win(C, p1);
+      finish_win(C);
       // levels wider than one pass (not produced by the default schedule)
       for (int base = p0 + U * cps; base < p1; base += U * cps) {
-        load_win<U, NP, KIN>(C, base + cslot, p1, cps, jq, rec);
+        load_win<U, NP, KIN>(C, base + cslot, p1, cps, jq, rec_rs);
         issue_far(C, p1);
-        finish_win(C, p1);
+        finish_win(C);
       }
       __syncthreads();
     };
     WR A, B, C;
-    load_win<U, NP, KIN>(A, lvl[0] + cslot, lvl[1], cps, jq, rec);
-    load_win<U, NP, KIN>(B, lvl[1] + cslot, lvl[2], cps, jq, rec);
+    load_win<U, NP, KIN>(A, lvl[0] + cslot, lvl[1], cps, jq, rec_rs);
+    load_win<U, NP, KIN>(B, lvl[1] + cslot, lvl[2], cps, jq, rec_rs);
     __builtin_amdgcn_s_waitcnt(0);
     issue_far(A, lvl[1]);
     __builtin_amdgcn_s_waitcnt(0);
@@ -547,8 +572,8 @@ __device__ void sweep_task(const S& src, int pid, int lt, SweepShared<MC>& sh, i
         a0 += cf[kk] * g[u][kk].x;
         a1 += cf[kk] * g[u][kk].y;
       }
-      const double t0 = (bb + lam0 * (qq + a0)) / (1.0 + lam0 * dd);
-      const double t1 = (bb + lam1 * (qq + a1)) / (1.0 + lam1 * dd);
+      const double t0 = upwind_value(bb + lam0 * (qq + a0), 1.0 + lam0 * dd);
+      const double t1 = upwind_value(bb + lam1 * (qq + a1), 1.0 + lam1 * dd);
       if (C.p[u] >= 0 && (MODE != 1 || act))
         *reinterpret_cast<double2*>(T + (int64_t)C.p[u] * MC + 2 * jl) = make_double2(t0, t1);
     }
@@ -1095,7 +1120,7 @@ __global__ __launch_bounds__(256) void k_modesum(S src) {
   if (p >= P.n) return;
   const int64_t gid = (int64_t)l * P.n + p;
   const double dt = P.delta_t[P.order[gid]];
-  const int64_t rows = (int64_t)P.n + P.ncut_max + 1;
+  const int64_t rows = (int64_t)P.n + P.ncut_max;
   double s = 0.0, j0 = 0.0, j1 = 0.0, j2 = 0.0;
   constexpr int H = MC / 2 < 8 ? MC / 2 : 8;  // 16-byte loads in flight per thread
   for (int c = 0; c < nch; ++c) {
@@ -1270,7 +1295,7 @@ struct SweepOne {
   template <bool WIN>
   static void go(const One& src, int mode, int ntask, int block, int cap, hipStream_t st) {
     dim3 grid(((ntask + 7) / 8) * 8), blk(block);
-    const size_t lds = WIN ? (size_t)cap * MC * sizeof(double) : 0;
+    const size_t lds = WIN ? ((size_t)cap * MC / 2 + 1) * sizeof(double2) : 0;  // window + zero slot
     if (mode == 0) hipLaunchKernelGGL((k_sweep<MC, 0, KIN, WIN>), grid, blk, lds, st, src, ntask, cap);
     else if (mode == 1) hipLaunchKernelGGL((k_sweep<MC, 1, KIN, WIN>), grid, blk, lds, st, src, ntask, cap);
     else hipLaunchKernelGGL((k_sweep<MC, 2, KIN, WIN>), grid, blk, lds, st, src, ntask, cap);
@@ -1289,7 +1314,7 @@ struct SweepMany {
   static void go(const Many& src, int mode, int groups, int block, const int32_t* qoff, int32_t* qcnt, int cap,
                  hipStream_t st) {
     dim3 grid(groups), blk(block);
-    const size_t lds = WIN ? (size_t)cap * MC * sizeof(double) : 0;
+    const size_t lds = WIN ? ((size_t)cap * MC / 2 + 1) * sizeof(double2) : 0;  // window + zero slot
     if (mode == 0) hipLaunchKernelGGL((k_sweep_batch<MC, 0, KIN, WIN>), grid, blk, lds, st, src, qoff, qcnt, cap);
     else if (mode == 1)
       hipLaunchKernelGGL((k_sweep_batch<MC, 1, KIN, WIN>), grid, blk, lds, st, src, qoff, qcnt, cap);

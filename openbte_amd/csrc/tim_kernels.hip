@@ -40,7 +40,7 @@ __global__ __launch_bounds__(256) void k_tim_face_grad(obte_bte_plan_t P, const 
   const int i = i_face[f], j = j_face[f];
   const int k = side_of_face[f];
   const double shift = jump0[f] + (k >= 0 ? tb[k] : 0.0);
-  const int64_t rows = (int64_t)n + P.ncut_max + 1;
+  const int64_t rows = (int64_t)n + P.ncut_max;
   const double* kf = P.k_face + (int64_t)f * dim;
   double acc = 0.0;
   for (int l = lane; l < L; l += kWave) {

@@ -326,7 +326,7 @@ class TIMSolver:
         """d kappa / d t of the contact sides through the flux formula itself."""
         op, geo, d = self.op, self.geo, self.dev
         n, L, M, mc, nch = op.n, op.n_dirs, op.n_mfp, d.cfg.mc, d.n_chunks
-        rows = n + d.ncut_max + 1
+        rows = n + d.ncut_max
         T = self.t_fwd.view(L, nch, rows, mc)
         pos = torch.as_tensor(self.sched.pos.reshape(L, n).astype(np.int64), device=d.dev)
         e1 = torch.as_tensor(op.ce1.astype(np.int64), device=d.dev)
