@@ -315,6 +315,10 @@ struct SweepShared {
 // 192 cells at MC = 8 and 256 at MC = 4)
 template <int MC, bool WIN>
 constexpr int units_of() { return WIN ? (MC == 8 ? 3 : 2) : (MC >= 16 ? 2 : 4); }
+// the persistent batch kernel carries its queue state as well: two cells per
+// thread in the windowed sweep keep it within three waves per SIMD
+template <int MC, bool WIN>
+constexpr int units_batch() { return WIN ? 2 : units_of<MC, WIN>(); }
 
 // ---------------------------------------------------------------------------
 // Windowed sweep.  The values of the last `cap` positions (cap a power of two
@@ -380,10 +384,9 @@ __device__ __forceinline__ void load_win(WinRegs<U, NP, KIN>& R, int base, int p
 
 // One (problem pid, task lt = l * n_chunks + chunk) sweep by the whole block;
 // WIN: with an LDS window of cap positions (dynamic shared memory).
-template <class S, int MC, int MODE, int KIN, bool WIN>
+template <class S, int MC, int MODE, int KIN, bool WIN, int U>
 __device__ void sweep_task(const S& src, int pid, int lt, SweepShared<MC>& sh, int cap) {
   constexpr int G = MC / 2;       // lanes per cell
-  constexpr int U = units_of<MC, WIN>();
   constexpr int GQ = G < 4 ? G : 4;  // lanes of a cell inside one quad
   constexpr int NP = 4 / GQ;         // record pieces per lane
   const obte_bte_plan_t& P = src.plan(pid);
@@ -721,7 +724,7 @@ __global__ __launch_bounds__(kSweepBlock) void k_sweep(One src, int ntask, int c
   if (t < 0) return;
   const obte_sweep_io_t& io = src.io;
   if (io.active != nullptr && io.active[t] == 0) return;  // uniform over the block
-  sweep_task<One, MC, MODE, KIN, WIN>(src, 0, t, sh, cap);
+  sweep_task<One, MC, MODE, KIN, WIN, units_of<MC, WIN>()>(src, 0, t, sh, cap);
 }
 
 // batch: persistent workgroups draining 8 XCD-local task queues (tasks of one
@@ -755,7 +758,7 @@ __global__ __launch_bounds__(kSweepBlock) void k_sweep_batch(Many src, const int
     if (!src.live(pid)) continue;
     const obte_sweep_io_t& io = src.sio(pid);
     if (io.active != nullptr && io.active[lt] == 0) continue;
-    sweep_task<Many, MC, MODE, KIN, WIN>(src, pid, lt, sh, cap);
+    sweep_task<Many, MC, MODE, KIN, WIN, units_batch<MC, WIN>()>(src, pid, lt, sh, cap);
   }
 }
 
