@@ -53,7 +53,6 @@ CONFIGS = {
     'plain2k': (10.0, 0.25, 2000, 30, 48, 1.0e4),
     'plain200k': (100.0, 0.25, 200_000, 60, 96, 1.0e5),
 }
-BYTES_PER_UPDATE_2D = 28.0
 HBM_PEAK = 8.0e12
 
 
@@ -553,14 +552,17 @@ def main(argv=None):
     updates = n * M * L * args.steps
     value = updates / dt
 
-    full = [(m, a.elapsed_time(b) * 1e-3) for (m, a, b) in timer if m != 1]
-    part = [(m, a.elapsed_time(b) * 1e-3) for (m, a, b) in timer if m == 1]
-    sweep_bytes = n * M * L * BYTES_PER_UPDATE_2D
-    t_full = float(np.mean([d for _, d in full]))
+    by_mode = {m: [a.elapsed_time(b) * 1e-3 for (mm, a, b) in timer if mm == m] for m in (0, 1, 2)}
+    mean_ms = {f'mode{m}': (float(np.mean(v)) * 1e3 if v else None) for m, v in by_mode.items()}
+    total_ms = {f'mode{m}': float(np.sum(v)) * 1e3 for m, v in by_mode.items()}
+    # roofline of the final full sweep (mode 2): its compulsory HBM bytes are
+    # the per-position records (64 B per direction and cell, read once), one
+    # write of every solution value and the interface inflows
+    sweep_bytes = float(L * n * 64 + L * n * M * 8 + dev.n_cut * M * 8)
+    t_full = float(np.mean(by_mode[2]))
     achieved = sweep_bytes / t_full
     print(f'# {args.steps} steps in {dt:.3f}s: {value:.3e} updates/s; extensions per step {exts}; '
-          f'full sweep {t_full * 1e3:.3f} ms; transfer sweeps {len(part)} mean '
-          f'{(np.mean([d for _, d in part]) * 1e3 if part else 0):.3f} ms; kappa {kappa:.6f}',
+          f'sweep ms mean by mode {mean_ms}, total {total_ms}; kappa {kappa:.6f}',
           file=sys.stderr, flush=True)
 
     traffic = None
@@ -569,7 +571,7 @@ def main(argv=None):
         rec = json.load(open(tf))
         if rec.get('mc') == cfg.mc and rec.get('block') == cfg.block:
             f_, w_ = rec['fetch_kib_per_dispatch'], rec['write_kib_per_dispatch']
-            traffic = 1024.0 * (f_['mode0'] + w_['mode0'] + f_['mode2'] + w_['mode2']) / 2
+            traffic = 1024.0 * (f_['mode2'] + w_['mode2'])
 
     out = {
         'metric': 'cell*mode updates/s (BTE outer iteration, all systems to direct-solver accuracy)',
@@ -594,8 +596,9 @@ def main(argv=None):
         'roofline': {'bound': 'hbm', 'achieved': achieved / 1e9, 'peak': HBM_PEAK / 1e9, 'unit': 'GB/s',
                      'frac': achieved / HBM_PEAK, 'traffic': traffic,
                      'traffic_source': 'rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE (profiles/)' if traffic else None,
-                     'kernel': 'k_sweep (full sweeps, modes 0/2)', 'bytes_per_launch': sweep_bytes,
-                     'mean_launch_s': t_full},
+                     'kernel': 'k_sweep mode 2 (final full sweep: all solution values written)',
+                     'bytes_per_launch': sweep_bytes, 'mean_launch_s': t_full,
+                     'sweep_ms_mean_by_mode': mean_ms, 'sweep_ms_total_by_mode': total_ms},
         'cpu_baseline': cpu,
     }
     print(json.dumps(out))

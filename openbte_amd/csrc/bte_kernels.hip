@@ -476,7 +476,10 @@ __device__ void sweep_task(const S& src, int pid, int lt, SweepShared<MC>& sh, i
           const bool inwin = (unsigned)(sr[kk] - lo_w) < (unsigned)cap;
           const bool dead = MODE == 1 && !act;
           N.woff[u][kk] = (inwin && !dead) ? (sr[kk] & cmask) * G + jl : zero_slot;
-          N.far[u][kk] = buf_load2(t_rs, (inwin || dead) ? kBufOob : sr[kk] * (MC * 8) + jl * 16);
+          // only lanes with a value outside the window load (a wave with none
+          // skips the instruction)
+          N.far[u][kk] = make_double2(0.0, 0.0);
+          if (!(inwin || dead)) N.far[u][kk] = buf_load2(t_rs, sr[kk] * (MC * 8) + jl * 16);
         }
       }
     };
