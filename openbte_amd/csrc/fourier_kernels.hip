@@ -22,7 +22,7 @@ namespace {
 
 constexpr int kB = 256;
 
-enum { S_RZ = 0, S_PQ, S_ALPHA, S_BETA, S_RR, S_BB, S_TOT, S_MAX, S_MIN };
+enum { S_RZ = 0, S_PQ, S_ALPHA, S_BETA, S_RR, S_BB, S_TOT, S_MAX, S_MIN, S_ITER };
 
 __device__ __forceinline__ double block_sum(double v, double* red) {
   red[threadIdx.x] = v;
@@ -178,6 +178,136 @@ __global__ void k_cg_dir(int n, const double* z, double* p, const double* scal) 
   if (i < n) p[i] = z[i] + scal[S_BETA] * p[i];
 }
 
+// ---------------------------------------------------------------------------
+// Small systems: the whole solve (right-hand side, CG iterations with the
+// residual test every iteration, centring) in one workgroup of 1024 threads,
+// one launch.  A multi-launch iteration costs five launches (~20 us); here an
+// iteration costs a few workgroup barriers and dependent L2 round trips, which
+// grow with the rows per thread: measured ~10 us per iteration at 2k cells,
+// ~34 us at 10k, so the one-workgroup solve takes systems up to 4096 cells.
+// ---------------------------------------------------------------------------
+constexpr int kCgBlock = 1024;
+constexpr int kCgBlockMaxN = 4096;
+
+// sums of NV values over the workgroup in a fixed order (xor tree inside each
+// wave, then the wave totals in wave order); red: NV * 16 doubles
+template <int NV>
+__device__ __forceinline__ void wg_sums(double (&v)[NV], double* red) {
+#pragma unroll
+  for (int k = 0; k < NV; ++k)
+    for (int o = 32; o > 0; o >>= 1) v[k] += __shfl_xor(v[k], o, 64);
+  const int w = threadIdx.x >> 6;
+  const int nw = blockDim.x >> 6;
+  __syncthreads();  // the previous call's readers are done with red
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) red[k * 16 + w] = v[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    double s = 0.0;
+    for (int j = 0; j < nw; ++j) s += red[k * 16 + j];
+    v[k] = s;
+  }
+}
+
+__global__ __launch_bounds__(kCgBlock) void k_cg_block(obte_fourier_plan_t P, const double* b, const double* c,
+                                                        double* x, double* ws, double* scal, double rtol,
+                                                        int maxit) {
+  __shared__ double red[3 * 16];
+  const int n = P.n;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  double* bh = ws;
+  double* r = ws + n;
+  double* z = ws + 2 * (int64_t)n;
+  double* p = ws + 3 * (int64_t)n;
+  double* q = ws + 4 * (int64_t)n;
+  // bh = V (b + c); the pinned entry is minus the sum of the rest
+  double s1[1] = {0.0};
+  for (int i = tid; i < n; i += nt) {
+    const double v = i == P.pin ? 0.0 : P.vol[i] * (b[i] + (c ? c[i] : 0.0));
+    bh[i] = v;
+    s1[0] += v;
+  }
+  wg_sums<1>(s1, red);
+  if (tid == 0 && P.pin >= 0) bh[P.pin] = -s1[0];
+  __syncthreads();
+  // r = bh - A x, z = r / diag, p = z
+  double s3[3] = {0.0, 0.0, 0.0};  // r.z, r.r, bh.bh
+  for (int i = tid; i < n; i += nt) {
+    const double ri = bh[i] - apply_row(P, x, i);
+    const double zi = ri / P.diag[i];
+    r[i] = ri;
+    z[i] = zi;
+    p[i] = zi;
+    s3[0] += ri * zi;
+    s3[1] += ri * ri;
+    s3[2] += bh[i] * bh[i];
+  }
+  wg_sums<3>(s3, red);
+  double rz = s3[0], rr = s3[1];
+  const double bnorm = sqrt(s3[2]);
+  const double target = rtol * (bnorm > 0.0 ? bnorm : 1.0);
+  int it = 0;
+  while (sqrt(rr) > target && it < maxit) {
+    double pq[1] = {0.0};
+    for (int i = tid; i < n; i += nt) {
+      const double qi = apply_row(P, p, i);
+      q[i] = qi;
+      pq[0] += p[i] * qi;
+    }
+    wg_sums<1>(pq, red);
+    const double alpha = pq[0] != 0.0 ? rz / pq[0] : 0.0;
+    double s2[2] = {0.0, 0.0};
+    for (int i = tid; i < n; i += nt) {
+      x[i] += alpha * p[i];
+      const double ri = r[i] - alpha * q[i];
+      const double zi = ri / P.diag[i];
+      r[i] = ri;
+      z[i] = zi;
+      s2[0] += ri * zi;
+      s2[1] += ri * ri;
+    }
+    wg_sums<2>(s2, red);
+    const double beta = rz != 0.0 ? s2[0] / rz : 0.0;
+    rz = s2[0];
+    rr = s2[1];
+    for (int i = tid; i < n; i += nt) p[i] = z[i] + beta * p[i];
+    __syncthreads();  // p of every row before the next product
+    ++it;
+  }
+  if (P.pin >= 0) {  // (max + min) / 2 centring (openbte/fourier.py:155-156)
+    double mx = -INFINITY, mn = INFINITY;
+    for (int i = tid; i < n; i += nt) {
+      mx = fmax(mx, x[i]);
+      mn = fmin(mn, x[i]);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      mx = fmax(mx, __shfl_xor(mx, o, 64));
+      mn = fmin(mn, __shfl_xor(mn, o, 64));
+    }
+    __syncthreads();
+    if ((tid & 63) == 0) {
+      red[tid >> 6] = mx;
+      red[16 + (tid >> 6)] = mn;
+    }
+    __syncthreads();
+    mx = -INFINITY;
+    mn = INFINITY;
+    for (int j = 0; j < (nt >> 6); ++j) {
+      mx = fmax(mx, red[j]);
+      mn = fmin(mn, red[16 + j]);
+    }
+    const double shift = (mx + mn) * 0.5;
+    for (int i = tid; i < n; i += nt) x[i] -= shift;
+  }
+  if (tid == 0) {
+    scal[S_RR] = rr;
+    scal[S_BB] = s3[2];
+    scal[S_ITER] = (double)it;
+  }
+}
+
 // (max + min) / 2 centring (openbte/fourier.py:155-156)
 __global__ void k_minmax(int n, const double* t, double* part, int nb) {
   __shared__ double mx[kB], mn[kB];
@@ -331,6 +461,20 @@ int obte_fourier_solve(const obte_fourier_plan_t* plan, const double* b, const d
   double* q = ws + 4 * (int64_t)n;
   double* part = ws + 5 * (int64_t)n;  // n + 64 doubles >= 3 * nb
   if (check_every <= 0) check_every = 16;
+  double host[16];
+
+  if (n <= kCgBlockMaxN) {
+    hipLaunchKernelGGL(k_cg_block, dim3(1), dim3(kCgBlock), 0, st, P, b, c, t, ws, scal, rtol, (int)maxit);
+    rc = status("obte_fourier_solve: launch failed");
+    if (rc) return rc;
+    if (hipMemcpyAsync(host, scal, 16 * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      return obte_fail(OBTE_EHIP, "obte_fourier_solve: readback failed");
+    const double bn = sqrt(host[S_BB]);
+    if (iters_out) *iters_out = (int32_t)host[S_ITER];
+    if (relres_out) *relres_out = sqrt(host[S_RR]) / (bn > 0.0 ? bn : 1.0);
+    return OBTE_OK;
+  }
 
   hipLaunchKernelGGL(k_rhs, dim3(nb), dim3(kB), 0, st, P, b, c, bh, part);
   if (P.pin >= 0) hipLaunchKernelGGL(k_rhs_fix, dim3(1), dim3(1024), 0, st, P, bh, part, (int)nb);
@@ -338,7 +482,6 @@ int obte_fourier_solve(const obte_fourier_plan_t* plan, const double* b, const d
   hipLaunchKernelGGL(k_cg_init_fin, dim3(1), dim3(1024), 0, st, part, (int)nb, scal);
   rc = status("obte_fourier_solve: init launch failed");
   if (rc) return rc;
-  double host[8];
   if (hipMemcpyAsync(host, scal, 8 * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
       hipStreamSynchronize(st) != hipSuccess)
     return obte_fail(OBTE_EHIP, "obte_fourier_solve: readback failed");

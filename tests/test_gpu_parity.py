@@ -38,6 +38,21 @@ def test_fourier_matches_oracle(small_case):
     assert dJ < 1e-6 * np.abs(ref.variables['Flux_Fourier']['data']).max()
 
 
+def test_fourier_multi_launch_path_matches_oracle():
+    """More cells than one workgroup's solver takes: the multi-launch CG."""
+    from oracle import fourier_oracle as fo
+    from openbte_amd.fourier import Fourier
+    geo, mat, bcs, eff = _case(20.0, 0.25, 10000, 4, 8, seed=9)
+    assert geo.n_elems > 4096
+    ref = fo.fourier(geo, mat.thermal_conductivity, bcs, eff)
+    out = Fourier(geo, mat.thermal_conductivity, bcs, eff, verbose=False)
+    assert out.aux['n_iter'] == ref.aux['n_iter']
+    dT = np.abs(out.variables['Temperature_Fourier']['data'] - ref.variables['Temperature_Fourier']['data']).max()
+    print('fourier 40k iters', out.aux['n_iter'], 'dT', dT)
+    assert dT < T_ATOL
+    assert abs(out.aux['kappa_diffusive'] / ref.aux['kappa_diffusive'] - 1) < 1e-9
+
+
 @pytest.mark.parametrize("mc,block", [(4, 64), (8, 128), (16, 256), (32, 64), (64, 128)])
 def test_bte_matches_oracle_small(small_case, mc, block):
     from oracle import bte_oracle as bo, fourier_oracle as fo
