@@ -555,10 +555,15 @@ def main(argv=None):
     by_mode = {m: [a.elapsed_time(b) * 1e-3 for (mm, a, b) in timer if mm == m] for m in (0, 1, 2)}
     mean_ms = {f'mode{m}': (float(np.mean(v)) * 1e3 if v else None) for m, v in by_mode.items()}
     total_ms = {f'mode{m}': float(np.sum(v)) * 1e3 for m, v in by_mode.items()}
-    # roofline of the final full sweep (mode 2): its compulsory HBM bytes are
-    # the per-position records (64 B per direction and cell, read once), one
-    # write of every solution value and the interface inflows
-    sweep_bytes = float(L * n * 64 + L * n * M * 8 + dev.n_cut * M * 8)
+    # roofline of the final full sweep (mode 2, every solution value written):
+    # algorithmic bytes = 28 B per cell*mode update for triangles, 32 B for
+    # tetrahedra (source read 8 + solution write 8 + 8 per upwind gather, 1.5 /
+    # 2 inflow faces on average) x the n*M*L updates of one launch; its
+    # compulsory HBM bytes (records read once, one write of every value, the
+    # interface inflows) are reported beside it
+    per_update = 28.0 if geo.dim == 2 else 32.0
+    sweep_bytes = float(n * M * L) * per_update
+    compulsory = float(L * n * 64 + L * n * M * 8 + dev.n_cut * M * 8)
     t_full = float(np.mean(by_mode[2]))
     achieved = sweep_bytes / t_full
     print(f'# {args.steps} steps in {dt:.3f}s: {value:.3e} updates/s; extensions per step {exts}; '
@@ -597,7 +602,8 @@ def main(argv=None):
                      'frac': achieved / HBM_PEAK, 'traffic': traffic,
                      'traffic_source': 'rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE (profiles/)' if traffic else None,
                      'kernel': 'k_sweep mode 2 (final full sweep: all solution values written)',
-                     'bytes_per_launch': sweep_bytes, 'mean_launch_s': t_full,
+                     'bytes_per_launch': sweep_bytes, 'bytes_per_update': per_update,
+                     'compulsory_bytes_per_launch': compulsory, 'mean_launch_s': t_full,
                      'sweep_ms_mean_by_mode': mean_ms, 'sweep_ms_total_by_mode': total_ms},
         'cpu_baseline': cpu,
     }
