@@ -337,6 +337,8 @@ def run_serve(args):
     arrivals = np.cumsum(np.random.default_rng(123).exponential(1.0 / lam, N))
     reqs, span, st = serve(arrivals)
     lat = np.array([r.latency for r in reqs])
+    prep = np.array([r.t_ready - r.t_submit for r in reqs])     # host operator, schedule, Fourier guess
+    queue = np.array([r.t_admit - r.t_ready for r in reqs])     # prepared, waiting for a slot / HBM
     # the plain form under the same arrivals: a FIFO queue in front of one solver
     end, plat = 0.0, []
     for a, s_ in zip(arrivals, service):
@@ -362,6 +364,9 @@ def run_serve(args):
                    'lambda_per_s': lam,
                    'served_p50_s': float(np.percentile(lat, 50)), 'served_p99_s': float(np.percentile(lat, 99)),
                    'plain_p50_s': float(np.percentile(plat, 50)), 'plain_p99_s': float(np.percentile(plat, 99)),
+                   'served_prep_p50_s': float(np.percentile(prep, 50)),
+                   'served_prep_p99_s': float(np.percentile(prep, 99)),
+                   'served_queue_p50_s': float(np.percentile(queue, 50)),
                    'open_loop_updates_per_s': total_updates / span,
                    'peak_live': st_sat.peak_live, 'peak_bytes': st_sat.peak_bytes,
                    'max_rel_kappa_diff_served_vs_alone': kdiff},
