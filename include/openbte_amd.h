@@ -123,8 +123,10 @@ typedef struct obte_bte_plan {
                               [3..5] -min(s . k, 0) of the inflow slots (0 if unused),
                               [6] int32 pair (src0, src1), [7] int32 pair (src2, keep): keep = 1
                               when a later position reads the value from t (outside the LDS
-                              window of the smallest window a sweep may use) or it is a cut
-                              exit; modes 0 and 1 of a windowed sweep store only those;
+                              window of the smallest window a sweep may use), it is a cut
+                              exit, or its cell is a diffuse-wall or kappa-contact cell;
+                              windowed sweeps store only those (final sweeps all of them
+                              unless part_chunk is set and keep_t is 0);
                               [1] and [2] are rewritten by obte_bte_source every outer iteration */
 
   /* state */
@@ -138,6 +140,15 @@ typedef struct obte_bte_plan {
   double* kappa;           /* [M][L] */
   double* supp;            /* [M][L] */
   double* scratch;         /* >= 4096 doubles of reduction scratch */
+
+  /* final-sweep mode sums (windowed sweeps) */
+  double* part_chunk;      /* [L][n_chunks][n][dim + 1] per-chunk sums sum_m t_ml T_ml and
+                              sum_m sigma_ml T_ml of each position, written by windowed final
+                              sweeps, which then store in t only the values read later (keep
+                              flags); NULL: final sweeps store every value and
+                              obte_bte_reduce forms the sums from t */
+  int32_t keep_t;          /* 1: windowed final sweeps still store every value in t (for
+                              callers that read t afterwards) */
 } obte_bte_plan_t;
 
 /* Input / output buffers of one sweep launch for one problem. */
@@ -170,7 +181,8 @@ int obte_bte_source(const obte_bte_plan_t* plan, obte_stream_t stream);
  * mode 1: zero source, interface inflow y -> exit values x (transfer operator)
  * mode 2: as 0, plus the wall temperatures (openbte/bte.py:227) into
  *         part_tb and kappa / supp (openbte/bte.py:232-243); the solutions
- *         stay in plan->t for obte_bte_reduce.
+ *         stay in plan->t for obte_bte_reduce, or, for a windowed sweep of a
+ *         plan with part_chunk, the per-chunk mode sums go to part_chunk.
  * y, x: [n_chunks][n_cut][mc].  active: [L][n_chunks] (or NULL = all);
  * lane_active: [L][n_chunks][mc] per-system flags used by mode 1 to skip
  * converged MFP bins (or NULL). */
@@ -240,6 +252,8 @@ typedef struct obte_bte_batch {
   int32_t max_walls;     /* max n_wall */
   int32_t lvl_pair_max;  /* max of the plans' lvl_pair_max (0: unknown) */
   int32_t sweep_window;  /* as in the plan */
+  int32_t part_sums;     /* 1: every plan has part_chunk (windowed final sweeps write
+                            per-chunk mode sums) */
   const obte_bte_plan_t* plans;  /* [n_problems] */
   const obte_gcr_t* gcrs;        /* [n_problems] */
   const obte_sweep_io_t* io;     /* [3][n_problems]: sweep buffers of modes 0, 1, 2 */

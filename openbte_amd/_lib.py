@@ -34,6 +34,7 @@ class BtePlan(C.Structure):
         ('pert', _p), ('rec', _p),
         ('delta_t', _p), ('tb', _p), ('t', _p), ('part_t', _p), ('part_j', _p), ('part_tb', _p),
         ('kappa', _p), ('supp', _p), ('scratch', _p),
+        ('part_chunk', _p), ('keep_t', _i32),
     ]
 
 
@@ -60,7 +61,7 @@ class Batch(C.Structure):
     _fields_ = [
         ('n_problems', _i32), ('n_tasks', _i32), ('mc', _i32), ('block', _i32), ('kin', _i32), ('groups', _i32),
         ('max_elems', _i32), ('max_dirs', _i32), ('max_cells', _i32), ('max_walls', _i32),
-        ('lvl_pair_max', _i32), ('sweep_window', _i32),
+        ('lvl_pair_max', _i32), ('sweep_window', _i32), ('part_sums', _i32),
         ('plans', _p), ('gcrs', _p), ('io', _p), ('rio', _p), ('live', _p), ('task_pid', _p), ('task_local', _p),
         ('queue_off', _p), ('queue_cnt', _p),
     ]

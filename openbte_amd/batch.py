@@ -154,6 +154,7 @@ class BTEBatch:
         B.max_walls = max(len(d.op.i_diffuse) for d in devs)
         B.lvl_pair_max = max(d.lvl_pair_max for d in devs)
         B.sweep_window = int(bool(self.cfg.window))
+        B.part_sums = int(all(d.plan.part_chunk for d in devs))
         B.plans, B.gcrs, B.io, B.rio = (self.plans_t.data_ptr(), self.gcrs_t.data_ptr(), self.io_t.data_ptr(),
                                         self.rio_t.data_ptr())
         B.live = self.live.data_ptr()
@@ -161,6 +162,12 @@ class BTEBatch:
         B.queue_off, B.queue_cnt = self.queue_off.data_ptr(), self.queue_cnt.data_ptr()
         if assemble:
             _lib.call('obte_batch_assemble', self.b, self.stream)
+
+    def keep_solutions(self, on: bool = True) -> None:
+        """``BTEDevice.keep_solutions`` for every problem of the batch."""
+        for d in self.devs:
+            d.keep_solutions(on)
+        self.plans_t.copy_(_struct_tensor([d.plan for d in self.devs], self.dev))
 
     # -- iteration ---------------------------------------------------------
     def reset(self, delta_t0: Sequence[np.ndarray]) -> None:

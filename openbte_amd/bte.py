@@ -70,6 +70,11 @@ def _kmax(cfg: SweepConfig, ny: int) -> int:
     return int(max(8, min(cfg.kmax, cfg.basis_bytes // (16 * ny))))
 
 
+def _window_cap(cfg: SweepConfig, pair_max: int) -> int:
+    """Positions of the LDS window of sweeps with this shape (0: none)."""
+    return int(_lib.load().obte_sweep_window(int(bool(cfg.window)), int(pair_max), cfg.mc))
+
+
 def device_bytes(op: HostOperator, sched: Schedule, cfg: Optional[SweepConfig] = None) -> int:
     """Device memory (bytes) a ``BTEDevice`` of this operator allocates; the
     serving loop admits a request only when this fits its HBM budget."""
@@ -95,6 +100,8 @@ def device_bytes(op: HostOperator, sched: Schedule, cfg: Optional[SweepConfig] =
     f8 += L * nch * (n + ncut_max) * mc                      # t
     f8 += L * n * (1 + dim) + one(L * nch * n_wall)          # part_t, part_j, part_tb
     f8 += 2 * M * L + 4096 + 8                               # kappa, supp, scratch
+    if _window_cap(cfg, level_pair_max(sched)) > 0:
+        f8 += L * nch * n * (1 + dim)                        # part_chunk
     ny = max(1, nch * n_cut * mc)
     f8 += 4 * ny + 2 * _kmax(cfg, ny) * ny + M * L           # y r x w, u c, rnorm
     i4 += 2 * L * nch + L * nch * mc                         # k, active, lane_active
@@ -186,6 +193,9 @@ class BTEDevice:
         b['kappa'] = torch.zeros(M * L, dtype=f64, device=dev)
         b['supp'] = torch.zeros(M * L, dtype=f64, device=dev)
         b['scratch'] = torch.zeros(4096 + 8, dtype=f64, device=dev)
+        if _window_cap(cfg, self.lvl_pair_max) > 0:
+            # per-chunk mode sums of the windowed final sweep
+            b['part_chunk'] = torch.empty(L * nch * n * (1 + dim), dtype=f64, device=dev)
         # interface Krylov state
         ny = max(1, nch * self.n_cut * mc)
         cfg.kmax = _kmax(cfg, ny)
@@ -217,7 +227,13 @@ class BTEDevice:
 
     def window_positions(self) -> int:
         """Positions of the LDS window the sweeps of this device use (0: none)."""
-        return int(_lib.load().obte_sweep_window(int(bool(self.cfg.window)), self.lvl_pair_max, self.cfg.mc))
+        return _window_cap(self.cfg, self.lvl_pair_max)
+
+    def keep_solutions(self, on: bool = True) -> None:
+        """Let final sweeps store every solution value in ``buf['t']`` (for
+        callers that read it); by default a windowed final sweep stores only
+        the values later positions read and forms the mode sums on chip."""
+        self.plan.keep_t = int(bool(on))
 
     # -- C structs -------------------------------------------------------
     def _make_plan(self) -> _lib.BtePlan:
@@ -240,6 +256,8 @@ class BTEDevice:
             P.t_face, P.c_t = b['t_face'].data_ptr(), b['c_t'].data_ptr()
         P.delta_t = b['dt'][self.cur].data_ptr()
         P.tb = b['tb'][self.cur].data_ptr()
+        P.part_chunk = b['part_chunk'].data_ptr() if 'part_chunk' in b else 0
+        P.keep_t = 0
         return P
 
     def _make_gcr(self) -> _lib.Gcr:

@@ -202,6 +202,27 @@ __global__ void k_mark_exits(S src) {
   }
 }
 
+// keep flag of the diffuse-wall and kappa-contact cells in every direction
+// (the final sweep's wall-temperature and kappa sums read their values from
+// t); grid-stride over (direction, entry), blockIdx.y = problem
+template <class S>
+__global__ void k_mark_cells(S src) {
+  const int pid = blockIdx.y;
+  const obte_bte_plan_t& P = src.plan(pid);
+  if (min_window(P.lvl_pair_max) == 0) return;
+  const int nc = P.contact_kind == 0 ? 0 : P.n_contact;
+  const int per = P.n_wall + (P.contact_kind == 1 ? 2 * nc : nc);
+  const int64_t total = (int64_t)per * P.n_dirs;
+  int32_t* words = reinterpret_cast<int32_t*>(P.rec);
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+    const int l = (int)(i / per);
+    const int k = (int)(i - (int64_t)l * per);
+    const int c = k < P.n_wall ? P.wall_cell[k] : (k - P.n_wall < nc ? P.ce1[k - P.n_wall] : P.ce2[k - P.n_wall - nc]);
+    const int p = P.pos[(int64_t)l * P.n + c];
+    atomicOr(&words[(((int64_t)l * P.n + p) * kRec + 7) * 2 + 1], 1);
+  }
+}
+
 // q = pert + sum over diffuse walls of (-TB_s) min(s . db, 0)   (openbte/bte.py:199-200,214)
 //     + sum over interface reflecting sides of (1 - t) (-TB_k) min(s . db, 0)
 // b0 = DeltaT + heat, both in position order
@@ -304,11 +325,37 @@ __device__ __forceinline__ double rec_get(const double2 (&pc)[NP]) {
 constexpr int kLvlStage = 2048;  // level pointers staged in LDS per refill
 
 template <int MC>
-struct SweepShared {
+struct alignas(16) SweepShared {
   double red[kSweepBlock];
-  double sig[3 * MC];
+  double sig[3 * MC];      // sigma of the chunk's bins, [bin][3] (final sweeps)
+  double wk[4][MC];        // mode-sum weights t, sigma_x, sigma_y, sigma_z per bin (final sweeps)
   int lvl[kLvlStage + 2];  // staged level starts of the running direction
 };
+
+// Per-chunk mode sums of one cell from its row of MC values (G = MC / 2
+// double2 pairs): e[0] = sum over the chunk's bins of t_m T_m, e[1 + d] =
+// sum of sigma_m[d] T_m, one FMA chain per component in bin order.  wk(k, j)
+// gives the weights of component k for the bin pair j.  The windowed final
+// sweep (row in its LDS window) and k_modesum (row in t) both use it, so
+// their sums have the same bits.
+// kSerial: keep each bin pair's loads next to their use (the sweep's register
+// budget), instead of letting the scheduler hoist them all.
+template <int G, bool kSerial, class Row, class W>
+__device__ __forceinline__ void chunk_sums(const Row& row, const W& wk, int dim, double (&e)[4]) {
+#pragma unroll
+  for (int j = 0; j < G; ++j) {
+    if constexpr (kSerial) __builtin_amdgcn_sched_barrier(0);
+    const double2 v = row(j);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (k <= dim) {
+        const double2 w = wk(k, j);
+        e[k] = j == 0 ? w.x * v.x : fma(w.x, v.x, e[k]);
+        e[k] = fma(w.y, v.y, e[k]);
+      }
+    }
+  }
+}
 
 // cells per thread and level pass; the windowed sweep also holds the far
 // upwind values of two levels, so it takes fewer (one 256-thread pass covers
@@ -347,6 +394,7 @@ extern __shared__ double2 win_lds[];
 constexpr int kBufWord3 = 0x00020000;
 constexpr int kBufOob = 0x7FFFFFF0;  // an offset past every buffer used here
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 typedef double f64x2 __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* base, int64_t bytes) {
@@ -384,8 +432,13 @@ __device__ __forceinline__ void load_win(WinRegs<U, NP, KIN>& R, int base, int p
 
 // One (problem pid, task lt = l * n_chunks + chunk) sweep by the whole block;
 // WIN: with an LDS window of cap positions (dynamic shared memory).
+// MODE: 0 source sweep, 1 transfer sweep, 2 final sweep, 3 final sweep that
+// also forms the per-chunk mode sums from its LDS window (windowed only).
 template <class S, int MC, int MODE, int KIN, bool WIN, int U>
 __device__ void sweep_task(const S& src, int pid, int lt, SweepShared<MC>& sh, int cap) {
+  constexpr bool kFinal = MODE >= 2;
+  constexpr bool kFuse = MODE == 3;
+  static_assert(WIN || !kFuse, "the fused mode sums read the LDS window");
   constexpr int G = MC / 2;       // lanes per cell
   constexpr int GQ = G < 4 ? G : 4;  // lanes of a cell inside one quad
   constexpr int NP = 4 / GQ;         // record pieces per lane
@@ -407,8 +460,9 @@ __device__ void sweep_task(const S& src, int pid, int lt, SweepShared<MC>& sh, i
   const int m0 = chunk * MC + 2 * jl;
   const bool v0 = m0 < P.n_mfp, v1 = m0 + 1 < P.n_mfp;
   const double lam0 = v0 ? P.mfps[m0] : 0.0, lam1 = v1 ? P.mfps[m0 + 1] : 0.0;
-  if (MODE == 2 && cslot == 0) {
-    // group velocities of the chunk's bins for the wall and contact sums
+  if (kFinal && cslot == 0) {
+    // group velocities and weights of the chunk's bins for the wall, contact
+    // and mode sums
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const int m = m0 + h;
@@ -417,6 +471,9 @@ __device__ void sweep_task(const S& src, int pid, int lt, SweepShared<MC>& sh, i
       sh.sig[3 * (2 * jl + h) + 0] = ok ? sg[0] : 0.0;
       sh.sig[3 * (2 * jl + h) + 1] = ok ? sg[1] : 0.0;
       sh.sig[3 * (2 * jl + h) + 2] = ok && dim == 3 ? sg[2] : 0.0;
+      sh.wk[0][2 * jl + h] = ok ? P.t_coeff[(int64_t)m * L + l] : 0.0;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) sh.wk[1 + d][2 * jl + h] = sh.sig[3 * (2 * jl + h) + d];
     }
   }
   bool act = true;  // transfer sweeps skip converged bin pairs
@@ -461,6 +518,11 @@ __device__ void sweep_task(const S& src, int pid, int lt, SweepShared<MC>& sh, i
     const int zero_slot = cap * G;  // a zero double2 past the window, never written
     const __amdgpu_buffer_rsrc_t t_rs = make_rsrc(T, ((int64_t)n + P.ncut_max) * MC * 8);
     const __amdgpu_buffer_rsrc_t rec_rs = make_rsrc(rec, (int64_t)n * kRec * 8);
+    // final sweeps store every value unless they form the mode sums here
+    // (then only values read later, unless the caller asked for all of them)
+    const bool store_all = MODE == 2 || (kFuse && P.keep_t != 0);
+    const __amdgpu_buffer_rsrc_t ps_rs =
+        make_rsrc(kFuse ? P.part_chunk + (int64_t)lt * n * (dim + 1) : T, kFuse ? (int64_t)n * (dim + 1) * 8 : 0);
     // the upwind values of N (a level ending at h): window slot, or a load
     // from T when outside the window (lanes with nothing to load read past
     // the buffer: no memory access, a zero value)
@@ -502,18 +564,41 @@ __device__ void sweep_task(const S& src, int pid, int lt, SweepShared<MC>& sh, i
         }
         const double2 tv = make_double2(upwind_value(bb + lam0 * (qq + a0), 1.0 + lam0 * dd),
                                         upwind_value(bb + lam1 * (qq + a1), 1.0 + lam1 * dd));
-        // modes 0 and 1: only values read later from t (or as exits) go to HBM
-        const bool keep = MODE == 2 || __double2hiint(rec_get<GQ, NP, 7>(C.pc[u])) != 0;
+        // only values read later from t (exits, wall and contact cells included)
+        // go to HBM, unless the final sweep stores them all
+        const bool keep = store_all || __double2hiint(rec_get<GQ, NP, 7>(C.pc[u])) != 0;
         buf_store2(t_rs, keep ? C.p[u] * (MC * 8) + jl * 16 : kBufOob, tv);
         win_lds[(C.p[u] & cmask) * G + jl] = tv;
       }
     };
+    // mode sums of a finished level (positions q0 .. q1), read back from the
+    // window while the next level is computed: the level's slots are not
+    // overwritten before the level after next (cap holds two adjacent levels).
+    // One thread per cell reads its row and stores its dim + 1 sums (the
+    // level's sums are one contiguous range: consecutive positions).
+    auto reduce_level = [&](int q0, int q1) {
+      for (int p = q0 + tid; p < q1; p += nthr) {
+        const double2* row = win_lds + (p & cmask) * G;
+        double e[4] = {0.0, 0.0, 0.0, 0.0};
+        chunk_sums<G, true>([&](int j) { return row[j]; },
+                            [&](int k, int j) { return *reinterpret_cast<const double2*>(&sh.wk[k][2 * j]); }, dim,
+                            e);
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          if (k <= dim)
+            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, e[k]), ps_rs, (p * (dim + 1) + k) * 8, 0,
+                                                  0);
+      }
+    };
+    int rq0 = 0, rq1 = 0;  // the previous level (its mode sums are formed during this one)
     // level kk (cells in C); issues the far loads of level kk + 1 (N) and
     // the records of level kk + 2 (into N2)
     auto body = [&](int kk, WR& C, WR& N, WR& N2) {
       const int p0 = lvl[kk];
       const int p1 = lvl[kk + 1];
-      __builtin_amdgcn_s_waitcnt(vmcnt_only(U));
+      // the U stores of the previous level (and its mode-sum stores) stay in
+      // flight
+      __builtin_amdgcn_s_waitcnt(vmcnt_only(kFuse ? U + 4 : U));
       __builtin_amdgcn_sched_barrier(0);
       issue_far(N, lvl[kk + 2]);
       __builtin_amdgcn_sched_barrier(0);
@@ -525,6 +610,11 @@ __device__ void sweep_task(const S& src, int pid, int lt, SweepShared<MC>& sh, i
         load_win<U, NP, KIN>(C, base + cslot, p1, cps, jq, rec_rs);
         issue_far(C, p1);
         finish_win(C);
+      }
+      if constexpr (kFuse) {
+        reduce_level(rq0, rq1);
+        rq0 = p0;
+        rq1 = p1;
       }
       __syncthreads();
     };
@@ -546,6 +636,7 @@ __device__ void sweep_task(const S& src, int pid, int lt, SweepShared<MC>& sh, i
         body(kk + 2, C, A, B);
       }
     }
+    if constexpr (kFuse) reduce_level(rq0, rq1);  // the last level
   } else {
 
   using Regs = LevelRegs<U, NP>;
@@ -642,7 +733,7 @@ __device__ void sweep_task(const S& src, int pid, int lt, SweepShared<MC>& sh, i
     }
   }
 
-  if (MODE == 2) {
+  if (kFinal) {
     const int32_t* __restrict__ pos = P.pos + lo;
     // wall temperatures: TB_s = -sum_ml T_ml[i_s] GG_ml,s  (openbte/bte.py:73-78,227)
     for (int sd = tid; sd < P.n_wall; sd += nthr) {
@@ -1097,66 +1188,102 @@ __global__ void k_gcr_reset(S src, int use_dt) {
 }
 
 // Stage 1 (openbte/bte.py:222-230): mode sums of each direction in sweep
-// order, straight from the solution workspace of the last full sweep:
-//   part_t[l][p] = sum_m t_ml T_ml,   part_j[l][p] = sum_m sigma_ml (T_ml - DeltaT_old)
-// (chunks then bins, a fixed order), in [L][n].  Grid: x = 256 positions,
-// y = direction, z = problem; the direction's weights are staged in LDS and
-// each thread reads its rows as 16-byte vectors (neighbouring threads read
-// neighbouring rows).
+// order,
+//   part_t[l][p] = sum_m t_ml T_ml,
+//   part_j[l][p] = sum_m sigma_ml T_ml - DeltaT_old sum_m sigma_ml,
+// summed chunk by chunk in order, each chunk's sum formed by chunk_partial.
+// k_modesum reads T from the solution workspace; k_modesum_part sums the
+// per-chunk sums a windowed final sweep wrote to part_chunk.  Both give the
+// same bits.  Grid: y = direction, z = problem.
 constexpr int kModesumMaxM = 1024;
 
+// sum over the bins of sigma_ml (fixed order), by one thread
+__device__ __forceinline__ void sigma_total(const obte_bte_plan_t& P, int l, double* s) {
+  const int L = P.n_dirs, dim = P.dim;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+  for (int m = 0; m < P.n_mfp; ++m) {
+    const double* sg = P.sigma + ((int64_t)m * L + l) * dim;
+    a0 += sg[0];
+    a1 += sg[1];
+    if (dim == 3) a2 += sg[2];
+  }
+  s[0] = a0;
+  s[1] = a1;
+  s[2] = a2;
+}
+
+__device__ __forceinline__ void modesum_store(const obte_bte_plan_t& P, int l, int p, const double (&acc)[4],
+                                              const double* stot) {
+  const int64_t gid = (int64_t)l * P.n + p;
+  const double dt = P.delta_t[P.order[gid]];
+  P.part_t[gid] = acc[0];
+  for (int d = 0; d < P.dim; ++d) P.part_j[gid * P.dim + d] = fma(-dt, stot[d], acc[1 + d]);
+}
+
+// From t: one thread per position, the bin pairs' weights staged in LDS.
 template <class S, int MC>
 __global__ __launch_bounds__(256) void k_modesum(S src) {
-  __shared__ double w[4 * kModesumMaxM];  // t, sigma_x, sigma_y, sigma_z per bin
+  constexpr int G = MC / 2;
+  __shared__ __align__(16) double w[8 * (kModesumMaxM / 2)];  // per bin pair: t0 t1 s0x s1x s0y s1y s0z s1z
+  __shared__ double stot[4];
   const int pid = blockIdx.z;
   if (!src.live(pid)) return;
   const obte_bte_plan_t& P = src.plan(pid);
   const int l = blockIdx.y;
-  const int L = P.n_dirs, M = P.n_mfp, dim = P.dim, nch = P.n_chunks;
-  if (l >= L || (int64_t)blockIdx.x * blockDim.x >= P.n) return;  // uniform over the block
-  for (int m = threadIdx.x; m < M; m += blockDim.x) {
-    const double* sg = P.sigma + ((int64_t)m * L + l) * dim;
-    w[4 * m + 0] = P.t_coeff[(int64_t)m * L + l];
-    w[4 * m + 1] = sg[0];
-    w[4 * m + 2] = sg[1];
-    w[4 * m + 3] = dim == 3 ? sg[2] : 0.0;
-  }
-  __syncthreads();
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= P.n) return;
-  const int64_t gid = (int64_t)l * P.n + p;
-  const double dt = P.delta_t[P.order[gid]];
-  const int64_t rows = (int64_t)P.n + P.ncut_max;
-  double s = 0.0, j0 = 0.0, j1 = 0.0, j2 = 0.0;
-  constexpr int H = MC / 2 < 8 ? MC / 2 : 8;  // 16-byte loads in flight per thread
-  for (int c = 0; c < nch; ++c) {
-    const double2* row = reinterpret_cast<const double2*>(P.t + (((int64_t)l * nch + c) * rows + p) * MC);
-    for (int h0 = 0; h0 < MC / 2; h0 += H) {
-      double2 v[H];
+  const int L = P.n_dirs, M = P.n_mfp, dim = P.dim, nch = P.n_chunks, n = P.n;
+  if (l >= L || (int64_t)blockIdx.x * blockDim.x >= n) return;  // uniform over the block
+  for (int q = threadIdx.x; q < nch * G; q += blockDim.x) {
 #pragma unroll
-      for (int h = 0; h < H; ++h) v[h] = row[h0 + h];
-#pragma unroll
-      for (int h = 0; h < H; ++h) {
-#pragma unroll
-        for (int e = 0; e < 2; ++e) {
-          const int m = c * MC + 2 * (h0 + h) + e;
-          if (m < M) {
-            const double x = e ? v[h].y : v[h].x;
-            const double d = x - dt;
-            const double* wm = w + 4 * m;
-            s += wm[0] * x;
-            j0 += wm[1] * d;
-            j1 += wm[2] * d;
-            j2 += wm[3] * d;
-          }
-        }
-      }
+    for (int h = 0; h < 2; ++h) {
+      const int m = 2 * q + h;
+      const bool ok = m < M;
+      const double* sg = P.sigma + ((int64_t)(ok ? m : 0) * L + l) * dim;
+      w[8 * q + h] = ok ? P.t_coeff[(int64_t)m * L + l] : 0.0;
+      w[8 * q + 2 + h] = ok ? sg[0] : 0.0;
+      w[8 * q + 4 + h] = ok ? sg[1] : 0.0;
+      w[8 * q + 6 + h] = ok && dim == 3 ? sg[2] : 0.0;
     }
   }
-  P.part_t[gid] = s;
-  P.part_j[gid * dim] = j0;
-  P.part_j[gid * dim + 1] = j1;
-  if (dim == 3) P.part_j[gid * dim + 2] = j2;
+  if (threadIdx.x == 0) sigma_total(P, l, stot);
+  __syncthreads();
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int64_t rows = (int64_t)n + P.ncut_max;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int c = 0; c < nch; ++c) {
+    const double2* row = reinterpret_cast<const double2*>(P.t + (((int64_t)l * nch + c) * rows + p) * MC);
+    const double2* wc = reinterpret_cast<const double2*>(w + 8 * c * G);
+    double e[4] = {0.0, 0.0, 0.0, 0.0};
+    chunk_sums<G, false>([&](int j) { return row[j]; }, [&](int k, int j) { return wc[4 * j + k]; }, dim, e);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) acc[k] += e[k];
+  }
+  modesum_store(P, l, p, acc, stot);
+}
+
+// From the per-chunk sums of a windowed final sweep: one thread per position.
+template <class S>
+__global__ __launch_bounds__(256) void k_modesum_part(S src) {
+  __shared__ double stot[4];
+  const int pid = blockIdx.z;
+  if (!src.live(pid)) return;
+  const obte_bte_plan_t& P = src.plan(pid);
+  const int l = blockIdx.y;
+  const int n = P.n, dim = P.dim, nch = P.n_chunks;
+  if (l >= P.n_dirs || (int64_t)blockIdx.x * blockDim.x >= n) return;  // uniform over the block
+  if (threadIdx.x == 0) sigma_total(P, l, stot);
+  __syncthreads();
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int c = 0; c < nch; ++c) {
+    const double* e = P.part_chunk + (((int64_t)l * nch + c) * n + p) * (dim + 1);
+    acc[0] += e[0];
+    acc[1] += e[1];
+    acc[2] += e[2];
+    if (dim == 3) acc[3] += e[3];
+  }
+  modesum_store(P, l, p, acc, stot);
 }
 
 // Stage 2: fixed-order sum over directions, gathering each direction's
@@ -1304,13 +1431,19 @@ struct SweepOne {
     const size_t lds = WIN ? ((size_t)cap * MC / 2 + 1) * sizeof(double2) : 0;  // window + zero slot
     if (mode == 0) hipLaunchKernelGGL((k_sweep<MC, 0, KIN, WIN>), grid, blk, lds, st, src, ntask, cap);
     else if (mode == 1) hipLaunchKernelGGL((k_sweep<MC, 1, KIN, WIN>), grid, blk, lds, st, src, ntask, cap);
-    else hipLaunchKernelGGL((k_sweep<MC, 2, KIN, WIN>), grid, blk, lds, st, src, ntask, cap);
+    else if constexpr (WIN) {
+      if (mode == 3) hipLaunchKernelGGL((k_sweep<MC, 3, KIN, true>), grid, blk, lds, st, src, ntask, cap);
+      else hipLaunchKernelGGL((k_sweep<MC, 2, KIN, true>), grid, blk, lds, st, src, ntask, cap);
+    } else {
+      hipLaunchKernelGGL((k_sweep<MC, 2, KIN, false>), grid, blk, lds, st, src, ntask, cap);
+    }
   }
+  // mode 3 = mode 2 forming the per-chunk mode sums (windowed sweeps only)
   static void run(const One& src, int mode, int ntask, int block, int cap, hipStream_t st) {
     if constexpr (MC <= 16) {
       if (cap > 0) return go<true>(src, mode, ntask, block, cap, st);
     }
-    go<false>(src, mode, ntask, block, 0, st);
+    go<false>(src, mode == 3 ? 2 : mode, ntask, block, 0, st);
   }
 };
 
@@ -1324,22 +1457,33 @@ struct SweepMany {
     if (mode == 0) hipLaunchKernelGGL((k_sweep_batch<MC, 0, KIN, WIN>), grid, blk, lds, st, src, qoff, qcnt, cap);
     else if (mode == 1)
       hipLaunchKernelGGL((k_sweep_batch<MC, 1, KIN, WIN>), grid, blk, lds, st, src, qoff, qcnt, cap);
-    else hipLaunchKernelGGL((k_sweep_batch<MC, 2, KIN, WIN>), grid, blk, lds, st, src, qoff, qcnt, cap);
+    else if constexpr (WIN) {
+      if (mode == 3) hipLaunchKernelGGL((k_sweep_batch<MC, 3, KIN, true>), grid, blk, lds, st, src, qoff, qcnt, cap);
+      else hipLaunchKernelGGL((k_sweep_batch<MC, 2, KIN, true>), grid, blk, lds, st, src, qoff, qcnt, cap);
+    } else {
+      hipLaunchKernelGGL((k_sweep_batch<MC, 2, KIN, false>), grid, blk, lds, st, src, qoff, qcnt, cap);
+    }
   }
   static void run(const Many& src, int mode, int groups, int block, const int32_t* qoff, int32_t* qcnt, int cap,
                   hipStream_t st) {
     if constexpr (MC <= 16) {
       if (cap > 0) return go<true>(src, mode, groups, block, qoff, qcnt, cap, st);
     }
-    go<false>(src, mode, groups, block, qoff, qcnt, 0, st);
+    go<false>(src, mode == 3 ? 2 : mode, groups, block, qoff, qcnt, 0, st);
   }
 };
 
+// parts: sum the per-chunk sums of a windowed final sweep; else read t
 template <int MC, int KIN>
 struct ModesumL {
   template <class S>
-  static void run(const S& src, dim3 grid, hipStream_t st) {
-    hipLaunchKernelGGL((k_modesum<S, MC>), grid, dim3(256), 0, st, src);
+  static void run(const S& src, bool parts, int max_cells, int dirs, int problems, hipStream_t st) {
+    if (parts)
+      hipLaunchKernelGGL((k_modesum_part<S>), dim3(blocks_for(max_cells, 256), dirs, problems), dim3(256), 0, st,
+                         src);
+    else
+      hipLaunchKernelGGL((k_modesum<S, MC>), dim3(blocks_for(max_cells, 256), dirs, problems), dim3(256), 0, st,
+                         src);
   }
 };
 
@@ -1409,6 +1553,7 @@ int obte_bte_assemble(const obte_bte_plan_t* plan, obte_stream_t stream) {
   hipLaunchKernelGGL((k_mark_keep<One>), dim3(blocks_for(total, 256), 1), dim3(256), 0, (hipStream_t)stream,
                      make_one(plan, nullptr));
   hipLaunchKernelGGL((k_mark_exits<One>), dim3(64, 1), dim3(256), 0, (hipStream_t)stream, make_one(plan, nullptr));
+  hipLaunchKernelGGL((k_mark_cells<One>), dim3(64, 1), dim3(256), 0, (hipStream_t)stream, make_one(plan, nullptr));
   return launch_status("obte_bte_assemble: launch failed");
 }
 
@@ -1436,8 +1581,9 @@ int obte_bte_sweep(const obte_bte_plan_t* plan, int32_t mode, const double* y, d
   src.io.x = x;
   src.io.active = active;
   src.io.lane_active = lane_active;
-  dispatch<SweepOne>(plan->mc, plan->kin, src, (int)mode, plan->n_dirs * plan->n_chunks, plan->block,
-                     window_cap(plan->sweep_window, plan->lvl_pair_max, plan->mc),
+  const int cap = window_cap(plan->sweep_window, plan->lvl_pair_max, plan->mc);
+  const int m = mode == 2 && cap > 0 && plan->part_chunk ? 3 : (int)mode;
+  dispatch<SweepOne>(plan->mc, plan->kin, src, m, plan->n_dirs * plan->n_chunks, plan->block, cap,
                      (hipStream_t)stream);
   return launch_status("obte_bte_sweep: launch failed");
 }
@@ -1484,7 +1630,8 @@ int obte_bte_reduce(const obte_bte_plan_t* plan, double* delta_t_new, double* j_
   src.rio.j_new = j_new;
   src.rio.tb_new = tb_new;
   src.rio.norm2 = norm2;
-  dispatch<ModesumL>(plan->mc, 2, src, dim3(blocks_for(plan->n, 256), plan->n_dirs, 1), st);
+  const bool parts = window_cap(plan->sweep_window, plan->lvl_pair_max, plan->mc) > 0 && plan->part_chunk;
+  dispatch<ModesumL>(plan->mc, 2, src, parts, plan->n, plan->n_dirs, 1, st);
   hipLaunchKernelGGL((k_reduce_cells<One>), dim3(blocks_for(plan->n, 256), 1), dim3(256), 0, st, src);
   if (plan->n_wall > 0)
     hipLaunchKernelGGL((k_reduce_walls<One>), dim3(blocks_for(plan->n_wall, 256), 1), dim3(256), 0, st, src);
@@ -1503,6 +1650,8 @@ int obte_batch_assemble(const obte_bte_batch_t* b, obte_stream_t stream) {
                      (hipStream_t)stream, make_many(b, 0));
   hipLaunchKernelGGL((k_mark_exits<Many>), dim3(64, b->n_problems), dim3(256), 0, (hipStream_t)stream,
                      make_many(b, 0));
+  hipLaunchKernelGGL((k_mark_cells<Many>), dim3(64, b->n_problems), dim3(256), 0, (hipStream_t)stream,
+                     make_many(b, 0));
   return launch_status("obte_batch_assemble: launch failed");
 }
 
@@ -1520,9 +1669,9 @@ int obte_batch_sweep(const obte_bte_batch_t* b, int32_t mode, obte_stream_t stre
   if (mode < 0 || mode > 2) return obte_fail(OBTE_EINVAL, "obte_batch_sweep: mode must be 0, 1 or 2");
   hipStream_t st = (hipStream_t)stream;
   hipLaunchKernelGGL(k_zero_queues, dim3(1), dim3(64), 0, st, b->queue_cnt);
-  dispatch<SweepMany>(b->mc, b->kin, make_many(b, mode), (int)mode, b->groups, b->block, b->queue_off, b->queue_cnt,
-                      window_cap(b->sweep_window, b->lvl_pair_max, b->mc),
-                      st);
+  const int cap = window_cap(b->sweep_window, b->lvl_pair_max, b->mc);
+  const int m = mode == 2 && cap > 0 && b->part_sums ? 3 : (int)mode;
+  dispatch<SweepMany>(b->mc, b->kin, make_many(b, mode), m, b->groups, b->block, b->queue_off, b->queue_cnt, cap, st);
   return launch_status("obte_batch_sweep: launch failed");
 }
 
@@ -1558,7 +1707,8 @@ int obte_batch_reduce(const obte_bte_batch_t* b, obte_stream_t stream) {
   if (rc) return rc;
   hipStream_t st = (hipStream_t)stream;
   Many src = make_many(b, 0);
-  dispatch<ModesumL>(b->mc, 2, src, dim3(blocks_for(b->max_cells, 256), b->max_dirs, b->n_problems), st);
+  const bool parts = window_cap(b->sweep_window, b->lvl_pair_max, b->mc) > 0 && b->part_sums;
+  dispatch<ModesumL>(b->mc, 2, src, parts, b->max_cells, b->max_dirs, b->n_problems, st);
   hipLaunchKernelGGL((k_reduce_cells<Many>), dim3(blocks_for(b->max_cells, 256), b->n_problems), dim3(256), 0, st,
                      src);
   if (b->max_walls > 0)

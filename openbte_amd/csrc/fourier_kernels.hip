@@ -9,8 +9,10 @@
 // reference centres T by (max + min) / 2 right after the solve, the pin's
 // constant drops out.
 //
-// Reductions are deterministic: per-block partial sums in a fixed tree,
-// then one block sums the partials in order.
+// Reductions are deterministic: per-block partial sums in a fixed tree, then
+// the partials summed in order.  A CG iteration is two launches: the kernel
+// after a dot product's partials sums them itself (every block, same order,
+// same bits), so no launch is spent on a one-block reduction.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -22,7 +24,7 @@ namespace {
 
 constexpr int kB = 256;
 
-enum { S_RZ = 0, S_PQ, S_ALPHA, S_BETA, S_RR, S_BB, S_TOT, S_MAX, S_MIN, S_ITER };
+enum { S_RZ0 = 0, S_RZ1, S_ALPHA, S_BETA, S_RR, S_BB, S_TOT, S_MAX, S_MIN, S_ITER };
 
 __device__ __forceinline__ double block_sum(double v, double* red) {
   red[threadIdx.x] = v;
@@ -64,10 +66,32 @@ __global__ void k_rhs_fix(obte_fourier_plan_t P, double* bh, const double* part,
   if (threadIdx.x == 0 && P.pin >= 0) bh[P.pin] = -s;
 }
 
-// r = bh - A x, z = r / diag, p = z; partials of r.z, r.r, bh.bh
-__global__ void k_cg_init(obte_fourier_plan_t P, const double* bh, const double* x, double* r, double* z, double* p,
+// Sum of v over the workgroup in a fixed order (xor tree inside each wave,
+// then the wave totals in wave order); every thread gets the same bits.
+// red: 16 doubles, free on entry (a barrier precedes the first write).
+__device__ __forceinline__ double wg_sum(double v, double* red) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int j = 0; j < (int)(blockDim.x >> 6); ++j) s += red[j];
+  return s;
+}
+
+// sum of the nb block partials, computed the same way by every block
+__device__ __forceinline__ double sum_partials(const double* part, int nb, double* red) {
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) s += part[i];
+  return wg_sum(s, red);
+}
+
+// Partial sums of the multi-launch CG live after the five work vectors:
+// [pq | rz | rr | bb], nb doubles each (4 nb <= n + 64 for every n).
+// r = bh - A x, z = r / diag; partials of r.z, r.r, bh.bh
+__global__ void k_cg_init(obte_fourier_plan_t P, const double* bh, const double* x, double* r, double* z,
                           double* part, int nb) {
-  __shared__ double red[kB];
+  __shared__ double red[16];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double rz = 0.0, rr = 0.0, bb = 0.0;
   if (i < P.n) {
@@ -75,119 +99,142 @@ __global__ void k_cg_init(obte_fourier_plan_t P, const double* bh, const double*
     const double zi = ri / P.diag[i];
     r[i] = ri;
     z[i] = zi;
-    p[i] = zi;
     rz = ri * zi;
     rr = ri * ri;
     bb = bh[i] * bh[i];
   }
-  rz = block_sum(rz, red);
-  rr = block_sum(rr, red);
-  bb = block_sum(bb, red);
+  rz = wg_sum(rz, red);
+  rr = wg_sum(rr, red);
+  bb = wg_sum(bb, red);
   if (threadIdx.x == 0) {
-    part[blockIdx.x] = rz;
-    part[nb + blockIdx.x] = rr;
-    part[2 * nb + blockIdx.x] = bb;
+    part[nb + blockIdx.x] = rz;
+    part[2 * nb + blockIdx.x] = rr;
+    part[3 * nb + blockIdx.x] = bb;
   }
 }
 
-__global__ void k_cg_init_fin(const double* part, int nb, double* scal) {
-  __shared__ double red[1024];
-  double a = 0.0, b = 0.0, c = 0.0;
-  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
-    a += part[i];
-    b += part[nb + i];
-    c += part[2 * nb + i];
-  }
-  a = block_sum(a, red);
-  b = block_sum(b, red);
-  c = block_sum(c, red);
+// scal[S_RR], scal[S_BB] from the partials (host convergence test)
+__global__ void k_cg_norms(const double* part, int nb, double* scal, int with_bb) {
+  __shared__ double red[16];
+  const double rr = sum_partials(part + 2 * nb, nb, red);
+  const double bb = with_bb ? sum_partials(part + 3 * nb, nb, red) : 0.0;
   if (threadIdx.x == 0) {
-    scal[S_RZ] = a;
-    scal[S_RR] = b;
-    scal[S_BB] = c;
+    scal[S_RR] = rr;
+    if (with_bb) scal[S_BB] = bb;
   }
 }
 
-// q = A p, partial p.q
-__global__ void k_cg_spmv(obte_fourier_plan_t P, const double* p, double* q, double* part) {
-  __shared__ double red[kB];
+// Iteration k, first half: rz_k = r.z (every block sums the partials of the
+// previous update), beta = rz_k / rz_(k-1), p_k = z + beta p_(k-1) formed on
+// the fly for the row and its neighbours (p_(k-1) is read from pold, p_k
+// written to pnew: no block reads what another writes), q = A p_k, partial
+// p.q.  rz_k goes to scal slot k & 1 (slot (k - 1) & 1 holds rz_(k-1)).
+// The row's loads (up to kRowHoist entries) are issued before the partial
+// sums, whose barriers do not wait for them.
+constexpr int kRowHoist = 4;
+
+__global__ void k_cg_pq(obte_fourier_plan_t P, const double* z, const double* pold, double* pnew, double* q,
+                        double* part, double* scal, int nb, int k) {
+  __shared__ double red[16];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  double pq = 0.0;
-  if (i < P.n) {
-    const double qi = apply_row(P, p, i);
-    q[i] = qi;
-    pq = p[i] * qi;
+  const bool row = i < P.n;
+  const bool lag = k > 0;
+  double di = 0.0, zi = 0.0, pi0 = 0.0;
+  double we[kRowHoist], ze[kRowHoist], pe[kRowHoist];
+  int e0 = 0, len = 0;
+  if (row) {
+    e0 = P.row_ptr[i];
+    len = P.row_ptr[i + 1] - e0;
+    di = P.diag[i];
+    zi = z[i];
+    pi0 = lag ? pold[i] : 0.0;
   }
-  pq = block_sum(pq, red);
+#pragma unroll
+  for (int e = 0; e < kRowHoist; ++e) {
+    we[e] = 0.0;
+    ze[e] = 0.0;
+    pe[e] = 0.0;
+    if (e < len) {
+      const int j = P.col[e0 + e];
+      we[e] = P.w[e0 + e];
+      ze[e] = z[j];
+      pe[e] = lag ? pold[j] : 0.0;
+    }
+  }
+  const double rz = sum_partials(part + nb, nb, red);
+  double beta = 0.0;
+  if (lag) {
+    const double rzo = scal[S_RZ0 + ((k - 1) & 1)];
+    beta = rzo != 0.0 ? rz / rzo : 0.0;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) scal[S_RZ0 + (k & 1)] = rz;
+  double pq = 0.0;
+  if (row) {
+    const double pi = lag ? zi + beta * pi0 : zi;
+    double v = di * pi;
+#pragma unroll
+    for (int e = 0; e < kRowHoist; ++e)
+      if (e < len) v -= we[e] * (lag ? ze[e] + beta * pe[e] : ze[e]);
+    for (int e = kRowHoist; e < len; ++e) {  // rows longer than the hoisted part
+      const int j = P.col[e0 + e];
+      v -= P.w[e0 + e] * (lag ? z[j] + beta * pold[j] : z[j]);
+    }
+    pnew[i] = pi;
+    q[i] = v;
+    pq = pi * v;
+  }
+  pq = wg_sum(pq, red);
   if (threadIdx.x == 0) part[blockIdx.x] = pq;
 }
 
-__global__ void k_cg_spmv_fin(const double* part, int nb, double* scal) {
-  __shared__ double red[1024];
-  double a = 0.0;
-  for (int i = threadIdx.x; i < nb; i += blockDim.x) a += part[i];
-  a = block_sum(a, red);
-  if (threadIdx.x == 0) {
-    scal[S_PQ] = a;
-    scal[S_ALPHA] = a != 0.0 ? scal[S_RZ] / a : 0.0;
-  }
-}
-
-// x += alpha p, r -= alpha q, z = r / diag; partials r.z, r.r
-__global__ void k_cg_update(obte_fourier_plan_t P, double* x, double* r, double* z, const double* p, const double* q,
-                            const double* scal, double* part, int nb) {
-  __shared__ double red[kB];
+// Iteration k, second half: alpha = rz_k / (p.q) (every block sums the p.q
+// partials); x += alpha p, r -= alpha q, z = r / diag; partials r.z, r.r.
+// The row's loads go out before the partial sums.
+__global__ void k_cg_xr(obte_fourier_plan_t P, double* x, double* r, double* z, const double* p, const double* q,
+                        double* part, const double* scal, int nb, int k) {
+  __shared__ double red[16];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const double alpha = scal[S_ALPHA];
-  double rz = 0.0, rr = 0.0;
-  if (i < P.n) {
-    x[i] += alpha * p[i];
-    const double ri = r[i] - alpha * q[i];
-    const double zi = ri / P.diag[i];
-    r[i] = ri;
-    z[i] = zi;
-    rz = ri * zi;
-    rr = ri * ri;
+  const bool row = i < P.n;
+  double xi = 0.0, pi = 0.0, ri = 0.0, qi = 0.0, di = 1.0;
+  if (row) {
+    xi = x[i];
+    pi = p[i];
+    ri = r[i];
+    qi = q[i];
+    di = P.diag[i];
   }
-  rz = block_sum(rz, red);
-  rr = block_sum(rr, red);
+  const double pq = sum_partials(part, nb, red);
+  const double rz = scal[S_RZ0 + (k & 1)];
+  const double alpha = pq != 0.0 ? rz / pq : 0.0;
+  double rzn = 0.0, rr = 0.0;
+  if (row) {
+    x[i] = xi + alpha * pi;
+    const double rn = ri - alpha * qi;
+    const double zn = rn / di;
+    r[i] = rn;
+    z[i] = zn;
+    rzn = rn * zn;
+    rr = rn * rn;
+  }
+  rzn = wg_sum(rzn, red);
+  rr = wg_sum(rr, red);
   if (threadIdx.x == 0) {
-    part[blockIdx.x] = rz;
-    part[nb + blockIdx.x] = rr;
+    part[nb + blockIdx.x] = rzn;
+    part[2 * nb + blockIdx.x] = rr;
   }
-}
-
-__global__ void k_cg_update_fin(const double* part, int nb, double* scal) {
-  __shared__ double red[1024];
-  double a = 0.0, b = 0.0;
-  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
-    a += part[i];
-    b += part[nb + i];
-  }
-  a = block_sum(a, red);
-  b = block_sum(b, red);
-  if (threadIdx.x == 0) {
-    scal[S_BETA] = scal[S_RZ] != 0.0 ? a / scal[S_RZ] : 0.0;
-    scal[S_RZ] = a;
-    scal[S_RR] = b;
-  }
-}
-
-__global__ void k_cg_dir(int n, const double* z, double* p, const double* scal) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = z[i] + scal[S_BETA] * p[i];
 }
 
 // ---------------------------------------------------------------------------
 // Small systems: the whole solve (right-hand side, CG iterations with the
 // residual test every iteration, centring) in one workgroup of 1024 threads,
-// one launch.  A multi-launch iteration costs five launches (~20 us); here an
-// iteration costs a few workgroup barriers and dependent L2 round trips, which
-// grow with the rows per thread: measured ~10 us per iteration at 2k cells,
-// ~34 us at 10k, so the one-workgroup solve takes systems up to 4096 cells.
+// one launch.  A multi-launch iteration costs two launches (~11.5 us from 5k
+// to 200k cells); here an iteration costs a few workgroup barriers and
+// dependent L2 round trips, which grow with the rows per thread: measured
+// ~9 us per iteration at 2k cells, ~12 us at 3k and ~14.5 us at 4k, so the
+// one-workgroup solve takes systems up to 3072 cells.
 // ---------------------------------------------------------------------------
 constexpr int kCgBlock = 1024;
-constexpr int kCgBlockMaxN = 4096;
+constexpr int kCgBlockMaxN = 3072;
 
 // sums of NV values over the workgroup in a fixed order (xor tree inside each
 // wave, then the wave totals in wave order); red: NV * 16 doubles
@@ -459,7 +506,7 @@ int obte_fourier_solve(const obte_fourier_plan_t* plan, const double* b, const d
   double* z = ws + 2 * (int64_t)n;
   double* p = ws + 3 * (int64_t)n;
   double* q = ws + 4 * (int64_t)n;
-  double* part = ws + 5 * (int64_t)n;  // n + 64 doubles >= 3 * nb
+  double* part = ws + 5 * (int64_t)n;  // n + 64 doubles >= 4 * nb
   if (check_every <= 0) check_every = 16;
   double host[16];
 
@@ -478,8 +525,10 @@ int obte_fourier_solve(const obte_fourier_plan_t* plan, const double* b, const d
 
   hipLaunchKernelGGL(k_rhs, dim3(nb), dim3(kB), 0, st, P, b, c, bh, part);
   if (P.pin >= 0) hipLaunchKernelGGL(k_rhs_fix, dim3(1), dim3(1024), 0, st, P, bh, part, (int)nb);
-  hipLaunchKernelGGL(k_cg_init, dim3(nb), dim3(kB), 0, st, P, bh, t, r, z, p, part, (int)nb);
-  hipLaunchKernelGGL(k_cg_init_fin, dim3(1), dim3(1024), 0, st, part, (int)nb, scal);
+  // bh is dead after the initial residual: it holds the second direction buffer
+  double* pbuf[2] = {p, bh};
+  hipLaunchKernelGGL(k_cg_init, dim3(nb), dim3(kB), 0, st, P, bh, t, r, z, part, (int)nb);
+  hipLaunchKernelGGL(k_cg_norms, dim3(1), dim3(1024), 0, st, part, (int)nb, scal, 1);
   rc = status("obte_fourier_solve: init launch failed");
   if (rc) return rc;
   if (hipMemcpyAsync(host, scal, 8 * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
@@ -491,12 +540,12 @@ int obte_fourier_solve(const obte_fourier_plan_t* plan, const double* b, const d
   const double target = rtol * (bnorm > 0.0 ? bnorm : 1.0);
   while (rnorm > target && it < maxit) {
     for (int k = 0; k < check_every && it < maxit; ++k, ++it) {
-      hipLaunchKernelGGL(k_cg_spmv, dim3(nb), dim3(kB), 0, st, P, p, q, part);
-      hipLaunchKernelGGL(k_cg_spmv_fin, dim3(1), dim3(1024), 0, st, part, (int)nb, scal);
-      hipLaunchKernelGGL(k_cg_update, dim3(nb), dim3(kB), 0, st, P, t, r, z, p, q, scal, part, (int)nb);
-      hipLaunchKernelGGL(k_cg_update_fin, dim3(1), dim3(1024), 0, st, part, (int)nb, scal);
-      hipLaunchKernelGGL(k_cg_dir, dim3(nb), dim3(kB), 0, st, n, z, p, scal);
+      double* pk = pbuf[it & 1];
+      hipLaunchKernelGGL(k_cg_pq, dim3(nb), dim3(kB), 0, st, P, z, pbuf[(it + 1) & 1], pk, q, part, scal, (int)nb,
+                         it);
+      hipLaunchKernelGGL(k_cg_xr, dim3(nb), dim3(kB), 0, st, P, t, r, z, pk, q, part, scal, (int)nb, it);
     }
+    hipLaunchKernelGGL(k_cg_norms, dim3(1), dim3(1024), 0, st, part, (int)nb, scal, 0);
     rc = status("obte_fourier_solve: iteration launch failed");
     if (rc) return rc;
     if (hipMemcpyAsync(host, scal, 8 * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||

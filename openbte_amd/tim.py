@@ -150,6 +150,7 @@ class TIMSolver:
             cfg = SweepConfig(**{**vars(cfg), 'rtol': 1e-13})
         self.dev = BTEDevice(op, sched=self.sched, cfg=cfg)
         d = self.dev
+        d.keep_solutions(True)  # the gradient reads the forward and adjoint solutions from t
         dv = d.dev
         self.vol = torch.as_tensor(np.asarray(geo.elem_volumes, np.float64), device=dv)
         self.i_face = torch.as_tensor(op.i_mat, dtype=torch.int32, device=dv)
