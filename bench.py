@@ -560,15 +560,23 @@ def main(argv=None):
     by_mode = {m: [a.elapsed_time(b) * 1e-3 for (mm, a, b) in timer if mm == m] for m in (0, 1, 2)}
     mean_ms = {f'mode{m}': (float(np.mean(v)) * 1e3 if v else None) for m, v in by_mode.items()}
     total_ms = {f'mode{m}': float(np.sum(v)) * 1e3 for m, v in by_mode.items()}
-    # roofline of the final full sweep (mode 2, every solution value written):
-    # algorithmic bytes = 28 B per cell*mode update for triangles, 32 B for
-    # tetrahedra (source read 8 + solution write 8 + 8 per upwind gather, 1.5 /
-    # 2 inflow faces on average) x the n*M*L updates of one launch; its
-    # compulsory HBM bytes (records read once, one write of every value, the
-    # interface inflows) are reported beside it
+    # roofline of the final full sweep (mode 2): algorithmic bytes = 28 B per
+    # cell*mode update for triangles, 32 B for tetrahedra (source read 8 +
+    # solution write 8 + 8 per upwind gather, 1.5 / 2 inflow faces on average)
+    # x the n*M*L updates of one launch.  Its compulsory HBM bytes are reported
+    # beside it: records read once, the interface inflows, and either one
+    # write of every value or, when the sweep forms the mode sums from its LDS
+    # window, the per-chunk sums (dim + 1 doubles per cell and chunk; the
+    # values later levels read from t come on top)
     per_update = 28.0 if geo.dim == 2 else 32.0
     sweep_bytes = float(n * M * L) * per_update
-    compulsory = float(L * n * 64 + L * n * M * 8 + dev.n_cut * M * 8)
+    fused = dev.window_positions() > 0 and 'part_chunk' in dev.buf and not dev.plan.keep_t
+    if fused:
+        compulsory = float(L * n * 64 + L * dev.n_chunks * n * (geo.dim + 1) * 8 + dev.n_cut * M * 8)
+        kernel = 'k_sweep final sweep (mode 2: wall temperatures, kappa, per-chunk mode sums from the LDS window)'
+    else:
+        compulsory = float(L * n * 64 + L * n * M * 8 + dev.n_cut * M * 8)
+        kernel = 'k_sweep final sweep (mode 2: every solution value written)'
     t_full = float(np.mean(by_mode[2]))
     achieved = sweep_bytes / t_full
     print(f'# {args.steps} steps in {dt:.3f}s: {value:.3e} updates/s; extensions per step {exts}; '
@@ -606,7 +614,7 @@ def main(argv=None):
         'roofline': {'bound': 'hbm', 'achieved': achieved / 1e9, 'peak': HBM_PEAK / 1e9, 'unit': 'GB/s',
                      'frac': achieved / HBM_PEAK, 'traffic': traffic,
                      'traffic_source': 'rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE (profiles/)' if traffic else None,
-                     'kernel': 'k_sweep mode 2 (final full sweep: all solution values written)',
+                     'kernel': kernel,
                      'bytes_per_launch': sweep_bytes, 'bytes_per_update': per_update,
                      'compulsory_bytes_per_launch': compulsory, 'mean_launch_s': t_full,
                      'sweep_ms_mean_by_mode': mean_ms, 'sweep_ms_total_by_mode': total_ms},

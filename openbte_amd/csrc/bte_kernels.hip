@@ -1246,22 +1246,23 @@ __global__ __launch_bounds__(256) void k_modesum(S src) {
   }
   if (threadIdx.x == 0) sigma_total(P, l, stot);
   __syncthreads();
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
   const int64_t rows = (int64_t)n + P.ncut_max;
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int c = 0; c < nch; ++c) {
-    const double2* row = reinterpret_cast<const double2*>(P.t + (((int64_t)l * nch + c) * rows + p) * MC);
-    const double2* wc = reinterpret_cast<const double2*>(w + 8 * c * G);
-    double e[4] = {0.0, 0.0, 0.0, 0.0};
-    chunk_sums<G, false>([&](int j) { return row[j]; }, [&](int k, int j) { return wc[4 * j + k]; }, dim, e);
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int c = 0; c < nch; ++c) {
+      const double2* row = reinterpret_cast<const double2*>(P.t + (((int64_t)l * nch + c) * rows + p) * MC);
+      const double2* wc = reinterpret_cast<const double2*>(w + 8 * c * G);
+      double e[4] = {0.0, 0.0, 0.0, 0.0};
+      chunk_sums<G, false>([&](int j) { return row[j]; }, [&](int k, int j) { return wc[4 * j + k]; }, dim, e);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) acc[k] += e[k];
+      for (int k = 0; k < 4; ++k) acc[k] += e[k];
+    }
+    modesum_store(P, l, p, acc, stot);
   }
-  modesum_store(P, l, p, acc, stot);
 }
 
-// From the per-chunk sums of a windowed final sweep: one thread per position.
+// From the per-chunk sums of a windowed final sweep: one thread per position,
+// the chunks added in order.
 template <class S>
 __global__ __launch_bounds__(256) void k_modesum_part(S src) {
   __shared__ double stot[4];
@@ -1273,17 +1274,17 @@ __global__ __launch_bounds__(256) void k_modesum_part(S src) {
   if (l >= P.n_dirs || (int64_t)blockIdx.x * blockDim.x >= n) return;  // uniform over the block
   if (threadIdx.x == 0) sigma_total(P, l, stot);
   __syncthreads();
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int c = 0; c < nch; ++c) {
-    const double* e = P.part_chunk + (((int64_t)l * nch + c) * n + p) * (dim + 1);
-    acc[0] += e[0];
-    acc[1] += e[1];
-    acc[2] += e[2];
-    if (dim == 3) acc[3] += e[3];
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int c = 0; c < nch; ++c) {
+      const double* e = P.part_chunk + (((int64_t)l * nch + c) * n + p) * (dim + 1);
+      acc[0] += e[0];
+      acc[1] += e[1];
+      acc[2] += e[2];
+      if (dim == 3) acc[3] += e[3];
+    }
+    modesum_store(P, l, p, acc, stot);
   }
-  modesum_store(P, l, p, acc, stot);
 }
 
 // Stage 2: fixed-order sum over directions, gathering each direction's
@@ -1473,17 +1474,22 @@ struct SweepMany {
   }
 };
 
-// parts: sum the per-chunk sums of a windowed final sweep; else read t
+// parts: sum the per-chunk sums of a windowed final sweep; else read t.
+// Grid-stride kernels with a bounded number of blocks per direction, so the
+// per-block set-up (weights, sum of sigma) is paid a few times per direction.
+constexpr int kModesumBlocksPerDir = 24;
+
 template <int MC, int KIN>
 struct ModesumL {
   template <class S>
   static void run(const S& src, bool parts, int max_cells, int dirs, int problems, hipStream_t st) {
-    if (parts)
-      hipLaunchKernelGGL((k_modesum_part<S>), dim3(blocks_for(max_cells, 256), dirs, problems), dim3(256), 0, st,
-                         src);
-    else
-      hipLaunchKernelGGL((k_modesum<S, MC>), dim3(blocks_for(max_cells, 256), dirs, problems), dim3(256), 0, st,
-                         src);
+    if (parts) {
+      const unsigned bx = min(blocks_for(max_cells, 256), (unsigned)kModesumBlocksPerDir);
+      hipLaunchKernelGGL((k_modesum_part<S>), dim3(bx, dirs, problems), dim3(256), 0, st, src);
+    } else {
+      const unsigned bx = min(blocks_for(max_cells, 256), (unsigned)kModesumBlocksPerDir);
+      hipLaunchKernelGGL((k_modesum<S, MC>), dim3(bx, dirs, problems), dim3(256), 0, st, src);
+    }
   }
 };
 
