@@ -4,11 +4,13 @@ The sweep kernel has two shape knobs: ``mc`` (MFP bins per task: the width of
 a cell's row in the solution workspace, hence MC/2 lanes per cell) and
 ``block`` (threads per workgroup, i.e. cells in flight per level of one task).
 Narrow levels (small meshes) favour small blocks and many resident tasks; wide
-levels favour 256-thread blocks.  The tuner times full sweeps (mode 0, all
-tasks, the dominant launch of an outer iteration) for each candidate on the
-actual problems, with HIP events on the launch stream, and keeps the fastest
-per cell*mode update.  Results are cached in memory and, if ``cache`` names a
-JSON file, on disk, keyed by the problem signature.
+levels favour 256-thread blocks; smaller chunks let transfer sweeps skip
+converged bins at a finer grain.  The tuner times whole outer iterations (the
+first ones of a solve from DeltaT = 0: source, transfer and final sweeps,
+interface Krylov steps, mode sums) for each candidate on the actual problems,
+with HIP events on the launch stream, and keeps the fastest.  Results are
+cached in memory and, if ``cache`` names a JSON file, on disk, keyed by the
+problem signature.
 """
 from __future__ import annotations
 
@@ -20,7 +22,6 @@ from typing import Dict, Iterable, List, Optional, Sequence, Tuple
 import numpy as np
 import torch
 
-from . import _lib
 from .bte import BTEDevice, SweepConfig
 
 CANDIDATES: Tuple[Tuple[int, int], ...] = ((4, 64), (4, 128), (8, 64), (8, 128), (8, 256), (16, 128), (16, 256))
@@ -33,23 +34,30 @@ def signature(ops: Sequence) -> str:
     return hashlib.sha1(json.dumps(rows).encode()).hexdigest()[:16]
 
 
-def _time_launch(fn, reps: int) -> float:
+def _time_iterations(obj, t0, n: int) -> float:
+    """Seconds per outer iteration of ``obj`` (a BTEDevice or BTEBatch) over
+    iterations 1..n from ``t0`` (after one untimed iteration that loads the
+    kernels), with HIP events on the launch stream."""
+    obj.reset(t0)
+    obj.outer_iteration()
+    obj.reset(t0)
     st = torch.cuda.current_stream()
-    fn()  # warm-up
+    torch.cuda.synchronize()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     e0.record(st)
-    for _ in range(reps):
-        fn()
+    for _ in range(n):
+        obj.outer_iteration()
     e1.record(st)
     e1.synchronize()
-    return e0.elapsed_time(e1) * 1e-3 / reps
+    return e0.elapsed_time(e1) * 1e-3 / n
 
 
 def tune(ops: Sequence, scheds: Optional[Sequence] = None, candidates: Iterable[Tuple[int, int]] = CANDIDATES,
-         reps: int = 3, cache: Optional[str] = None, verbose: bool = False) -> SweepConfig:
+         reps: int = 2, cache: Optional[str] = None, verbose: bool = False) -> SweepConfig:
     """Pick (mc, block) for solving ``ops`` together (one problem: the plain
-    path; several: the batched path).  Returns a ``SweepConfig``."""
+    path; several: the batched path): the candidate with the shortest outer
+    iterations 1..reps from DeltaT = 0.  Returns a ``SweepConfig``."""
     from .batch import BTEBatch
     key = signature(ops)
     rec = _memo.get(key)
@@ -60,26 +68,20 @@ def tune(ops: Sequence, scheds: Optional[Sequence] = None, candidates: Iterable[
         updates = float(sum(o.n * o.n_mfp * o.n_dirs for o in ops))
         results: List[dict] = []
         for mc, block in candidates:
-            cfg = SweepConfig(mc=mc, block=block, kmax=8)
+            cfg = SweepConfig(mc=mc, block=block)
             if len(ops) == 1:
-                d = BTEDevice(ops[0], sched=None if scheds is None else scheds[0], cfg=cfg)
-                d.reset(np.zeros(ops[0].n))
-                _lib.call('obte_bte_source', d.plan, d.stream)
-                fn = (lambda d=d: _lib.call('obte_bte_sweep', d.plan, 0, d.gbuf['y'].data_ptr(),
-                                            d.gbuf['x'].data_ptr(), 0, 0, torch.cuda.current_stream().cuda_stream))
+                obj = BTEDevice(ops[0], sched=None if scheds is None else scheds[0], cfg=cfg)
+                t0 = np.zeros(ops[0].n)
             else:
-                b = BTEBatch(ops, cfg=cfg, scheds=scheds)
-                b.reset([np.zeros(o.n) for o in ops])
-                _lib.call('obte_batch_source', b.b, b.stream)
-                fn = (lambda b=b: _lib.call('obte_batch_sweep', b.b, 0, torch.cuda.current_stream().cuda_stream))
-            t = _time_launch(fn, reps)
-            results.append({'mc': mc, 'block': block, 'sweep_s': t, 'updates_per_s': updates / t})
+                obj = BTEBatch(ops, cfg=cfg, scheds=scheds)
+                t0 = [np.zeros(o.n) for o in ops]
+            t = _time_iterations(obj, t0, reps)
+            results.append({'mc': mc, 'block': block, 'iter_s': t, 'updates_per_s': updates / t})
             if verbose:
-                print(f'# tune mc={mc} block={block}: sweep {t * 1e3:.3f} ms, {updates / t:.3e} updates/s',
-                      flush=True)
-            del fn
+                print(f'# tune mc={mc} block={block}: {t * 1e3:.3f} ms per outer iteration', flush=True)
+            del obj
             torch.cuda.empty_cache()
-        best = min(results, key=lambda r: r['sweep_s'])
+        best = min(results, key=lambda r: r['iter_s'])
         rec = {'mc': best['mc'], 'block': best['block'], 'results': results}
         if cache:
             data = {}

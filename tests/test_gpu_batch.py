@@ -75,3 +75,37 @@ def test_batch_graph_replay_matches_eager():
         for de, dg in zip(eager.devs, graph.devs):
             np.testing.assert_array_equal(de.buf['dt'][0].cpu().numpy(), dg.buf['dt'][0].cpu().numpy())
             np.testing.assert_array_equal(de.buf['kappa'].cpu().numpy(), dg.buf['kappa'].cpu().numpy())
+
+
+def test_tuner_picks_a_candidate_and_memoizes():
+    """The launch-shape tuner times every candidate (whole outer iterations)
+    on the actual problems, returns the fastest and reuses it for the same
+    problem signature; the tuned shape solves like the default."""
+    from openbte_amd import tune
+    from openbte_amd.batch import BTEBatch
+    from openbte_amd.bte import SweepConfig
+    from openbte_amd.operator import build_host_operator
+    probs = [_problem(21, 400), _problem(22, 700)]
+    ops = [build_host_operator(p['geo'], p['mat'], p['bcs'], p['effective_thermal_conductivity']) for p in probs]
+    cands = ((4, 128), (8, 256), (16, 256))
+    print('# tune batch', flush=True)
+    cfg = tune.tune(ops, candidates=cands, reps=2)
+    res = tune.last_results(ops)
+    assert (cfg.mc, cfg.block) in cands
+    assert len(res) == len(cands) and all(r['iter_s'] > 0 for r in res)
+    assert min(res, key=lambda r: r['iter_s'])['mc'] == cfg.mc
+    again = tune.tune(ops, candidates=cands)
+    assert (again.mc, again.block) == (cfg.mc, cfg.block) and tune.last_results(ops) is res
+    print('# tune single', flush=True)
+    one = tune.tune(ops[:1], candidates=cands, reps=2)
+    assert (one.mc, one.block) in cands
+    t0s = [np.zeros(o.n) for o in ops]
+    ks = []
+    for c in (cfg, SweepConfig()):
+        b = BTEBatch(ops, cfg=SweepConfig(mc=c.mc, block=c.block))
+        b.reset(t0s)
+        for _ in range(3):
+            b.outer_iteration()
+        ks.append([float(d.buf['kappa'].sum().item()) for d in b.devs])
+        del b
+    np.testing.assert_allclose(ks[0], ks[1], rtol=1e-9)
