@@ -311,10 +311,17 @@ typedef struct obte_fourier_plan {
   const double* c_coef;      /* v_orth (periodic) or conductance-limited v_orth (mixed) */
   double c_value;            /* periodic jump or wall temperature */
   double normalization;
+  /* two-level preconditioner of the multi-launch CG (optional): aggregate k is
+   * cells 256 k .. 256 k + 255 (cells numbered so that these are compact);
+   * coarse_inv is the inverse of the aggregated operator P^T A P (plus a
+   * constant when A is singular) */
+  int32_t n_coarse;          /* aggregates (= ceil(n / 256)), or 0: Jacobi only */
+  const double* coarse_inv;  /* [n_coarse][n_coarse] */
 } obte_fourier_plan_t;
 
 /* Solve F T = B + C (openbte/fourier.py:152-156) as the symmetric system
- * A T = V (B + C) by Jacobi-preconditioned CG, T holding the initial guess;
+ * A T = V (B + C) by preconditioned CG (Jacobi, plus the additive coarse
+ * correction P coarse_inv P^T r when n_coarse > 0), T holding the initial guess;
  * with a pinned cell the pinned row is replaced by the compatibility
  * condition and T is centred by (max + min) / 2.  ws: 6n + 64 doubles; scal: 16
  * doubles.  iters_out / relres_out are host pointers (may be NULL). */

@@ -76,6 +76,7 @@ class FourierPlan(C.Structure):
         ('side_e', _p), ('side_w', _p), ('side_vn', _p), ('slot_side', _p), ('slot_sgn', _p),
         ('n_contact', _i32), ('contact_kind', _i32), ('ce1', _p), ('ce2', _p), ('c_coef', _p),
         ('c_value', _f64), ('normalization', _f64),
+        ('n_coarse', _i32), ('coarse_inv', _p),
     ]
 
 

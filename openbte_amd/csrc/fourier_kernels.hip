@@ -87,8 +87,13 @@ __device__ __forceinline__ double sum_partials(const double* part, int nb, doubl
 }
 
 // Partial sums of the multi-launch CG live after the five work vectors:
-// [pq | rz | rr | bb], nb doubles each (4 nb <= n + 64 for every n).
-// r = bh - A x, z = r / diag; partials of r.z, r.r, bh.bh
+// [pq | rz | rr | bb | rc | ec], nb doubles each (6 nb <= n + 64 for every n).
+// rc[k] = sum of r over block k (the restriction P^T r: block k is aggregate
+// k), ec = coarse_inv rc (the coarse correction, one value per aggregate).
+// The preconditioned residual is z + ec[i / 256] with z = r / diag.
+constexpr int kAggShift = 8;  // 256 cells per aggregate = one block of the CG kernels
+
+// r = bh - A x, z = r / diag; partials of r.z, r.r, bh.bh and r
 __global__ void k_cg_init(obte_fourier_plan_t P, const double* bh, const double* x, double* r, double* z,
                           double* part, int nb) {
   __shared__ double red[16];
@@ -103,6 +108,9 @@ __global__ void k_cg_init(obte_fourier_plan_t P, const double* bh, const double*
     rr = ri * ri;
     bb = bh[i] * bh[i];
   }
+  double rsum = 0.0;
+  if (i < P.n) rsum = r[i];
+  const double rs = wg_sum(rsum, red);
   rz = wg_sum(rz, red);
   rr = wg_sum(rr, red);
   bb = wg_sum(bb, red);
@@ -110,7 +118,35 @@ __global__ void k_cg_init(obte_fourier_plan_t P, const double* bh, const double*
     part[nb + blockIdx.x] = rz;
     part[2 * nb + blockIdx.x] = rr;
     part[3 * nb + blockIdx.x] = bb;
+    part[4 * nb + blockIdx.x] = rs;
   }
+}
+
+// ec = coarse_inv rc: four aggregates per block, each row summed in a fixed
+// order.
+constexpr int kCoarseRows = 4;
+
+__global__ void k_cg_coarse(obte_fourier_plan_t P, double* part, int nb) {
+  __shared__ double red[16];
+  const int nc = P.n_coarse;
+  const double* rc = part + 4 * nb;
+  double* ec = part + 5 * nb;
+  for (int q = 0; q < kCoarseRows; ++q) {
+    const int I = blockIdx.x * kCoarseRows + q;  // uniform over the block
+    if (I >= nc) break;
+    const double* row = P.coarse_inv + (int64_t)I * nc;
+    double s = 0.0;
+    for (int J = threadIdx.x; J < nc; J += blockDim.x) s += row[J] * rc[J];
+    s = wg_sum(s, red);
+    if (threadIdx.x == 0) ec[I] = s;
+  }
+}
+
+// rc . ec (the coarse part of r.M^-1 r), computed the same way by every block
+__device__ __forceinline__ double coarse_dot(const obte_fourier_plan_t& P, const double* part, int nb, double* red) {
+  double s = 0.0;
+  for (int I = threadIdx.x; I < P.n_coarse; I += blockDim.x) s += part[4 * nb + I] * part[5 * nb + I];
+  return wg_sum(s, red);
 }
 
 // scal[S_RR], scal[S_BB] from the partials (host convergence test)
@@ -139,6 +175,8 @@ __global__ void k_cg_pq(obte_fourier_plan_t P, const double* z, const double* po
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool row = i < P.n;
   const bool lag = k > 0;
+  const bool two = P.n_coarse > 0;
+  const double* ec = part + 5 * nb;
   double di = 0.0, zi = 0.0, pi0 = 0.0;
   double we[kRowHoist], ze[kRowHoist], pe[kRowHoist];
   int e0 = 0, len = 0;
@@ -146,7 +184,7 @@ __global__ void k_cg_pq(obte_fourier_plan_t P, const double* z, const double* po
     e0 = P.row_ptr[i];
     len = P.row_ptr[i + 1] - e0;
     di = P.diag[i];
-    zi = z[i];
+    zi = two ? z[i] + ec[i >> kAggShift] : z[i];
     pi0 = lag ? pold[i] : 0.0;
   }
 #pragma unroll
@@ -157,11 +195,12 @@ __global__ void k_cg_pq(obte_fourier_plan_t P, const double* z, const double* po
     if (e < len) {
       const int j = P.col[e0 + e];
       we[e] = P.w[e0 + e];
-      ze[e] = z[j];
+      ze[e] = two ? z[j] + ec[j >> kAggShift] : z[j];
       pe[e] = lag ? pold[j] : 0.0;
     }
   }
-  const double rz = sum_partials(part + nb, nb, red);
+  double rz = sum_partials(part + nb, nb, red);
+  if (two) rz += coarse_dot(P, part, nb, red);
   double beta = 0.0;
   if (lag) {
     const double rzo = scal[S_RZ0 + ((k - 1) & 1)];
@@ -177,7 +216,8 @@ __global__ void k_cg_pq(obte_fourier_plan_t P, const double* z, const double* po
       if (e < len) v -= we[e] * (lag ? ze[e] + beta * pe[e] : ze[e]);
     for (int e = kRowHoist; e < len; ++e) {  // rows longer than the hoisted part
       const int j = P.col[e0 + e];
-      v -= P.w[e0 + e] * (lag ? z[j] + beta * pold[j] : z[j]);
+      const double zj = two ? z[j] + ec[j >> kAggShift] : z[j];
+      v -= P.w[e0 + e] * (lag ? zj + beta * pold[j] : zj);
     }
     pnew[i] = pi;
     q[i] = v;
@@ -188,7 +228,8 @@ __global__ void k_cg_pq(obte_fourier_plan_t P, const double* z, const double* po
 }
 
 // Iteration k, second half: alpha = rz_k / (p.q) (every block sums the p.q
-// partials); x += alpha p, r -= alpha q, z = r / diag; partials r.z, r.r.
+// partials); x += alpha p, r -= alpha q, z = r / diag; partials r.z, r.r
+// and (two-level) r.
 // The row's loads go out before the partial sums.
 __global__ void k_cg_xr(obte_fourier_plan_t P, double* x, double* r, double* z, const double* p, const double* q,
                         double* part, const double* scal, int nb, int k) {
@@ -206,21 +247,23 @@ __global__ void k_cg_xr(obte_fourier_plan_t P, double* x, double* r, double* z, 
   const double pq = sum_partials(part, nb, red);
   const double rz = scal[S_RZ0 + (k & 1)];
   const double alpha = pq != 0.0 ? rz / pq : 0.0;
-  double rzn = 0.0, rr = 0.0;
+  double rzn = 0.0, rr = 0.0, rn = 0.0;
   if (row) {
     x[i] = xi + alpha * pi;
-    const double rn = ri - alpha * qi;
+    rn = ri - alpha * qi;
     const double zn = rn / di;
     r[i] = rn;
     z[i] = zn;
     rzn = rn * zn;
     rr = rn * rn;
   }
+  const double rs = P.n_coarse > 0 ? wg_sum(rn, red) : 0.0;
   rzn = wg_sum(rzn, red);
   rr = wg_sum(rr, red);
   if (threadIdx.x == 0) {
     part[nb + blockIdx.x] = rzn;
     part[2 * nb + blockIdx.x] = rr;
+    part[4 * nb + blockIdx.x] = rs;
   }
 }
 
@@ -506,7 +549,7 @@ int obte_fourier_solve(const obte_fourier_plan_t* plan, const double* b, const d
   double* z = ws + 2 * (int64_t)n;
   double* p = ws + 3 * (int64_t)n;
   double* q = ws + 4 * (int64_t)n;
-  double* part = ws + 5 * (int64_t)n;  // n + 64 doubles >= 4 * nb
+  double* part = ws + 5 * (int64_t)n;  // n + 64 doubles >= 6 * nb
   if (check_every <= 0) check_every = 16;
   double host[16];
 
@@ -527,7 +570,11 @@ int obte_fourier_solve(const obte_fourier_plan_t* plan, const double* b, const d
   if (P.pin >= 0) hipLaunchKernelGGL(k_rhs_fix, dim3(1), dim3(1024), 0, st, P, bh, part, (int)nb);
   // bh is dead after the initial residual: it holds the second direction buffer
   double* pbuf[2] = {p, bh};
+  if (P.n_coarse != 0 && (P.n_coarse != (int)nb || !P.coarse_inv))
+    return obte_fail(OBTE_EINVAL, "obte_fourier_solve: n_coarse must be ceil(n / 256) with coarse_inv");
+  const unsigned nbc = (unsigned)((P.n_coarse + kCoarseRows - 1) / kCoarseRows);
   hipLaunchKernelGGL(k_cg_init, dim3(nb), dim3(kB), 0, st, P, bh, t, r, z, part, (int)nb);
+  if (P.n_coarse > 0) hipLaunchKernelGGL(k_cg_coarse, dim3(nbc), dim3(kB), 0, st, P, part, (int)nb);
   hipLaunchKernelGGL(k_cg_norms, dim3(1), dim3(1024), 0, st, part, (int)nb, scal, 1);
   rc = status("obte_fourier_solve: init launch failed");
   if (rc) return rc;
@@ -544,6 +591,7 @@ int obte_fourier_solve(const obte_fourier_plan_t* plan, const double* b, const d
       hipLaunchKernelGGL(k_cg_pq, dim3(nb), dim3(kB), 0, st, P, z, pbuf[(it + 1) & 1], pk, q, part, scal, (int)nb,
                          it);
       hipLaunchKernelGGL(k_cg_xr, dim3(nb), dim3(kB), 0, st, P, t, r, z, pk, q, part, scal, (int)nb, it);
+      if (P.n_coarse > 0) hipLaunchKernelGGL(k_cg_coarse, dim3(nbc), dim3(kB), 0, st, P, part, (int)nb);
     }
     hipLaunchKernelGGL(k_cg_norms, dim3(1), dim3(1024), 0, st, part, (int)nb, scal, 0);
     rc = status("obte_fourier_solve: iteration launch failed");
