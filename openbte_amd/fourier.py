@@ -240,7 +240,19 @@ def coarse_inverse(h: FourierHost) -> np.ndarray:
     Ac = Ac.reshape(nc, nc)
     if h.pin >= 0:
         Ac += np.trace(Ac) / (nc * nc)
-    return np.ascontiguousarray(np.linalg.inv(Ac))
+    # a few BLAS threads only: preparation runs in several worker threads at
+    # once (the serving loop), where all-core BLAS calls oversubscribe the host
+    with _blas_threads(4):
+        return np.ascontiguousarray(np.linalg.inv(Ac))
+
+
+def _blas_threads(k: int):
+    try:
+        from threadpoolctl import threadpool_limits
+        return threadpool_limits(limits=k, user_api='blas')
+    except ImportError:  # the limit is an optimisation only
+        import contextlib
+        return contextlib.nullcontext()
 
 
 class FourierDevice:
