@@ -50,42 +50,39 @@ inline double face_dot(const double* s, const double* k, int dim) {
 void build_direction(int32_t n, int64_t n_faces, int dim, const int32_t* i_mat, const int32_t* j_mat,
                      const double* k_mat, const uint8_t* cut_candidate, const double* s, int32_t max_width,
                      DirSchedule& out) {
-  // inflow faces grouped by row cell
-  std::vector<int32_t> cnt(n + 1, 0);
-  std::vector<uint8_t> inflow(n_faces, 0);
+  // one pass: inflow and cut flags, inflow faces per row cell (cnt), kept
+  // (uncut) inflow faces per upstream cell (dcnt) and per row cell (indeg)
+  std::vector<int32_t> cnt(n + 1, 0), dcnt(n + 1, 0), indeg(n, 0);
+  std::vector<uint8_t> flag(n_faces, 0);  // 1 inflow, 3 inflow and cut
   for (int64_t f = 0; f < n_faces; ++f) {
     if (face_dot(s, k_mat + f * dim, dim) < 0.0) {
-      inflow[f] = 1;
-      cnt[i_mat[f] + 1]++;
+      const int32_t i = i_mat[f];
+      cnt[i + 1]++;
+      if (cut_candidate[f]) {
+        flag[f] = 3;
+      } else {
+        flag[f] = 1;
+        dcnt[j_mat[f] + 1]++;
+        indeg[i]++;
+      }
     }
   }
-  for (int32_t c = 0; c < n; ++c) cnt[c + 1] += cnt[c];
-  std::vector<int32_t> row_faces(cnt[n]);
-  {
-    std::vector<int32_t> fill(cnt.begin(), cnt.end() - 1);
-    for (int64_t f = 0; f < n_faces; ++f)
-      if (inflow[f]) row_faces[fill[i_mat[f]]++] = (int32_t)f;
+  for (int32_t c = 0; c < n; ++c) {
+    cnt[c + 1] += cnt[c];
+    dcnt[c + 1] += dcnt[c];
   }
+  // second pass: the faces of each row cell and each upstream cell
+  std::vector<int32_t> row_faces(cnt[n]), down_face(dcnt[n]);
   std::vector<uint8_t> is_cut(n_faces, 0);
-  for (int64_t f = 0; f < n_faces; ++f)
-    if (inflow[f] && cut_candidate[f]) is_cut[f] = 1;
-
-  // downstream adjacency (upstream cell -> faces feeding others)
-  std::vector<int32_t> dcnt(n + 1, 0);
-  for (int64_t f = 0; f < n_faces; ++f)
-    if (inflow[f] && !is_cut[f]) dcnt[j_mat[f] + 1]++;
-  for (int32_t c = 0; c < n; ++c) dcnt[c + 1] += dcnt[c];
-  std::vector<int32_t> down_face(dcnt[n]);
   {
-    std::vector<int32_t> fill(dcnt.begin(), dcnt.end() - 1);
-    for (int64_t f = 0; f < n_faces; ++f)
-      if (inflow[f] && !is_cut[f]) down_face[fill[j_mat[f]]++] = (int32_t)f;
+    std::vector<int32_t> fr(cnt.begin(), cnt.end() - 1), fd(dcnt.begin(), dcnt.end() - 1);
+    for (int64_t f = 0; f < n_faces; ++f) {
+      if (!flag[f]) continue;
+      row_faces[fr[i_mat[f]]++] = (int32_t)f;
+      if (flag[f] == 3) is_cut[f] = 1;
+      else down_face[fd[j_mat[f]]++] = (int32_t)f;
+    }
   }
-
-  std::vector<int32_t> indeg(n, 0);
-  for (int32_t c = 0; c < n; ++c)
-    for (int32_t e = cnt[c]; e < cnt[c + 1]; ++e)
-      if (!is_cut[row_faces[e]]) indeg[c]++;
 
   // Kahn's algorithm by levels (longest-path level of each cell).  With
   // max_width > 0 a level takes at most max_width ready cells; the rest wait
@@ -156,6 +153,8 @@ void build_direction(int32_t n, int64_t n_faces, int dim, const int32_t* i_mat, 
   out.in_face.clear();
   out.cut_exit.clear();
   out.cut_face.clear();
+  out.in_src.reserve(cnt[n]);
+  out.in_face.reserve(cnt[n]);
   for (int32_t p = 0; p < n; ++p) {
     int32_t c = out.order[p];
     for (int32_t e = cnt[c]; e < cnt[c + 1]; ++e) {
