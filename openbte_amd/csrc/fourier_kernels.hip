@@ -122,24 +122,22 @@ __global__ void k_cg_init(obte_fourier_plan_t P, const double* bh, const double*
   }
 }
 
-// ec = coarse_inv rc: four aggregates per block, each row summed in a fixed
-// order.
+// ec = coarse_inv rc: one wave per aggregate (four per block), each row
+// summed in a fixed order (lane-strided partial sums, then an xor tree).
 constexpr int kCoarseRows = 4;
 
 __global__ void k_cg_coarse(obte_fourier_plan_t P, double* part, int nb) {
-  __shared__ double red[16];
   const int nc = P.n_coarse;
   const double* rc = part + 4 * nb;
   double* ec = part + 5 * nb;
-  for (int q = 0; q < kCoarseRows; ++q) {
-    const int I = blockIdx.x * kCoarseRows + q;  // uniform over the block
-    if (I >= nc) break;
-    const double* row = P.coarse_inv + (int64_t)I * nc;
-    double s = 0.0;
-    for (int J = threadIdx.x; J < nc; J += blockDim.x) s += row[J] * rc[J];
-    s = wg_sum(s, red);
-    if (threadIdx.x == 0) ec[I] = s;
-  }
+  const int lane = threadIdx.x & 63;
+  const int I = blockIdx.x * kCoarseRows + (threadIdx.x >> 6);  // uniform over the wave
+  if (I >= nc) return;
+  const double* row = P.coarse_inv + (int64_t)I * nc;
+  double s = 0.0;
+  for (int J = lane; J < nc; J += 64) s += row[J] * rc[J];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if (lane == 0) ec[I] = s;
 }
 
 // rc . ec (the coarse part of r.M^-1 r), computed the same way by every block
