@@ -53,13 +53,27 @@ CONFIGS = {
     'plain2k': (10.0, 0.25, 2000, 30, 48, 1.0e4),
     'plain200k': (100.0, 0.25, 200_000, 60, 96, 1.0e5),
 }
+CONFIGS3D = {
+    # name: (tetrahedra, porosity, n_mfp, (n_theta, n_phi), mfp_max nm): a periodic
+    # porous slab (cylindrical pore, diffuse pore wall and slab faces)
+    'tet150k': (150_000, 0.2, 40, (8, 8), 1.0e4),
+}
 HBM_PEAK = 8.0e12
 
 
 def make_case(name, seed=0):
-    from openbte_amd.geometry import pore_cell_2d
-    from openbte_amd.material import synthetic_material_2d
+    from openbte_amd.geometry import pore_cell_2d, pore_slab_3d
+    from openbte_amd.material import synthetic_material_2d, synthetic_material_3d
     from openbte_amd.objects import BoundaryConditions, EffectiveThermalConductivity
+    if name in CONFIGS3D:
+        n, phi, M, (nth, nph), mfp_max = CONFIGS3D[name]
+        L = float(np.sqrt(n * 0.45 ** 3 / (1 - phi) / 4.0) ** (2 / 3) * 4.0 ** (1 / 3))
+        H = L / 4.0
+        geo = pore_slab_3d(L, H, phi, n, seed=seed)
+        mat = synthetic_material_3d(M, nth, nph, 1.0, mfp_max, seed=seed)
+        bcs = BoundaryConditions(periodic={'Periodic_x': 1.0, 'Periodic_y': 0.0}, diffuse='Boundary')
+        eff = EffectiveThermalConductivity(contact='Periodic_x', normalization=-1.0 / H)
+        return geo, mat, bcs, eff
     L, por, n, M, nphi, mfp_max = CONFIGS[name]
     geo = pore_cell_2d(L, por, n, seed=seed)
     mat = synthetic_material_2d(M, nphi, 1.0, mfp_max, seed=seed)
@@ -464,7 +478,7 @@ def main(argv=None):
     ap.add_argument('--steps', type=int, default=20)
     ap.add_argument('--warmup', type=int, default=2)
     ap.add_argument('--config', default='plain200k',
-                    choices=sorted(CONFIGS) + sorted(BATCH_CONFIGS) + sorted(SERVE_CONFIGS)
+                    choices=sorted(CONFIGS) + sorted(CONFIGS3D) + sorted(BATCH_CONFIGS) + sorted(SERVE_CONFIGS)
                     + sorted(INVERSE_CONFIGS))
     ap.add_argument('--no-graphs', action='store_true', help='batched configs: launch without hipGraphs')
     ap.add_argument('--no-window', action='store_true', help='sweeps read every upwind value from HBM')
@@ -495,7 +509,7 @@ def main(argv=None):
     from openbte_amd.fourier import Fourier
 
     geo, mat, bcs, eff = make_case(args.config)
-    n, M, L = geo.n_elems, mat.grid[0], mat.grid[1]
+    n, M, L = geo.n_elems, mat.grid[0], int(np.prod(mat.grid[1:]))
     print(f'# config {args.config}: n={n} M={M} L={L}', file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu:
@@ -606,9 +620,12 @@ def main(argv=None):
         'scaling': 'weak',
         'vs_baseline': None,
         'dtype': 'f64',
-        'data': 'synthetic (seeded jittered-Delaunay pore cell, seeded lognormal Si-like material)',
-        'config': {'workload': f'{args.config}: L={CONFIGS[args.config][0]:g}nm porosity 0.25, n={n} cells, '
-                               f'{M} MFP x {L} directions = {M * L} systems',
+        'data': ('synthetic (seeded jittered-Delaunay pore cell, seeded lognormal Si-like material)'
+                 if geo.dim == 2 else 'synthetic (seeded tetrahedral porous slab, seeded lognormal Si-like material)'),
+        'config': {'workload': (f'{args.config}: L={CONFIGS[args.config][0]:g}nm porosity 0.25, n={n} cells, '
+                                if args.config in CONFIGS else
+                                f'{args.config}: periodic porous slab, n={n} tetrahedra, ')
+                               + f'{M} MFP x {L} directions = {M * L} systems',
                    'mc': cfg.mc, 'block': cfg.block, 'lds_window_positions': dev.window_positions(),
                    'kappa_eff_after_steps': kappa,
                    'extensions_per_step': exts, 'n_cut_faces': dev.n_cut,

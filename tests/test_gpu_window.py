@@ -73,7 +73,8 @@ def test_window_equals_global_2d(mc, block, max_width):
     _same(c, _sums_only(b))
 
 
-def test_window_equals_global_3d():
+@pytest.mark.parametrize('max_width', [192, 0])
+def test_window_equals_global_3d(max_width):
     from openbte_amd.bte import SweepConfig
     from openbte_amd.geometry import pore_slab_3d
     from openbte_amd.material import synthetic_material_3d
@@ -84,9 +85,9 @@ def test_window_equals_global_3d():
     bcs = BoundaryConditions(periodic={'Periodic_x': 1.0, 'Periodic_y': 0.0}, diffuse='Boundary')
     eff = EffectiveThermalConductivity(contact='Periodic_x', normalization=-1.0)
     op = build_host_operator(geo, mat, bcs, eff)
-    sc = build_schedule(op)
+    sc = build_schedule(op, max_width=max_width)  # 0: levels wider than one pass
     t0 = np.linspace(-0.5, 0.5, op.n)
-    print('# window 3d', op.n, flush=True)
+    print('# window 3d', op.n, max_width, flush=True)
     a, ea, cap_a = _iterate(op, sc, SweepConfig(mc=8, block=256, window=True), t0)
     b, eb, cap_b = _iterate(op, sc, SweepConfig(mc=8, block=256, window=False), t0)
     c, ec, _ = _iterate(op, sc, SweepConfig(mc=8, block=256, window=True), t0, keep=False)
