@@ -512,7 +512,10 @@ def main(argv=None):
     n, M, L = geo.n_elems, mat.grid[0], int(np.prod(mat.grid[1:]))
     print(f'# config {args.config}: n={n} M={M} L={L}', file=sys.stderr, flush=True)
     cpu = None
-    if not args.no_cpu:
+    if args.config in CONFIGS3D and not args.no_cpu:
+        # one SuperLU factorisation of a 3D system of this size takes minutes
+        print('# no CPU leg for the 3D configuration (SuperLU fill-in)', file=sys.stderr, flush=True)
+    elif not args.no_cpu:
         # CPU baseline first, before this process touches the GPU (its workers fork)
         from oracle import fourier_oracle as fo
         T_cpu = fo.fourier(geo, mat.thermal_conductivity, bcs, eff).variables['Temperature_Fourier']['data']
