@@ -21,7 +21,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .bte import BTEDevice, SweepConfig
+from .bte import BTEDevice, SweepConfig, default_config
 from .objects import SolverResults
 from .operator import build_host_operator
 
@@ -317,6 +317,6 @@ def BTE_RTA_batch(problems, bte_max_iter: int = 100, bte_rtol: float = 1e-4,
         f = pr.get('fourier')
         t0.append(np.zeros(geo.n_elems) if f is None else
                   np.asarray(f.variables['Temperature_Fourier']['data'], np.float64))
-    batch = BTEBatch(ops, cfg=config)
+    batch = BTEBatch(ops, cfg=config or default_config(ops))
     batch.solve(t0, bte_max_iter=bte_max_iter, bte_rtol=bte_rtol)
     return [batch.results(p) for p in range(len(problems))]

@@ -21,7 +21,7 @@ from __future__ import annotations
 
 import time
 from dataclasses import dataclass, field
-from typing import Optional
+from typing import Optional, Sequence
 
 import numpy as np
 import torch
@@ -356,6 +356,22 @@ class BTEDevice:
         return DeltaT, J, K, S
 
 
+def default_config(ops: Sequence[HostOperator]) -> SweepConfig:
+    """Launch shape of ``BTE_RTA`` / ``BTE_RTA_batch`` when none is given:
+    chunks of 8 MFP bins, or of 4 when chunks of 8 leave fewer than two tasks
+    (direction x chunk) per compute unit for some problem (measured: 30 bins
+    x 48 directions +19%, 40 x 64 +13% at mc 4; 60 x 96 is faster at mc 8).
+    The batched and single forms take the same rule, so their results stay
+    equal bit for bit."""
+    cfg = SweepConfig()
+    cus = 256
+    if torch.cuda.is_available():  # (without a GPU, BTEDevice raises)
+        cus = getattr(torch.cuda.get_device_properties(torch.cuda.current_device()), 'multi_processor_count', cus)
+    if min(op.n_dirs * ((op.n_mfp + 7) // 8) for op in ops) < 2 * cus:
+        cfg.mc = 4
+    return cfg
+
+
 def BTE_RTA(geo, mat, bcs, fourier=None, effective_thermal_conductivity=None, heat_source=None,
             direct: bool = True, bte_max_iter: int = 100, bte_rtol: float = 1e-4,
             config: Optional[SweepConfig] = None, verbose: bool = False,
@@ -370,7 +386,7 @@ def BTE_RTA(geo, mat, bcs, fourier=None, effective_thermal_conductivity=None, he
         dt0 = np.zeros(geo.n_elems)
     else:
         dt0 = np.asarray(fourier.variables['Temperature_Fourier']['data'], dtype=np.float64)
-    solver = BTEDevice(op, cfg=config)
+    solver = BTEDevice(op, cfg=config or default_config([op]))
     n_iter = solver.solve(dt0, bte_max_iter=bte_max_iter, bte_rtol=bte_rtol, verbose=verbose, stats=stats)
     DeltaT, J, K, S = solver.results()
     if op.contact_kind == 0:
