@@ -40,14 +40,6 @@ __device__ __forceinline__ double dotv(const double* a, const double* b, int dim
   return g;
 }
 
-// Sum of v over the MC consecutive lanes of one cell (fixed xor-tree order).
-template <int MC>
-__device__ __forceinline__ double lane_group_sum(double v) {
-#pragma unroll
-  for (int o = MC / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, MC);
-  return v;
-}
-
 // ---------------------------------------------------------------------------
 // Problem sources.  Every kernel is written once against an accessor that
 // yields the plan / Krylov state / IO pointers of problem `pid`:
@@ -1191,7 +1183,7 @@ __global__ void k_gcr_reset(S src, int use_dt) {
 // order,
 //   part_t[l][p] = sum_m t_ml T_ml,
 //   part_j[l][p] = sum_m sigma_ml T_ml - DeltaT_old sum_m sigma_ml,
-// summed chunk by chunk in order, each chunk's sum formed by chunk_partial.
+// summed chunk by chunk in order, each chunk's sum formed by chunk_sums.
 // k_modesum reads T from the solution workspace; k_modesum_part sums the
 // per-chunk sums a windowed final sweep wrote to part_chunk.  Both give the
 // same bits.  Grid: y = direction, z = problem.
