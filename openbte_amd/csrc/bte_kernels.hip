@@ -1315,18 +1315,33 @@ __global__ void k_reduce_cells(S src) {
   if (threadIdx.x == 0) P.scratch[blockIdx.x] = red[0];
 }
 
+// TB_s = sum over the tasks of part_tb[task][s]: 16 wall sides per block,
+// 16 lanes per side each summing every 16th task, the 16 partial sums then
+// added in lane order (a fixed order).
+constexpr int kWallsPerBlock = 16;
+
 template <class S>
-__global__ void k_reduce_walls(S src) {
+__global__ __launch_bounds__(256) void k_reduce_walls(S src) {
+  __shared__ double part[16][kWallsPerBlock + 1];
   const int pid = blockIdx.y;
   if (!src.live(pid)) return;
   const obte_bte_plan_t& P = src.plan(pid);
   const obte_reduce_io_t& R = src.red(pid);
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= P.n_wall) return;
+  if ((int64_t)blockIdx.x * kWallsPerBlock >= P.n_wall) return;  // uniform
+  const int w = threadIdx.x % kWallsPerBlock;
+  const int g = threadIdx.x / kWallsPerBlock;
+  const int s = blockIdx.x * kWallsPerBlock + w;
   const int ntask = P.n_dirs * P.n_chunks;
   double v = 0.0;
-  for (int t = 0; t < ntask; ++t) v += P.part_tb[(int64_t)t * P.n_wall + s];
-  R.tb_new[s] = v;
+  if (s < P.n_wall)
+    for (int t = g; t < ntask; t += 16) v += P.part_tb[(int64_t)t * P.n_wall + s];
+  part[g][w] = v;
+  __syncthreads();
+  if (g == 0 && s < P.n_wall) {
+    double acc = 0.0;
+    for (int k = 0; k < 16; ++k) acc += part[k][w];
+    R.tb_new[s] = acc;
+  }
 }
 
 // norm2 = sum of the block partials in order; one block per problem
@@ -1632,7 +1647,8 @@ int obte_bte_reduce(const obte_bte_plan_t* plan, double* delta_t_new, double* j_
   dispatch<ModesumL>(plan->mc, 2, src, parts, plan->n, plan->n_dirs, 1, st);
   hipLaunchKernelGGL((k_reduce_cells<One>), dim3(blocks_for(plan->n, 256), 1), dim3(256), 0, st, src);
   if (plan->n_wall > 0)
-    hipLaunchKernelGGL((k_reduce_walls<One>), dim3(blocks_for(plan->n_wall, 256), 1), dim3(256), 0, st, src);
+    hipLaunchKernelGGL((k_reduce_walls<One>), dim3(blocks_for(plan->n_wall, kWallsPerBlock), 1), dim3(256), 0, st,
+                       src);
   hipLaunchKernelGGL((k_reduce_final<One>), dim3(1), dim3(1024), 0, st, src);
   return launch_status("obte_bte_reduce: launch failed");
 }
@@ -1710,8 +1726,8 @@ int obte_batch_reduce(const obte_bte_batch_t* b, obte_stream_t stream) {
   hipLaunchKernelGGL((k_reduce_cells<Many>), dim3(blocks_for(b->max_cells, 256), b->n_problems), dim3(256), 0, st,
                      src);
   if (b->max_walls > 0)
-    hipLaunchKernelGGL((k_reduce_walls<Many>), dim3(blocks_for(b->max_walls, 256), b->n_problems), dim3(256), 0, st,
-                       src);
+    hipLaunchKernelGGL((k_reduce_walls<Many>), dim3(blocks_for(b->max_walls, kWallsPerBlock), b->n_problems),
+                       dim3(256), 0, st, src);
   hipLaunchKernelGGL((k_reduce_final<Many>), dim3(b->n_problems), dim3(1024), 0, st, src);
   hipLaunchKernelGGL(k_commit, dim3(blocks_for(b->max_cells > b->max_walls ? b->max_cells : b->max_walls, 256),
                                     b->n_problems), dim3(256), 0, st, src);
