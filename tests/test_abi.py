@@ -4,6 +4,8 @@ import ctypes
 import os
 import re
 
+import pytest
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
@@ -54,3 +56,35 @@ def test_sweep_window_rule():
     assert lib.obte_sweep_window(1, 100, 32) == 0       # wide chunks keep the global path
     assert lib.obte_sweep_window(0, 100, 8) == 0        # disabled
     assert lib.obte_sweep_window(1, 0, 8) == 0          # level sizes unknown
+
+
+def test_struct_offsets_match_c_compiler(tmp_path):
+    """The ctypes mirrors have the C compiler's field offsets and struct sizes
+    (types, not only names and order)."""
+    import shutil
+    import subprocess
+    from openbte_amd import _lib
+    cc = shutil.which('gcc') or shutil.which('cc')
+    if cc is None:
+        pytest.skip('no C compiler')
+    structs = (('obte_bte_plan_t', _lib.BtePlan), ('obte_gcr_t', _lib.Gcr), ('obte_fourier_plan_t', _lib.FourierPlan),
+               ('obte_sweep_io_t', _lib.SweepIO), ('obte_reduce_io_t', _lib.ReduceIO),
+               ('obte_bte_batch_t', _lib.Batch))
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "openbte_amd.h"', 'int main(void) {']
+    for cname, py in structs:
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f in py._fields_:
+            lines.append(f'  printf("{cname} {f[0]} %zu\\n", offsetof({cname}, {f[0]}));')
+    lines += ['  return 0;', '}']
+    src = tmp_path / 'offsets.c'
+    src.write_text('\n'.join(lines) + '\n')
+    exe = tmp_path / 'offsets'
+    subprocess.run([cc, '-std=c11', '-I', os.path.join(ROOT, 'include'), str(src), '-o', str(exe)], check=True)
+    got = {}
+    for line in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.splitlines():
+        s, f, v = line.split()
+        got[(s, f)] = int(v)
+    for cname, py in structs:
+        assert got[(cname, 'size')] == ctypes.sizeof(py), cname
+        for f in py._fields_:
+            assert got[(cname, f[0])] == getattr(py, f[0]).offset, (cname, f[0])
