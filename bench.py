@@ -504,7 +504,7 @@ def main(argv=None):
         return run_inverse(args)
 
     import torch
-    from openbte_amd.bte import BTEDevice, SweepConfig
+    from openbte_amd.bte import BTEDevice, default_config
     from openbte_amd.operator import build_host_operator
     from openbte_amd.fourier import Fourier
 
@@ -535,12 +535,13 @@ def main(argv=None):
                         'cg_iterations': [c[0] for c in f.aux['cg']],
                         'kappa_diffusive': f.aux['kappa_diffusive']}
         print(f'# fourier done ({f.aux["n_iter"]} iterations, {tf:.2f}s)', file=sys.stderr, flush=True)
-    cfg = SweepConfig(window=not args.no_window)
+    op = build_host_operator(geo, mat, bcs, eff)
+    cfg = default_config([op])  # the shape BTE_RTA takes for this problem (mc 8 at 200k cells)
+    cfg.window = not args.no_window
     if args.mc:
         cfg.mc = args.mc
     if args.block:
         cfg.block = args.block
-    op = build_host_operator(geo, mat, bcs, eff)
     dev = BTEDevice(op, cfg=cfg)
     torch.cuda.synchronize()
     t_setup = time.perf_counter() - t_setup
