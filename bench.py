@@ -482,6 +482,7 @@ def main(argv=None):
                     + sorted(INVERSE_CONFIGS))
     ap.add_argument('--no-graphs', action='store_true', help='batched configs: launch without hipGraphs')
     ap.add_argument('--no-window', action='store_true', help='sweeps read every upwind value from HBM')
+    ap.add_argument('--kmax', type=int, default=0, help='recycled interface basis capacity (vectors)')
     ap.add_argument('--mc', type=int, default=None)
     ap.add_argument('--block', type=int, default=None)
     ap.add_argument('--no-cpu', action='store_true')
@@ -542,6 +543,8 @@ def main(argv=None):
         cfg.mc = args.mc
     if args.block:
         cfg.block = args.block
+    if args.kmax:
+        cfg.kmax = args.kmax
     dev = BTEDevice(op, cfg=cfg)
     torch.cuda.synchronize()
     t_setup = time.perf_counter() - t_setup

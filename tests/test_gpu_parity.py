@@ -103,6 +103,23 @@ def test_bte_matches_oracle_small(small_case, mc, block):
     assert np.abs(out.variables['Flux_BTE']['data'] - Jr).max() < 1e-7 * np.abs(Jr).max()
 
 
+def test_bte_small_basis_matches_oracle(small_case):
+    """A recycled interface basis of 8 vectors (restarted when full) still
+    solves every system exactly."""
+    from oracle import bte_oracle as bo, fourier_oracle as fo
+    from openbte_amd.bte import BTE_RTA, SweepConfig
+    geo, mat, bcs, eff = small_case
+    f = fo.fourier(geo, mat.thermal_conductivity, bcs, eff)
+    ref = bo.bte_rta(geo, mat, bcs, fourier=f, effective_thermal_conductivity=eff, bte_max_iter=10)
+    out = BTE_RTA(geo, mat, bcs, fourier=f, effective_thermal_conductivity=eff, bte_max_iter=10,
+                  config=SweepConfig(kmax=8))
+    assert out.aux['n_iter'] == ref.aux['n_iter']
+    rel = abs(out.kappa_eff / ref.kappa_eff - 1)
+    dT = np.abs(out.variables['Temperature_BTE']['data'] - ref.variables['Temperature_BTE']['data']).max()
+    print('basis kmax 8: kappa rel', rel, 'dT', dT)
+    assert rel < KAPPA_RTOL and dT < T_ATOL
+
+
 def test_bte_closed_form_quads():
     """Uniform quads (kin = 2 or 3 inflow slots): exact closed-form kappa after one iteration."""
     from oracle import fourier_oracle as fo
