@@ -21,7 +21,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .bte import BTEDevice, SweepConfig, default_config
+from .bte import BTEDevice, SweepConfig, default_config, warn_unconverged
 from .objects import SolverResults
 from .operator import build_host_operator
 
@@ -216,6 +216,8 @@ class BTEBatch:
         while ext < max_extend and int(self.n_active.sum().item()) > 0:
             self._round()
             ext += 1
+        if ext >= max_extend and int(self.n_active.sum().item()) > 0:
+            warn_unconverged(int(self.n_active.sum().item()), max_extend)
         self._tail()
         return ext
 
@@ -265,6 +267,8 @@ class BTEBatch:
             while ext + extra < max_extend and int(self.n_active.sum().item()) > 0:
                 self._round()
                 extra += 1
+            if ext + extra >= max_extend and int(self.n_active.sum().item()) > 0:
+                warn_unconverged(int(self.n_active.sum().item()), max_extend)
         self._tail_graph.replay()
         self._next_rounds = max(1, ext + extra)
         return ext + extra

@@ -20,6 +20,7 @@ the device buffers (torch tensors) and drives the loop.
 from __future__ import annotations
 
 import time
+import warnings
 from dataclasses import dataclass, field
 from typing import Optional, Sequence
 
@@ -42,6 +43,18 @@ class SweepConfig:
     max_extend: int = 64     # cap on basis extensions per outer iteration
     basis_bytes: float = 24e9  # HBM budget of the recycled basis (u and c)
     window: bool = True      # keep recent levels of a sweep in LDS where they fit
+
+
+class InterfaceNotConverged(UserWarning):
+    """An outer iteration reached ``SweepConfig.max_extend`` interface Krylov
+    extensions with systems still above the interface tolerance: its systems
+    are not solved to direct-solver accuracy."""
+
+
+def warn_unconverged(n_tasks: int, max_extend: int) -> None:
+    warnings.warn(f'{n_tasks} (direction, chunk) tasks still above the interface tolerance after '
+                  f'max_extend={max_extend} Krylov extensions in this outer iteration; raise '
+                  f'SweepConfig.max_extend', InterfaceNotConverged, stacklevel=3)
 
 
 @dataclass
@@ -311,6 +324,8 @@ class BTEDevice:
                 self._sweep(1, g['w'].data_ptr(), g['active'].data_ptr(), timer, g['lane_active'].data_ptr())
                 _lib.call('obte_gcr_end_extend', P, G, st)
                 ext += 1
+            if ext >= max_extend and self._n_active() > 0:
+                warn_unconverged(self._n_active(), max_extend)
         self._sweep(2, g['y'].data_ptr(), 0, timer)
         nxt = 1 - self.cur
         b = self.buf

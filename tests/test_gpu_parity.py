@@ -120,6 +120,17 @@ def test_bte_small_basis_matches_oracle(small_case):
     assert rel < KAPPA_RTOL and dT < T_ATOL
 
 
+def test_interface_not_converged_warns(small_case):
+    """An outer iteration capped below the extensions it needs says so."""
+    from openbte_amd.bte import BTEDevice, InterfaceNotConverged, SweepConfig
+    from openbte_amd.operator import build_host_operator
+    geo, mat, bcs, eff = small_case
+    dev = BTEDevice(build_host_operator(geo, mat, bcs, eff), cfg=SweepConfig(max_extend=1))
+    dev.reset(np.zeros(geo.n_elems))
+    with pytest.warns(InterfaceNotConverged):
+        dev.outer_iteration()
+
+
 def test_bte_closed_form_quads():
     """Uniform quads (kin = 2 or 3 inflow slots): exact closed-form kappa after one iteration."""
     from oracle import fourier_oracle as fo
