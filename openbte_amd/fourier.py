@@ -18,6 +18,7 @@ caller's cell order.
 from __future__ import annotations
 
 import ctypes as C
+import warnings
 from typing import NamedTuple, Optional
 
 import numpy as np
@@ -318,6 +319,9 @@ class FourierDevice:
         _lib.call('obte_fourier_solve', self.plan, b['B'].data_ptr(), b['C'].data_ptr(), b['T'].data_ptr(),
                   b['ws'].data_ptr(), b['scal'].data_ptr(), rtol, maxit, 16, C.byref(it), C.byref(rr), self.stream)
         self.cg_iters.append((it.value, rr.value))
+        if rr.value > rtol:
+            warnings.warn(f'Fourier CG stopped at {it.value} iterations with relative residual {rr.value:.2e} '
+                          f'> {rtol:g}', RuntimeWarning, stacklevel=2)
 
     def gradient(self):
         _lib.call('obte_fourier_gradient', self.plan, self.b['T'].data_ptr(), self.b['grad'].data_ptr(), self.stream)
