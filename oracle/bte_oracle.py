@@ -5,8 +5,11 @@ mathematics of openbte/bte.py:19-271.  Only the tests, the smoke check and the
 CPU-baseline leg of the benchmark may import it; the product path never does.
 
 Pinning (how each function is checked, see tests/test_oracle_*.py):
-  assemble          -- dense brute force on tiny meshes, face-closure and
+  assemble          -- dense brute force on tiny meshes (one with cells that
+                       have two diffuse walls), face-closure and
                        zero-net-flux invariants, closed form on uniform quads
+  rhs               -- dense fixed point on the two-wall mesh (the wall
+                       source accumulates over a cell's walls)
   solve_system      -- dense numpy.linalg.solve of the same matrix
   bte_rta           -- closed form kappa on uniform quads (exact upwind
                        solution), dense fixed point of the coupled system,

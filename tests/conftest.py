@@ -1,6 +1,7 @@
 import os
 import sys
 
+import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -31,3 +32,39 @@ def small_case():
     bcs = BoundaryConditions(periodic={'Periodic_x': 1.0, 'Periodic_y': 0.0}, diffuse='Boundary')
     eff = EffectiveThermalConductivity(contact='Periodic_x', normalization=-1.0)
     return geo, mat, bcs, eff
+
+
+def two_wall_quads(L: float = 10.0, N: int = 10):
+    """Uniform N x N periodic quad cell with a pore made of removed cells: a
+    diagonal staircase (cells beside two of its steps have walls on two
+    adjacent sides) and two columns around a one-cell-wide slot (walls on two
+    opposite sides).  Returns the mesh and the number of cells with two walls."""
+    from openbte_amd.geometry import _REGIONS, _edge_region
+    from openbte_amd.mesh import build_mesh
+    xs = np.linspace(-L / 2, L / 2, N + 1)
+    X, Y = np.meshgrid(xs, xs, indexing='ij')
+    pts = np.stack([X.ravel(), Y.ravel()], 1)
+    idx = np.arange((N + 1) ** 2).reshape(N + 1, N + 1)
+    q = np.stack([idx[:-1, :-1], idx[1:, :-1], idx[1:, 1:], idx[:-1, 1:]], axis=-1)  # [i][j][4]
+    removed = np.zeros((N, N), bool)
+    for k in range(2, 6):
+        removed[k, k] = True
+    removed[6, 2:6] = True
+    removed[8, 2:6] = True
+    geo = build_mesh(pts, q[~removed].reshape(-1, 4), _edge_region(L, L), _REGIONS,
+                     periodic={'Periodic_x': np.array([L, 0.0]), 'Periodic_y': np.array([0.0, L])})
+    walls = np.bincount(geo.side_elem_map[np.asarray(geo.side_physical_regions['Boundary']), 0],
+                        minlength=geo.n_elems)
+    return geo, int((walls >= 2).sum())
+
+
+@pytest.fixture(scope="session")
+def two_wall_case():
+    """two_wall_quads with 3 MFP bins x 8 directions and diffuse pore walls."""
+    from openbte_amd.material import synthetic_material_2d
+    from openbte_amd.objects import BoundaryConditions, EffectiveThermalConductivity
+    geo, n2 = two_wall_quads()
+    mat = synthetic_material_2d(3, 8, 1.0, 1.0e3)
+    bcs = BoundaryConditions(periodic={'Periodic_x': 1.0, 'Periodic_y': 0.0}, diffuse='Boundary')
+    eff = EffectiveThermalConductivity(contact='Periodic_x', normalization=-1.0)
+    return geo, mat, bcs, eff, n2

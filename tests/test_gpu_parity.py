@@ -103,6 +103,28 @@ def test_bte_matches_oracle_small(small_case, mc, block):
     assert np.abs(out.variables['Flux_BTE']['data'] - Jr).max() < 1e-7 * np.abs(Jr).max()
 
 
+@pytest.mark.parametrize("mc,block", [(4, 64), (8, 256)])
+def test_bte_two_wall_cells_match_oracle(two_wall_case, mc, block):
+    """Quads with cells that have two diffuse walls (the source adds both
+    walls' re-emission, openbte/bte.py:199-214)."""
+    from oracle import bte_oracle as bo, fourier_oracle as fo
+    from openbte_amd.bte import BTE_RTA, SweepConfig
+    geo, mat, bcs, eff, n2 = two_wall_case
+    assert n2 >= 8
+    f = fo.fourier(geo, mat.thermal_conductivity, bcs, eff)
+    ref = bo.bte_rta(geo, mat, bcs, fourier=f, effective_thermal_conductivity=eff, bte_max_iter=6)
+    out = BTE_RTA(geo, mat, bcs, fourier=f, effective_thermal_conductivity=eff, bte_max_iter=6,
+                  config=SweepConfig(mc=mc, block=block))
+    assert out.aux['n_iter'] == ref.aux['n_iter']
+    rel = abs(out.kappa_eff / ref.kappa_eff - 1)
+    dT = np.abs(out.variables['Temperature_BTE']['data'] - ref.variables['Temperature_BTE']['data']).max()
+    print('two-wall kappa', out.kappa_eff, ref.kappa_eff, 'rel', rel, 'dT', dT)
+    assert rel < KAPPA_RTOL
+    assert dT < T_ATOL
+    Jr = ref.variables['Flux_BTE']['data']
+    assert np.abs(out.variables['Flux_BTE']['data'] - Jr).max() < 1e-7 * np.abs(Jr).max()
+
+
 def test_bte_small_basis_matches_oracle(small_case):
     """A recycled interface basis of 8 vectors (restarted when full) still
     solves every system exactly."""

@@ -76,8 +76,12 @@ def _independent_matrix(geo, mat, bcs, l, m):
     return A
 
 
-def test_system_matrix_brute_force():
-    geo = pore_cell_2d(10.0, 0.25, 150, seed=7)
+@pytest.mark.parametrize("mesh", ["pore", "two_wall"])
+def test_system_matrix_brute_force(mesh, two_wall_case):
+    if mesh == "pore":
+        geo = pore_cell_2d(10.0, 0.25, 150, seed=7)
+    else:  # cells with two diffuse walls: their wall terms add on the diagonal
+        geo = two_wall_case[0]
     mat = synthetic_material_2d(3, 8, 1.0, 1.0e4)
     ops = bo.assemble(geo, mat, BCS, EFF)
     rng = np.random.default_rng(0)
@@ -139,6 +143,27 @@ def test_dense_fixed_point():
     d = got - DT
     assert np.abs(d - d.mean()).max() < 1e-7
     # kappa from the dense solution (constants cancel in the contact formula)
+    K = 0.0
+    for l in range(ops.n_dirs):
+        for m in range(ops.n_mfp):
+            acc = bo._empty_sums(ops)
+            bo.accumulate(ops, l, m, Tml[l, m], DT, acc)
+            K += acc.K[m, l]
+    assert abs(r.kappa_eff / K - 1) < 1e-6
+
+
+def test_dense_fixed_point_two_wall_cells(two_wall_case):
+    """Cells with two diffuse walls take the wall re-emission of both sides
+    (the accumulation of openbte/bte.py:199-214): the dense coupled system adds
+    each wall side's term separately, so a scatter that kept only one of a
+    cell's walls fails the comparison."""
+    geo, mat, bcs, eff, n2 = two_wall_case
+    assert n2 >= 8
+    f = fo.fourier(geo, mat.thermal_conductivity, bcs, eff)
+    r = bo.bte_rta(geo, mat, bcs, fourier=f, effective_thermal_conductivity=eff, bte_max_iter=3000, bte_rtol=1e-15)
+    ops, DT, Tml = _dense_fixed_point(geo, mat, bcs, eff)
+    d = r.variables['Temperature_BTE']['data'] - DT
+    assert np.abs(d - d.mean()).max() < 1e-7
     K = 0.0
     for l in range(ops.n_dirs):
         for m in range(ops.n_mfp):
