@@ -9,7 +9,9 @@ Pinning (how each function is checked, see tests/test_oracle_*.py):
                        have two diffuse walls), face-closure and
                        zero-net-flux invariants, closed form on uniform quads
   rhs               -- dense fixed point on the two-wall mesh (the wall
-                       source accumulates over a cell's walls)
+                       source accumulates over a cell's walls); isothermal
+                       source against a side-by-side assembly, uniform
+                       temperature between two contacts at the same T0
   solve_system      -- dense numpy.linalg.solve of the same matrix
   bte_rta           -- closed form kappa on uniform quads (exact upwind
                        solution), dense fixed point of the coupled system,

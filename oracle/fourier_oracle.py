@@ -7,7 +7,9 @@ check and the benchmark's CPU-baseline leg import it.
 Pinning (tests/test_oracle_fourier.py): exact linear temperature on pore-free
 periodic meshes; kappa_F = kappa_xx on orthogonal (uniform quad) meshes;
 the Laplacian solve equals scipy.sparse.linalg.spsolve of the same graph
-Laplacian up to the (max+min)/2 shift; LSQ gradient exact for linear fields.
+Laplacian up to the (max+min)/2 shift; LSQ gradient exact for linear fields;
+isothermal contacts: the exact 1D series solution of a strip between two
+contacts of finite conductance.
 Parity unpinned: the mixed-boundary source is assigned, not accumulated
 (openbte/fourier.py:86), so a cell with two mixed sides keeps the last side's
 term; no check fixes that quirk, it follows the reference line by line.

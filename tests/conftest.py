@@ -68,3 +68,40 @@ def two_wall_case():
     bcs = BoundaryConditions(periodic={'Periodic_x': 1.0, 'Periodic_y': 0.0}, diffuse='Boundary')
     eff = EffectiveThermalConductivity(contact='Periodic_x', normalization=-1.0)
     return geo, mat, bcs, eff, n2
+
+
+def isothermal_quads(L: float = 10.0, Nx: int = 10, Ny: int = 6, pore: bool = False):
+    """Uniform Nx x Ny quad cell, periodic in y, with the x = -L/2 and x = +L/2
+    sides in regions 'Left' and 'Right' (isothermal contacts); with ``pore``
+    a staircase of removed cells adds diffuse walls ('Boundary')."""
+    from openbte_amd.geometry import _edge_region
+    from openbte_amd.mesh import build_mesh
+    Ly = L * Ny / Nx
+    xs = np.linspace(-L / 2, L / 2, Nx + 1)
+    ys = np.linspace(-Ly / 2, Ly / 2, Ny + 1)
+    X, Y = np.meshgrid(xs, ys, indexing='ij')
+    pts = np.stack([X.ravel(), Y.ravel()], 1)
+    idx = np.arange((Nx + 1) * (Ny + 1)).reshape(Nx + 1, Ny + 1)
+    q = np.stack([idx[:-1, :-1], idx[1:, :-1], idx[1:, 1:], idx[:-1, 1:]], axis=-1)
+    removed = np.zeros((Nx, Ny), bool)
+    if pore:
+        for k in range(2, min(Nx, Ny) - 1):
+            removed[k, k - 1] = True
+    return build_mesh(pts, q[~removed].reshape(-1, 4), _edge_region(L, Ly),
+                      ['Left', 'Right', 'Periodic_y_a', 'Periodic_y_b', 'Boundary'],
+                      periodic={'Periodic_y': np.array([0.0, Ly])})
+
+
+@pytest.fixture(scope="session")
+def isothermal_case():
+    """isothermal_quads with a pore: contacts at +0.5 / -0.5, kappa at 'Left'."""
+    from openbte_amd.material import synthetic_material_2d
+    from openbte_amd.objects import BoundaryConditions, EffectiveThermalConductivity
+    L, Nx, Ny = 10.0, 12, 8
+    geo = isothermal_quads(L, Nx, Ny, pore=True)
+    mat = synthetic_material_2d(4, 8, 1.0, 1.0e3)
+    bcs = BoundaryConditions(periodic={'Periodic_y': 0.0}, diffuse='Boundary',
+                             mixed={'Left': {'value': 0.5, 'boundary_conductance': 100.0},
+                                    'Right': {'value': -0.5, 'boundary_conductance': 100.0}})
+    eff = EffectiveThermalConductivity(contact='Left', normalization=-Nx / Ny)
+    return geo, mat, bcs, eff

@@ -38,6 +38,23 @@ def test_fourier_matches_oracle(small_case):
     assert dJ < 1e-6 * np.abs(ref.variables['Flux_Fourier']['data']).max()
 
 
+def test_fourier_isothermal_contacts_match_oracle(isothermal_case):
+    """Mixed (isothermal, finite conductance) contacts: a nonsingular system
+    with no pinned cell and no centring."""
+    from oracle import fourier_oracle as fo
+    from openbte_amd.fourier import Fourier
+    geo, mat, bcs, eff = isothermal_case
+    ref = fo.fourier(geo, mat.thermal_conductivity, bcs, eff)
+    out = Fourier(geo, mat.thermal_conductivity, bcs, eff, verbose=False)
+    assert out.aux['n_iter'] == ref.aux['n_iter']
+    dT = np.abs(out.variables['Temperature_Fourier']['data'] - ref.variables['Temperature_Fourier']['data']).max()
+    print('isothermal fourier dT', dT, out.aux['kappa_diffusive'], ref.aux['kappa_diffusive'])
+    assert dT < T_ATOL
+    assert abs(out.aux['kappa_diffusive'] / ref.aux['kappa_diffusive'] - 1) < 1e-9
+    dJ = np.abs(out.variables['Flux_Fourier']['data'] - ref.variables['Flux_Fourier']['data']).max()
+    assert dJ < 1e-6 * np.abs(ref.variables['Flux_Fourier']['data']).max()
+
+
 def test_fourier_multi_launch_path_matches_oracle():
     """More cells than one workgroup's solver takes: the multi-launch CG."""
     from oracle import fourier_oracle as fo
@@ -121,6 +138,28 @@ def test_bte_two_wall_cells_match_oracle(two_wall_case, mc, block):
     print('two-wall kappa', out.kappa_eff, ref.kappa_eff, 'rel', rel, 'dT', dT)
     assert rel < KAPPA_RTOL
     assert dT < T_ATOL
+    Jr = ref.variables['Flux_BTE']['data']
+    assert np.abs(out.variables['Flux_BTE']['data'] - Jr).max() < 1e-7 * np.abs(Jr).max()
+
+
+@pytest.mark.parametrize("mc,block", [(4, 64), (8, 256)])
+def test_bte_isothermal_contacts_match_oracle(isothermal_case, mc, block):
+    """Isothermal contacts beside diffuse walls, kappa at the 'Left' contact
+    (openbte/bte.py:90-96 source, :235-243 contact formula)."""
+    from oracle import bte_oracle as bo, fourier_oracle as fo
+    from openbte_amd.bte import BTE_RTA, SweepConfig
+    geo, mat, bcs, eff = isothermal_case
+    f = fo.fourier(geo, mat.thermal_conductivity, bcs, eff)
+    ref = bo.bte_rta(geo, mat, bcs, fourier=f, effective_thermal_conductivity=eff, bte_max_iter=6)
+    out = BTE_RTA(geo, mat, bcs, fourier=f, effective_thermal_conductivity=eff, bte_max_iter=6,
+                  config=SweepConfig(mc=mc, block=block))
+    assert out.aux['n_iter'] == ref.aux['n_iter']
+    rel = abs(out.kappa_eff / ref.kappa_eff - 1)
+    dT = np.abs(out.variables['Temperature_BTE']['data'] - ref.variables['Temperature_BTE']['data']).max()
+    print('isothermal kappa', out.kappa_eff, ref.kappa_eff, 'rel', rel, 'dT', dT)
+    assert rel < KAPPA_RTOL
+    assert dT < T_ATOL
+    np.testing.assert_allclose(out.aux['suppression'], ref.aux['suppression'], rtol=1e-8, atol=1e-12)
     Jr = ref.variables['Flux_BTE']['data']
     assert np.abs(out.variables['Flux_BTE']['data'] - Jr).max() < 1e-7 * np.abs(Jr).max()
 

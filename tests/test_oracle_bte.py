@@ -68,7 +68,10 @@ def _independent_matrix(geo, mat, bcs, l, m):
                 A[row, row] += lam * g
             else:
                 A[row, nb] += lam * g
-    for sd in geo.side_physical_regions.get(bcs.diffuse, []):
+    walls = list(geo.side_physical_regions.get(bcs.diffuse, []))
+    for key in bcs.mixed:
+        walls += list(geo.side_physical_regions[key])
+    for sd in walls:
         a = sem[sd, 0]
         g = (s @ geo.normal_areas[sd]) / geo.elem_volumes[a]
         if g > 0:
@@ -76,18 +79,39 @@ def _independent_matrix(geo, mat, bcs, l, m):
     return A
 
 
-@pytest.mark.parametrize("mesh", ["pore", "two_wall"])
-def test_system_matrix_brute_force(mesh, two_wall_case):
+def _independent_isothermal_source(geo, mat, bcs, l):
+    """Inflow through an isothermal side brings the wall temperature:
+    row c gets -(s.nA/V) T_wall for every inflow (s.n < 0) side of the contact."""
+    s = mat.vmfp[l, :geo.dim]
+    out = np.zeros(geo.n_elems)
+    for key, contact in bcs.mixed.items():
+        for sd in geo.side_physical_regions[key]:
+            a = geo.side_elem_map[sd, 0]
+            g = (s @ geo.normal_areas[sd]) / geo.elem_volumes[a]
+            if g < 0:
+                out[a] += -g * contact['value']
+    return out
+
+
+@pytest.mark.parametrize("mesh", ["pore", "two_wall", "isothermal"])
+def test_system_matrix_brute_force(mesh, two_wall_case, isothermal_case):
+    bcs, eff = BCS, EFF
     if mesh == "pore":
         geo = pore_cell_2d(10.0, 0.25, 150, seed=7)
-    else:  # cells with two diffuse walls: their wall terms add on the diagonal
+    elif mesh == "two_wall":  # cells with two diffuse walls: their wall terms add on the diagonal
         geo = two_wall_case[0]
+    else:  # isothermal contacts beside diffuse walls
+        geo, _, bcs, eff = isothermal_case
     mat = synthetic_material_2d(3, 8, 1.0, 1.0e4)
-    ops = bo.assemble(geo, mat, BCS, EFF)
+    ops = bo.assemble(geo, mat, bcs, eff)
     rng = np.random.default_rng(0)
+    for l in range(8):
+        if bcs.mixed:
+            np.testing.assert_allclose(ops.perturbation[l], _independent_isothermal_source(geo, mat, bcs, l),
+                                       rtol=1e-13, atol=1e-15)
     for l in (0, 3, 5):
         for m in range(3):
-            A = _independent_matrix(geo, mat, BCS, l, m)
+            A = _independent_matrix(geo, mat, bcs, l, m)
             # entries with s.n ~ 0 may land on either side of the split: absolute
             # tolerance relative to the matrix scale
             np.testing.assert_allclose(bo.system_matrix(ops, l, m).toarray(), A, rtol=1e-13,
@@ -171,6 +195,26 @@ def test_dense_fixed_point_two_wall_cells(two_wall_case):
             bo.accumulate(ops, l, m, Tml[l, m], DT, acc)
             K += acc.K[m, l]
     assert abs(r.kappa_eff / K - 1) < 1e-6
+
+
+def test_isothermal_uniform_temperature():
+    """Both contacts at T0, no diffuse walls, no jump: every mode and the
+    lattice temperature equal T0 exactly (pins the sign and size of the
+    isothermal source against the outflow diagonal), and the contact kappa is
+    T0 sum sigma.n = 0 over a symmetric set of directions."""
+    from tests.conftest import isothermal_quads
+    geo = isothermal_quads(10.0, 8, 4)
+    mat = synthetic_material_2d(3, 8, 1.0, 1.0e3)
+    T0 = 0.37
+    bcs = BoundaryConditions(periodic={'Periodic_y': 0.0},
+                             mixed={'Left': {'value': T0, 'boundary_conductance': 1.0},
+                                    'Right': {'value': T0, 'boundary_conductance': 1.0}})
+    eff = EffectiveThermalConductivity(contact='Left', normalization=-2.0)
+    f = fo.fourier(geo, mat.thermal_conductivity, bcs, eff)
+    np.testing.assert_allclose(f.variables['Temperature_Fourier']['data'], T0, rtol=0, atol=1e-13)
+    r = bo.bte_rta(geo, mat, bcs, fourier=f, effective_thermal_conductivity=eff, bte_max_iter=3)
+    np.testing.assert_allclose(r.variables['Temperature_BTE']['data'], T0, rtol=0, atol=1e-13)
+    assert abs(r.kappa_eff) < 1e-12 * np.abs(mat.sigma).sum()
 
 
 def test_zero_net_wall_flux_and_worker_split():

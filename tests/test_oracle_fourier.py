@@ -64,3 +64,29 @@ def test_symmetric_laplacian_structure():
     assert np.abs(Ak - Ak.T).max() < 1e-12 * np.abs(A).max()
     assert np.abs(A[keep].sum(axis=1)).max() < 1e-12 * np.abs(A).max()
     assert A[fo.PINNED_CELL, fo.PINNED_CELL] == geo.elem_volumes[fo.PINNED_CELL]
+
+
+@pytest.mark.parametrize("h", [100.0, 1.0e9])
+def test_mixed_contacts_closed_form(h):
+    """Isothermal contacts (mixed BCs) at x = -L/2 (T_L) and +L/2 (T_R) of a
+    pore-free quad strip, periodic in y: the two-point flux scheme is exact for
+    1D conduction, the contact being the conductance h in series with the
+    half cell.  Flux q = (T_L - T_R) / (L / k + 2 / h); cell centres
+    T(x) = T_L - q (1 / h + (x + L / 2) / k); kappa_diffusive = q L_x / dT
+    with normalization -L_x / (L_y dT).  A wrong contact conductance, sign of
+    the wall term or pinned row fails it."""
+    from tests.conftest import isothermal_quads
+    L, Nx, Ny, k = 10.0, 10, 4, 150.0
+    TL, TR = 0.7, -0.3
+    geo = isothermal_quads(L, Nx, Ny)
+    Ly = L * Ny / Nx
+    bcs = BoundaryConditions(periodic={'Periodic_y': 0.0},
+                             mixed={'Left': {'value': TL, 'boundary_conductance': h},
+                                    'Right': {'value': TR, 'boundary_conductance': h}})
+    eff = EffectiveThermalConductivity(contact='Left', normalization=-L / (Ly * (TL - TR)))
+    r = fo.fourier(geo, k * np.eye(2), bcs, eff)
+    q = (TL - TR) / (L / k + 2.0 / h)
+    x = geo.elem_centroids[:, 0]
+    expect = TL - q * (1.0 / h + (x + L / 2) / k)
+    np.testing.assert_allclose(r.variables['Temperature_Fourier']['data'], expect, rtol=0, atol=1e-12)
+    assert abs(r.aux['kappa_diffusive'] / (q * L / (TL - TR)) - 1) < 1e-12
