@@ -11,7 +11,11 @@ Pinning (how each function is checked, see tests/test_oracle_*.py):
   rhs               -- dense fixed point on the two-wall mesh (the wall
                        source accumulates over a cell's walls); isothermal
                        source against a side-by-side assembly, uniform
-                       temperature between two contacts at the same T0
+                       temperature between two contacts at the same T0;
+                       parity unpinned: the heat term heat_source *
+                       heat_source_coeff added to every system
+                       (openbte/bte.py:199-214) follows the reference line
+                       by line
   solve_system      -- dense numpy.linalg.solve of the same matrix
   bte_rta           -- closed form kappa on uniform quads (exact upwind
                        solution), dense fixed point of the coupled system,

@@ -13,6 +13,8 @@ contacts of finite conductance.
 Parity unpinned: the mixed-boundary source is assigned, not accumulated
 (openbte/fourier.py:86), so a cell with two mixed sides keeps the last side's
 term; no check fixes that quirk, it follows the reference line by line.
+Parity unpinned as well: the heat-source scaling heat_source * 1e-18
+(openbte/fourier.py:87-93).
 """
 from __future__ import annotations
 

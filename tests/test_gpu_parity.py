@@ -55,6 +55,43 @@ def test_fourier_isothermal_contacts_match_oracle(isothermal_case):
     assert dJ < 1e-6 * np.abs(ref.variables['Flux_Fourier']['data']).max()
 
 
+def _heat(geo, scale):
+    x = geo.elem_centroids
+    return scale * (1.2 + np.sin(0.6 * x[:, 0]) * np.cos(0.9 * x[:, 1]))
+
+
+def test_fourier_heat_source_matches_oracle(isothermal_case):
+    """Per-cell heat source between isothermal contacts (openbte/fourier.py:87-93)."""
+    from oracle import fourier_oracle as fo
+    from openbte_amd.fourier import Fourier
+    geo, mat, bcs, eff = isothermal_case
+    hs = _heat(geo, 5.0e18)
+    ref = fo.fourier(geo, mat.thermal_conductivity, bcs, eff, heat_source=hs)
+    out = Fourier(geo, mat.thermal_conductivity, bcs, eff, heat_source=hs, verbose=False)
+    assert out.aux['n_iter'] == ref.aux['n_iter']
+    Tr = ref.variables['Temperature_Fourier']['data']
+    dT = np.abs(out.variables['Temperature_Fourier']['data'] - Tr).max()
+    print('heat fourier dT', dT, 'T range', Tr.min(), Tr.max())
+    assert dT < T_ATOL
+    assert abs(out.aux['kappa_diffusive'] / ref.aux['kappa_diffusive'] - 1) < 1e-9
+
+
+def test_fourier_modified_matches_oracle(small_case):
+    """modified_fourier: F + I with the global temperature added to the
+    source, no pinned cell and no centring (openbte/fourier.py:111-113,150-153)."""
+    from oracle import fourier_oracle as fo
+    from openbte_amd.fourier import Fourier
+    geo, mat, bcs, eff = small_case
+    kw = dict(modified_fourier=True, global_temperature=0.2)
+    ref = fo.fourier(geo, mat.thermal_conductivity, bcs, eff, **kw)
+    out = Fourier(geo, mat.thermal_conductivity, bcs, eff, verbose=False, **kw)
+    assert out.aux['n_iter'] == ref.aux['n_iter']
+    dT = np.abs(out.variables['Temperature_Fourier']['data'] - ref.variables['Temperature_Fourier']['data']).max()
+    print('modified fourier dT', dT)
+    assert dT < T_ATOL
+    assert abs(out.aux['kappa_diffusive'] / ref.aux['kappa_diffusive'] - 1) < 1e-9
+
+
 def test_fourier_multi_launch_path_matches_oracle():
     """More cells than one workgroup's solver takes: the multi-launch CG."""
     from oracle import fourier_oracle as fo
@@ -162,6 +199,25 @@ def test_bte_isothermal_contacts_match_oracle(isothermal_case, mc, block):
     np.testing.assert_allclose(out.aux['suppression'], ref.aux['suppression'], rtol=1e-8, atol=1e-12)
     Jr = ref.variables['Flux_BTE']['data']
     assert np.abs(out.variables['Flux_BTE']['data'] - Jr).max() < 1e-7 * np.abs(Jr).max()
+
+
+def test_bte_heat_source_matches_oracle(isothermal_case):
+    """Per-cell heat source (heat_source x heat_source_coeff added to every
+    system's source, openbte/bte.py:199-214) between isothermal contacts."""
+    from oracle import bte_oracle as bo, fourier_oracle as fo
+    from openbte_amd.bte import BTE_RTA, SweepConfig
+    geo, mat, bcs, eff = isothermal_case
+    hs = _heat(geo, 0.1 / mat.heat_source_coeff)
+    f = fo.fourier(geo, mat.thermal_conductivity, bcs, eff)
+    ref = bo.bte_rta(geo, mat, bcs, fourier=f, effective_thermal_conductivity=eff, bte_max_iter=6, heat_source=hs)
+    out = BTE_RTA(geo, mat, bcs, fourier=f, effective_thermal_conductivity=eff, bte_max_iter=6, heat_source=hs,
+                  config=SweepConfig(mc=4, block=64))
+    assert out.aux['n_iter'] == ref.aux['n_iter']
+    rel = abs(out.kappa_eff / ref.kappa_eff - 1)
+    dT = np.abs(out.variables['Temperature_BTE']['data'] - ref.variables['Temperature_BTE']['data']).max()
+    print('heat bte kappa', out.kappa_eff, ref.kappa_eff, 'rel', rel, 'dT', dT)
+    assert rel < KAPPA_RTOL
+    assert dT < T_ATOL
 
 
 def test_bte_small_basis_matches_oracle(small_case):
