@@ -42,6 +42,34 @@ def test_batch_equals_single_and_oracle():
                       - ref.variables['Temperature_BTE']['data']).max() < 1e-8
 
 
+def test_batch_mixed_boundary_kinds(isothermal_case, two_wall_case):
+    """One batch holding an isothermal-contact problem with a heat source,
+    quads with two-wall cells and a periodic pore cell: each result is bitwise
+    its standalone solve."""
+    from oracle import fourier_oracle as fo
+    from openbte_amd.bte import BTE_RTA
+    from openbte_amd.batch import BTE_RTA_batch
+    geo, mat, bcs, eff = isothermal_case
+    x = geo.elem_centroids
+    iso = dict(geo=geo, mat=mat, bcs=bcs, effective_thermal_conductivity=eff,
+               heat_source=0.1 / mat.heat_source_coeff * (1.2 + np.sin(0.6 * x[:, 0])))
+    g2, m2, b2, e2, _ = two_wall_case
+    probs = [iso, dict(geo=g2, mat=m2, bcs=b2, effective_thermal_conductivity=e2), _problem(14, 400)]
+    for pr in probs:
+        pr['fourier'] = fo.fourier(pr['geo'], pr['mat'].thermal_conductivity, pr['bcs'],
+                                   pr['effective_thermal_conductivity'])
+    outs = BTE_RTA_batch(probs, bte_max_iter=6)
+    for pr, out in zip(probs, outs):
+        single = BTE_RTA(pr['geo'], pr['mat'], pr['bcs'], fourier=pr['fourier'],
+                         effective_thermal_conductivity=pr['effective_thermal_conductivity'],
+                         heat_source=pr.get('heat_source'), bte_max_iter=6)
+        assert out.aux['n_iter'] == single.aux['n_iter']
+        assert out.kappa_eff == single.kappa_eff
+        np.testing.assert_array_equal(out.variables['Temperature_BTE']['data'],
+                                      single.variables['Temperature_BTE']['data'])
+        np.testing.assert_array_equal(out.variables['Flux_BTE']['data'], single.variables['Flux_BTE']['data'])
+
+
 def test_batch_different_iteration_counts():
     """Problems stop at their own iteration; the early ones stay frozen."""
     from openbte_amd.bte import BTE_RTA
