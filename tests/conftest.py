@@ -105,3 +105,9 @@ def isothermal_case():
                                     'Right': {'value': -0.5, 'boundary_conductance': 100.0}})
     eff = EffectiveThermalConductivity(contact='Left', normalization=-Nx / Ny)
     return geo, mat, bcs, eff
+
+
+@pytest.fixture(scope="session")
+def isothermal_mesh():
+    """The isothermal_quads builder, for tests that choose its sizes."""
+    return isothermal_quads

@@ -197,13 +197,12 @@ def test_dense_fixed_point_two_wall_cells(two_wall_case):
     assert abs(r.kappa_eff / K - 1) < 1e-6
 
 
-def test_isothermal_uniform_temperature():
+def test_isothermal_uniform_temperature(isothermal_mesh):
     """Both contacts at T0, no diffuse walls, no jump: every mode and the
     lattice temperature equal T0 exactly (pins the sign and size of the
     isothermal source against the outflow diagonal), and the contact kappa is
     T0 sum sigma.n = 0 over a symmetric set of directions."""
-    from tests.conftest import isothermal_quads
-    geo = isothermal_quads(10.0, 8, 4)
+    geo = isothermal_mesh(10.0, 8, 4)
     mat = synthetic_material_2d(3, 8, 1.0, 1.0e3)
     T0 = 0.37
     bcs = BoundaryConditions(periodic={'Periodic_y': 0.0},

@@ -67,7 +67,7 @@ def test_symmetric_laplacian_structure():
 
 
 @pytest.mark.parametrize("h", [100.0, 1.0e9])
-def test_mixed_contacts_closed_form(h):
+def test_mixed_contacts_closed_form(h, isothermal_mesh):
     """Isothermal contacts (mixed BCs) at x = -L/2 (T_L) and +L/2 (T_R) of a
     pore-free quad strip, periodic in y: the two-point flux scheme is exact for
     1D conduction, the contact being the conductance h in series with the
@@ -75,10 +75,9 @@ def test_mixed_contacts_closed_form(h):
     T(x) = T_L - q (1 / h + (x + L / 2) / k); kappa_diffusive = q L_x / dT
     with normalization -L_x / (L_y dT).  A wrong contact conductance, sign of
     the wall term or pinned row fails it."""
-    from tests.conftest import isothermal_quads
     L, Nx, Ny, k = 10.0, 10, 4, 150.0
     TL, TR = 0.7, -0.3
-    geo = isothermal_quads(L, Nx, Ny)
+    geo = isothermal_mesh(L, Nx, Ny)
     Ly = L * Ny / Nx
     bcs = BoundaryConditions(periodic={'Periodic_y': 0.0},
                              mixed={'Left': {'value': TL, 'boundary_conductance': h},
