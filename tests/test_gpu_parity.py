@@ -248,8 +248,10 @@ def test_interface_not_converged_warns(small_case):
         dev.outer_iteration()
 
 
-def test_bte_closed_form_quads():
-    """Uniform quads (kin = 2 or 3 inflow slots): exact closed-form kappa after one iteration."""
+@pytest.mark.parametrize("axis", [0, 1])
+def test_bte_closed_form_quads(axis):
+    """Uniform quads (kin = 2 or 3 inflow slots): exact closed-form kappa after
+    one iteration, driven and measured along x (axis 0) or y (axis 1)."""
     from oracle import fourier_oracle as fo
     from openbte_amd.bte import BTE_RTA
     from openbte_amd.geometry import quad_cell_2d
@@ -258,12 +260,12 @@ def test_bte_closed_form_quads():
     L, N = 10.0, 12
     geo = quad_cell_2d(L, N)
     mat = synthetic_material_2d(5, 12, 1.0, 1.0e3)
-    bcs = BoundaryConditions(periodic={'Periodic_x': 1.0, 'Periodic_y': 0.0})
-    eff = EffectiveThermalConductivity(contact='Periodic_x', normalization=-1.0)
+    bcs = BoundaryConditions(periodic={'Periodic_x': 1.0 - axis, 'Periodic_y': float(axis)})
+    eff = EffectiveThermalConductivity(contact=('Periodic_x', 'Periodic_y')[axis], normalization=-1.0)
     f = fo.fourier(geo, mat.thermal_conductivity, bcs, eff)
     out = BTE_RTA(geo, mat, bcs, fourier=f, effective_thermal_conductivity=eff, bte_max_iter=5)
-    sx = mat.vmfp[:, 0]
-    sig = mat.sigma[:, :, 0]
+    sx = mat.vmfp[:, axis]
+    sig = mat.sigma[:, :, axis]
     expect = np.sum(mat.mfps[:, None] * sx[None, :] * sig) + 0.5 * (L / N) * np.sum(np.abs(sig))
     assert abs(out.kappa_eff / expect - 1) < 1e-11
     assert out.aux['n_iter'] == 1

@@ -29,22 +29,26 @@ BCS = BoundaryConditions(periodic={'Periodic_x': 1.0, 'Periodic_y': 0.0}, diffus
 EFF = EffectiveThermalConductivity(contact='Periodic_x', normalization=-1.0)
 
 
-@pytest.mark.parametrize("N,M,nphi", [(8, 3, 8), (12, 5, 12)])
-def test_closed_form_uniform_quads(N, M, nphi):
+@pytest.mark.parametrize("N,M,nphi,axis", [(8, 3, 8, 0), (12, 5, 12, 0), (8, 3, 8, 1), (10, 4, 12, 1)])
+def test_closed_form_uniform_quads(N, M, nphi, axis):
+    """axis 0: jump and contact on Periodic_x; axis 1: on Periodic_y (the y
+    jump, its sign and the contact's e1/e2 roles on the y sides)."""
     L = 10.0
     geo = quad_cell_2d(L, N)
     mat = synthetic_material_2d(M, nphi, 1.0, 1.0e3)
-    bcs = BoundaryConditions(periodic={'Periodic_x': 1.0, 'Periodic_y': 0.0})
-    f = fo.fourier(geo, mat.thermal_conductivity, bcs, EFF)
+    name = ('Periodic_x', 'Periodic_y')[axis]
+    bcs = BoundaryConditions(periodic={'Periodic_x': 1.0 - axis, 'Periodic_y': float(axis)})
+    eff = EffectiveThermalConductivity(contact=name, normalization=-1.0)
+    f = fo.fourier(geo, mat.thermal_conductivity, bcs, eff)
     TL = f.variables['Temperature_Fourier']['data']
-    # the Fourier guess is exactly linear with slope 1/L along x
-    x = geo.elem_centroids[:, 0]
+    # the Fourier guess is exactly linear with slope 1/L along the axis
+    x = geo.elem_centroids[:, axis]
     assert np.abs(TL - (x / L)).max() < 1e-12
-    assert abs(f.aux['kappa_diffusive'] - mat.thermal_conductivity[0, 0]) < 1e-9 * 150
-    r = bo.bte_rta(geo, mat, bcs, fourier=f, effective_thermal_conductivity=EFF, bte_max_iter=5)
+    assert abs(f.aux['kappa_diffusive'] - mat.thermal_conductivity[axis, axis]) < 1e-9 * 150
+    r = bo.bte_rta(geo, mat, bcs, fourier=f, effective_thermal_conductivity=eff, bte_max_iter=5)
     h = L / N
-    sx = mat.vmfp[:, 0]
-    sig = mat.sigma[:, :, 0]
+    sx = mat.vmfp[:, axis]
+    sig = mat.sigma[:, :, axis]
     expect = np.sum(mat.mfps[:, None] * sx[None, :] * sig) + 0.5 * h * np.sum(np.abs(sig))
     assert abs(r.kappa_eff / expect - 1) < 1e-11
     assert np.abs(r.variables['Temperature_BTE']['data'] - TL).max() < 1e-12
