@@ -92,6 +92,14 @@ def test_fourier_modified_matches_oracle(small_case):
     assert abs(out.aux['kappa_diffusive'] / ref.aux['kappa_diffusive'] - 1) < 1e-9
 
 
+def test_fourier_cg_stop_above_tolerance_warns(small_case):
+    """A CG that ends at its iteration limit above its tolerance says so."""
+    from openbte_amd.fourier import Fourier
+    geo, mat, bcs, eff = small_case
+    with pytest.warns(RuntimeWarning, match='Fourier CG stopped'):
+        Fourier(geo, mat.thermal_conductivity, bcs, eff, verbose=False, fourier_max_iter=1, cg_rtol=1e-40)
+
+
 def test_fourier_multi_launch_path_matches_oracle():
     """More cells than one workgroup's solver takes: the multi-launch CG."""
     from oracle import fourier_oracle as fo
