@@ -803,8 +803,15 @@ __device__ __forceinline__ int task_of_block(int b, int ntask) {
 }
 
 // single problem: one task per block
+// minimum waves per SIMD asked of the compiler: the fused final sweep on
+// tetrahedra at MC = 4 lands two registers over the three-wave budget
+// (168 VGPRs) unasked; every other variant keeps the compiler's choice
 template <int MC, int MODE, int KIN, bool WIN>
-__global__ __launch_bounds__(kSweepBlock) void k_sweep(One src, int ntask, int cap) {
+constexpr int sweep_min_waves() { return (MC == 4 && KIN == 3 && MODE == 3 && WIN) ? 3 : 1; }
+
+template <int MC, int MODE, int KIN, bool WIN>
+__global__ __launch_bounds__(kSweepBlock) __attribute__((amdgpu_waves_per_eu(sweep_min_waves<MC, MODE, KIN, WIN>())))
+void k_sweep(One src, int ntask, int cap) {
   __shared__ SweepShared<MC> sh;
   const int t = task_of_block(blockIdx.x, ntask);
   if (t < 0) return;
