@@ -822,9 +822,15 @@ void k_sweep(One src, int ntask, int cap) {
 
 // batch: persistent workgroups draining 8 XCD-local task queues (tasks of one
 // (problem, direction) share a queue; a drained queue steals from the next)
+// the batch kernel's fused final sweep on tetrahedra at MC = 8 lands seven
+// registers over the three-wave budget unasked (three waves: -2% per outer
+// iteration of a batch of tetrahedral slabs)
 template <int MC, int MODE, int KIN, bool WIN>
-__global__ __launch_bounds__(kSweepBlock) void k_sweep_batch(Many src, const int32_t* __restrict__ qoff,
-                                                           int32_t* __restrict__ qcnt, int cap) {
+constexpr int batch_min_waves() { return (MC == 8 && KIN == 3 && MODE == 3 && WIN) ? 3 : 1; }
+
+template <int MC, int MODE, int KIN, bool WIN>
+__global__ __launch_bounds__(kSweepBlock) __attribute__((amdgpu_waves_per_eu(batch_min_waves<MC, MODE, KIN, WIN>())))
+void k_sweep_batch(Many src, const int32_t* __restrict__ qoff, int32_t* __restrict__ qcnt, int cap) {
   __shared__ SweepShared<MC> sh;
   __shared__ int s_task;
   const int xcd = blockIdx.x % 8;
