@@ -305,7 +305,7 @@ def run_serve(args):
     for g, m, b, e in sorted(probs, key=lambda p: p[0].n_elems)[:1] + [p for p in probs if p[0].dim == 3][:1]:
         f = Fourier(g, m.thermal_conductivity, b, e, verbose=False)
         BTE_RTA(g, m, b, fourier=f, effective_thermal_conductivity=e, bte_max_iter=2,
-                config=SweepConfig(**vars(cfg)))
+                config=SweepConfig(**vars(cfg)), verbose=False)
     torch.cuda.synchronize()
 
     # plain form: each request alone (Fourier guess + BTE solve), one after the other
@@ -315,7 +315,7 @@ def run_serve(args):
         ts = time.perf_counter()
         f = Fourier(g, m.thermal_conductivity, b, e, verbose=False)
         r = BTE_RTA(g, m, b, fourier=f, effective_thermal_conductivity=e, bte_max_iter=maxit, bte_rtol=rtol,
-                    config=SweepConfig(**vars(cfg)))
+                    config=SweepConfig(**vars(cfg)), verbose=False)
         torch.cuda.synchronize()
         service.append(time.perf_counter() - ts)
         plain_res.append(r)

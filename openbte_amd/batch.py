@@ -282,7 +282,8 @@ class BTEBatch:
             nrm = np.sqrt(self.norm2.cpu().numpy()[self.slots])
             for p in np.nonzero(live)[0]:
                 self.n_iter[p] += 1
-                self.error[p] = abs(nrm[p] - self.norm_old[p]) / nrm[p]
+                with np.errstate(divide='ignore', invalid='ignore'):  # zero field: nan ends the loop
+                    self.error[p] = abs(nrm[p] - self.norm_old[p]) / nrm[p]
                 self.norm_old[p] = nrm[p]
                 if not (self.error[p] > bte_rtol and self.n_iter[p] < bte_max_iter):
                     live[p] = False

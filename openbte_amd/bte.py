@@ -128,7 +128,8 @@ class BTEDevice:
     def __init__(self, op: HostOperator, sched: Optional[Schedule] = None, cfg: Optional[SweepConfig] = None,
                  device=None, n_active=None, norm2=None):
         self.op = op
-        self.cfg = cfg or SweepConfig()
+        # a private copy: the basis capacity is clamped per problem below
+        self.cfg = SweepConfig(**vars(cfg)) if cfg is not None else SweepConfig()
         self.dev = device or _dev()
         self.sched = sched if sched is not None else build_schedule(op)
         dev, f64, i32 = self.dev, torch.float64, torch.int32
@@ -339,15 +340,17 @@ class BTEDevice:
     def solve(self, delta_t0: np.ndarray, bte_max_iter: int = 100, bte_rtol: float = 1e-4,
               verbose: bool = False, stats: Optional[IterationStats] = None, sync_each: bool = True):
         self.reset(delta_t0)
-        norm_old = float(np.linalg.norm(delta_t0))
+        norm_old = np.float64(np.linalg.norm(delta_t0))
         error, n_iter = 1.0, 0
-        M, L = self.op.n_mfp, self.op.n_dirs
         while error > bte_rtol and n_iter < bte_max_iter:
             n_iter += 1
             t0 = time.perf_counter()
             ext = self.outer_iteration()
-            norm_t = float(np.sqrt(self.norm2.item()))
-            error = abs(norm_t - norm_old) / norm_t
+            norm_t = np.sqrt(np.float64(self.norm2.item()))
+            # NumPy division as in openbte/bte.py:251-252: a zero field gives
+            # nan (not an exception), which ends the loop after this iteration
+            with np.errstate(divide='ignore', invalid='ignore'):
+                error = float(np.abs(norm_t - norm_old) / norm_t)
             norm_old = norm_t
             if stats is not None or verbose:
                 kappa = float(self.buf['kappa'].sum().item())
@@ -389,13 +392,15 @@ def default_config(ops: Sequence[HostOperator]) -> SweepConfig:
 
 def BTE_RTA(geo, mat, bcs, fourier=None, effective_thermal_conductivity=None, heat_source=None,
             direct: bool = True, bte_max_iter: int = 100, bte_rtol: float = 1e-4,
-            config: Optional[SweepConfig] = None, verbose: bool = False,
+            config: Optional[SweepConfig] = None, verbose: bool = True,
             stats: Optional[IterationStats] = None) -> SolverResults:
     """Drop-in for ``openbte.bte.BTE_RTA`` (openbte/bte.py:19-27): same
     arguments, same ``SolverResults`` (kappa_eff, aux{'suppression',
     'kappa_eff'}, variables{'Temperature_BTE', 'Flux_BTE'}).  ``direct`` is
     accepted for signature compatibility; every system is solved to
-    direct-solver accuracy."""
+    direct-solver accuracy.  Like the reference (openbte/bte.py:256) it
+    prints kappa and the outer error every iteration; ``verbose=False``
+    silences that."""
     op = build_host_operator(geo, mat, bcs, effective_thermal_conductivity, heat_source)
     if fourier is None:
         dt0 = np.zeros(geo.n_elems)

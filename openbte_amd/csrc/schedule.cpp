@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
+#include <memory>
 #include <new>
 #include <string>
 #include <vector>
@@ -186,15 +187,29 @@ int obte_sched_create(int32_t n, int64_t n_faces, int32_t dim, const int32_t* i_
     for (int64_t f = 0; f < n_faces; ++f)
       if (i_mat[f] < 0 || i_mat[f] >= n || j_mat[f] < 0 || j_mat[f] >= n)
         return obte_fail(OBTE_EINVAL, "obte_sched_create: face cell index out of range");
-    auto* s = new Schedule();
+    std::unique_ptr<Schedule> s(new Schedule());
     s->n = n;
     s->n_dirs = n_dirs;
     s->dirs.resize(n_dirs);
+    // an exception may not leave an OpenMP region: each direction records
+    // its own failure and the first one is reported after the region
+    std::vector<int> err(n_dirs, OBTE_OK);
 #pragma omp parallel for schedule(dynamic)
-    for (int32_t l = 0; l < n_dirs; ++l)
-      build_direction(n, n_faces, dim, i_mat, j_mat, k_mat, cut_candidate, dirs + (int64_t)l * dim, max_width,
-                      s->dirs[l]);
-    *handle = s;
+    for (int32_t l = 0; l < n_dirs; ++l) {
+      try {
+        build_direction(n, n_faces, dim, i_mat, j_mat, k_mat, cut_candidate, dirs + (int64_t)l * dim,
+                        max_width, s->dirs[l]);
+      } catch (const std::bad_alloc&) {
+        err[l] = OBTE_ENOMEM;
+      } catch (...) {
+        err[l] = OBTE_EINTERNAL;
+      }
+    }
+    for (int32_t l = 0; l < n_dirs; ++l) {
+      if (err[l] == OBTE_ENOMEM) return obte_fail(OBTE_ENOMEM, "obte_sched_create: out of host memory");
+      if (err[l] != OBTE_OK) return obte_fail(OBTE_EINTERNAL, "obte_sched_create: internal error");
+    }
+    *handle = s.release();
     return OBTE_OK;
   } catch (const std::bad_alloc&) {
     return obte_fail(OBTE_ENOMEM, "obte_sched_create: out of host memory");

@@ -12,10 +12,12 @@ Pinning (how each function is checked, see tests/test_oracle_*.py):
                        source accumulates over a cell's walls); isothermal
                        source against a side-by-side assembly, uniform
                        temperature between two contacts at the same T0;
-                       parity unpinned: the heat term heat_source *
-                       heat_source_coeff added to every system
-                       (openbte/bte.py:199-214) follows the reference line
-                       by line
+                       the heat term heat_source * heat_source_coeff added
+                       to every system (openbte/bte.py:199-214) by energy
+                       balance: the generated heat V H 1e-18 leaves through
+                       the isothermal contacts, and every column of sides
+                       passes on the heat generated behind it
+                       (tests/test_oracle_energy.py)
   solve_system      -- dense numpy.linalg.solve of the same matrix
   bte_rta           -- closed form kappa on uniform quads (exact upwind
                        solution), dense fixed point of the coupled system,

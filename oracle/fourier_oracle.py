@@ -9,12 +9,15 @@ periodic meshes; kappa_F = kappa_xx on orthogonal (uniform quad) meshes;
 the Laplacian solve equals scipy.sparse.linalg.spsolve of the same graph
 Laplacian up to the (max+min)/2 shift; LSQ gradient exact for linear fields;
 isothermal contacts: the exact 1D series solution of a strip between two
-contacts of finite conductance.
+contacts of finite conductance; the heat-source scaling heat_source * 1e-18
+(openbte/fourier.py:101-107) by energy balance: the conductive flux through a
+column of sides grows by the heat V H 1e-18 generated between two columns
+(tests/test_oracle_energy.py).
 Parity unpinned: the mixed-boundary source is assigned, not accumulated
 (openbte/fourier.py:86), so a cell with two mixed sides keeps the last side's
-term; no check fixes that quirk, it follows the reference line by line.
-Parity unpinned as well: the heat-source scaling heat_source * 1e-18
-(openbte/fourier.py:87-93).
+term; no check fixes that quirk, it follows the reference line by line.  It
+does not arise on meshes whose cells touch at most one mixed side (every
+workload here).
 """
 from __future__ import annotations
 

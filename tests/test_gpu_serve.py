@@ -1,5 +1,7 @@
-"""Continuous-batching server: every result equals the result of the same
-solve alone (same launch shape), whatever else shares the device; admission
+"""Continuous-batching server: every served result matches the oracle
+(kappa 1e-6 relative, temperature 1e-8 K) and equals the result of the same
+solve alone (same launch shape), whatever else shares the device, also for a
+mix that holds a 200k-triangle cell and a ~150k-tetrahedron slab; admission
 respects the HBM budget; the footprint estimate is exact."""
 import numpy as np
 import pytest
@@ -74,6 +76,42 @@ def test_served_results_equal_alone(alone):
         _same(r.result, a)
 
 
+@pytest.fixture(scope='module')
+def oracle_results(alone):
+    from oracle import bte_oracle as bo
+    probs, fs, cfg, _ = alone
+    return [bo.bte_rta(g, m, b, fourier=f, effective_thermal_conductivity=e, bte_max_iter=12, bte_rtol=1e-6,
+                       n_workers=4) for (g, m, b, e), f in zip(probs, fs)]
+
+
+def test_served_results_match_oracle(alone, oracle_results):
+    """Every request of a staggered served mix (three 2D pore cells and a
+    tetrahedral slab) against the oracle's SuperLU solve of the same inputs
+    with the same outer-iteration count."""
+    from openbte_amd.bte import SweepConfig
+    from openbte_amd.serve import BTEServer, Request
+    probs, fs, cfg, _ = alone
+    srv = BTEServer(cfg=SweepConfig(**vars(cfg)), max_slots=8)
+    reqs = [Request(g, m, b, e, fourier=f, bte_max_iter=12, bte_rtol=1e-6) for (g, m, b, e), f in zip(probs, fs)]
+    srv.submit(reqs[3])
+    srv.submit(reqs[0])
+    srv.wait_prepared()
+    srv.step()
+    srv.submit(reqs[1])
+    srv.submit(reqs[2])
+    srv.run_until_idle()
+    srv.close()
+    for r, o in zip(reqs, oracle_results):
+        assert r.error is None
+        assert r.result.aux['n_iter'] == o.aux['n_iter']
+        rel = abs(r.result.kappa_eff / o.kappa_eff - 1)
+        dT = np.abs(r.result.variables['Temperature_BTE']['data'] - o.variables['Temperature_BTE']['data']).max()
+        Jo = o.variables['Flux_BTE']['data']
+        dJ = np.abs(r.result.variables['Flux_BTE']['data'] - Jo).max() / np.abs(Jo).max()
+        print('served vs oracle', r.id, r.result.kappa_eff, o.kappa_eff, rel, dT, dJ)
+        assert rel < 1e-6 and dT < 1e-8 and dJ < 1e-8
+
+
 def test_admission_by_footprint(alone):
     from openbte_amd.bte import SweepConfig, device_bytes
     from openbte_amd.operator import build_host_operator, build_schedule
@@ -110,3 +148,54 @@ def test_footprint_estimate_is_exact(alone):
         sc = build_schedule(op)
         d = BTEDevice(op, sched=sc, cfg=SweepConfig(**vars(cfg)))
         assert device_bytes(op, sc, cfg) == d.nbytes()
+
+
+def test_full_size_mix_served_equals_alone():
+    """The serving workload's largest requests -- a 200k-triangle pore cell
+    (20 MFP x 48 directions) and a ~150k-tetrahedron porous slab (20 MFP x
+    64 directions) -- served together with two small requests arriving while
+    they run: each result equals its solve alone bit for bit (the kernels treat
+    problems independently and reduce in a fixed order)."""
+    import torch
+    from openbte_amd.bte import BTE_RTA, SweepConfig
+    from openbte_amd.fourier import Fourier
+    from openbte_amd.geometry import pore_cell_2d, pore_slab_3d
+    from openbte_amd.material import synthetic_material_2d, synthetic_material_3d
+    from openbte_amd.objects import BoundaryConditions, EffectiveThermalConductivity
+    from openbte_amd.serve import BTEServer, Request
+    bcs = BoundaryConditions(periodic={'Periodic_x': 1.0, 'Periodic_y': 0.0}, diffuse='Boundary')
+    eff2 = EffectiveThermalConductivity(contact='Periodic_x', normalization=-1.0)
+    L3 = float(np.sqrt(150_000 * 0.45 ** 3 / 0.8 / 4.0) ** (2 / 3) * 4.0 ** (1 / 3))
+    eff3 = EffectiveThermalConductivity(contact='Periodic_x', normalization=-4.0 / L3)
+    probs = [(pore_cell_2d(100.0, 0.25, 200_000, seed=21), synthetic_material_2d(20, 48, 1.0, 1.0e4, seed=7), eff2),
+             (pore_slab_3d(L3, L3 / 4.0, 0.2, 150_000, seed=22), synthetic_material_3d(20, 8, 8, 1.0, 1.0e4, seed=7),
+              eff3),
+             (pore_cell_2d(10.0, 0.2, 1500, seed=23), synthetic_material_2d(30, 48, 1.0, 1.0e4, seed=7), eff2),
+             (pore_slab_3d(10.0, 3.0, 0.2, 6000, seed=24), synthetic_material_3d(24, 8, 8, 1.0, 1.0e4, seed=7),
+              EffectiveThermalConductivity(contact='Periodic_x', normalization=-1.0 / 3.0))]
+    print('# full-size mix:', [g.n_elems for g, _, _ in probs], flush=True)
+    assert probs[0][0].n_elems > 190_000 and probs[1][0].n_elems > 140_000
+    fs = [Fourier(g, m.thermal_conductivity, bcs, e, verbose=False) for g, m, e in probs]
+    cfg = SweepConfig()
+    alone = [BTE_RTA(g, m, bcs, fourier=f, effective_thermal_conductivity=e, config=SweepConfig(**vars(cfg)),
+                     verbose=False) for (g, m, e), f in zip(probs, fs)]
+    torch.cuda.empty_cache()
+    srv = BTEServer(cfg=SweepConfig(**vars(cfg)), max_slots=8)
+    reqs = [Request(g, m, bcs, e, fourier=f) for (g, m, e), f in zip(probs, fs)]
+    srv.submit(reqs[0])
+    srv.submit(reqs[1])
+    srv.wait_prepared()
+    srv.step()
+    srv.step()
+    srv.submit(reqs[2])
+    srv.submit(reqs[3])
+    srv.run_until_idle()
+    srv.close()
+    assert srv.stats.peak_live == 4
+    for r, a in zip(reqs, alone):
+        assert r.error is None
+        print('full-size served', r.id, r.result.kappa_eff, 'alone', a.kappa_eff, r.result.aux['n_iter'], flush=True)
+        assert r.result.aux['n_iter'] == a.aux['n_iter']
+        assert r.result.kappa_eff == a.kappa_eff
+        assert np.array_equal(r.result.variables['Temperature_BTE']['data'], a.variables['Temperature_BTE']['data'])
+        assert np.array_equal(r.result.variables['Flux_BTE']['data'], a.variables['Flux_BTE']['data'])
