@@ -329,6 +329,20 @@ int obte_fourier_solve(const obte_fourier_plan_t* plan, const double* b, const d
                        double* scal, double rtol, int32_t maxit, int32_t check_every, int32_t* iters_out,
                        double* relres_out, obte_stream_t stream);
 
+/* obte_fourier_solve without the ws / scal arguments, allocating nothing:
+ * its workspace is a file-scope device array of this library (6 * 262144 +
+ * 80 doubles, 12 MiB, held on every device the library runs on until the
+ * process ends).  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (tight serving loops, harnesses that forbid
+ * hipMalloc).  Plans with more than OBTE_FOURIER_NOALLOC_MAX_CELLS cells fail
+ * with OBTE_ELIMIT.  All calls on one device share that device's array and
+ * leave their contents in it: calls that overlap on one device (two streams
+ * or two threads) race and must be serialised by the caller. */
+#define OBTE_FOURIER_NOALLOC_MAX_CELLS 262144
+int obte_fourier_solve_noalloc(const obte_fourier_plan_t* plan, const double* b, const double* c, double* t,
+                               double rtol, int32_t maxit, int32_t check_every, int32_t* iters_out,
+                               double* relres_out, obte_stream_t stream);
+
 /* LSQ cell gradient of T: grad [n][dim]. */
 int obte_fourier_gradient(const obte_fourier_plan_t* plan, const double* t, double* grad, obte_stream_t stream);
 

@@ -107,6 +107,8 @@ _SIGNATURES = {
     'obte_bte_reduce': [C.POINTER(BtePlan), _p, _p, _p, _p, _p],
     'obte_fourier_solve': [C.POINTER(FourierPlan), _p, _p, _p, _p, _p, _f64, _i32, _i32,
                            C.POINTER(_i32), C.POINTER(_f64), _p],
+    'obte_fourier_solve_noalloc': [C.POINTER(FourierPlan), _p, _p, _p, _f64, _i32, _i32,
+                                   C.POINTER(_i32), C.POINTER(_f64), _p],
     'obte_fourier_gradient': [C.POINTER(FourierPlan), _p, _p, _p],
     'obte_fourier_correction': [C.POINTER(FourierPlan), _p, _p, _p, _p, _p, _p],
     'obte_fourier_kappa': [C.POINTER(FourierPlan), _p, _p, _p],

@@ -17,6 +17,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <mutex>
 
 #include "capi_common.h"
 
@@ -444,6 +445,19 @@ __global__ void k_shift(int n, double* t, const double* scal) {
   if (i < n) t[i] -= (scal[S_MAX] + scal[S_MIN]) * 0.5;
 }
 
+__global__ void k_fill(int64_t n, double* a, double v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+// Workspace of obte_fourier_solve_noalloc: the solve's 6 n + 64 doubles plus
+// its 16 scalars, for n up to OBTE_FOURIER_NOALLOC_MAX_CELLS (12 MiB).  Host
+// code never names this array in a launch: it reaches it through the address
+// hipGetSymbolAddress returns on each device (noalloc_scratch()).
+constexpr int64_t kNoAllocWs = 6 * (int64_t)OBTE_FOURIER_NOALLOC_MAX_CELLS + 64;
+constexpr int64_t kNoAllocDoubles = kNoAllocWs + 16;
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
 __global__ void k_gradient(obte_fourier_plan_t P, const double* t, double* grad) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= P.n) return;
@@ -528,6 +542,29 @@ int check(const obte_fourier_plan_t* P) {
   return OBTE_OK;
 }
 
+// Device address of `scratch` on the current device, looked up once per
+// device (each device holds its own copy of the array at its own address).
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+int noalloc_scratch(double** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return obte_fail_hip("obte_fourier_solve_noalloc: hipGetDevice", e);
+  if (dev < 0 || dev >= kMaxDevices) return obte_fail(OBTE_ELIMIT, "obte_fourier_solve_noalloc: device index >= 64");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return obte_fail_hip("obte_fourier_solve_noalloc: hipGetSymbolAddress", e);
+    if (!p) return obte_fail(OBTE_EHIP, "obte_fourier_solve_noalloc: null scratch address");
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return OBTE_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -607,6 +644,25 @@ int obte_fourier_solve(const obte_fourier_plan_t* plan, const double* b, const d
   if (iters_out) *iters_out = it;
   if (relres_out) *relres_out = rnorm / (bnorm > 0.0 ? bnorm : 1.0);
   return status("obte_fourier_solve: finish launch failed");
+}
+
+int obte_fourier_solve_noalloc(const obte_fourier_plan_t* plan, const double* b, const double* c, double* t,
+                               double rtol, int32_t maxit, int32_t check_every, int32_t* iters_out,
+                               double* relres_out, obte_stream_t stream) {
+  int rc = check(plan);
+  if (rc) return rc;
+  if (plan->n > OBTE_FOURIER_NOALLOC_MAX_CELLS)
+    return obte_fail(OBTE_ELIMIT, "obte_fourier_solve_noalloc: more cells than OBTE_FOURIER_NOALLOC_MAX_CELLS "
+                                  "(262144); use obte_fourier_solve with a caller workspace");
+  double* ws = nullptr;
+  rc = noalloc_scratch(&ws);
+  if (rc) return rc;
+  double* scal = ws + kNoAllocWs;
+  // the scalars of the previous call on this device are stale: start from zero
+  hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, (hipStream_t)stream, (int64_t)16, scal, 0.0);
+  rc = status("obte_fourier_solve_noalloc: fill launch failed");
+  if (rc) return rc;
+  return obte_fourier_solve(plan, b, c, t, ws, scal, rtol, maxit, check_every, iters_out, relres_out, stream);
 }
 
 int obte_fourier_gradient(const obte_fourier_plan_t* plan, const double* t, double* grad, obte_stream_t stream) {

@@ -88,3 +88,17 @@ def test_struct_offsets_match_c_compiler(tmp_path):
         assert got[(cname, 'size')] == ctypes.sizeof(py), cname
         for f in py._fields_:
             assert got[(cname, f[0])] == getattr(py, f[0]).offset, (cname, f[0])
+
+
+def test_noalloc_fourier_rejects_oversized_plan():
+    """The no-allocation Fourier solve refuses plans beyond its fixed
+    workspace before touching the device (no GPU needed)."""
+    from openbte_amd import _lib
+    lib = _lib.load()
+    P = _lib.FourierPlan()
+    P.n, P.dim, P.nv, P.pin = 262_145, 2, 3, -1
+    it, rr = ctypes.c_int32(0), ctypes.c_double(0.0)
+    rc = lib.obte_fourier_solve_noalloc(ctypes.byref(P), None, None, None, 1e-13, 10, 16, ctypes.byref(it),
+                                        ctypes.byref(rr), None)
+    assert rc == -5  # OBTE_ELIMIT
+    assert b'OBTE_FOURIER_NOALLOC_MAX_CELLS' in lib.obte_last_error()
