@@ -59,6 +59,7 @@ CONFIGS3D = {
     'tet150k': (150_000, 0.2, 40, (8, 8), 1.0e4),
 }
 HBM_PEAK = 8.0e12
+BASELINE_METRIC = 'cell\u00b7mode updates/s + p50/p99 solve latency vs plain GPU, kappa rel err \u22641e-6'
 
 
 def make_case(name, seed=0):
@@ -82,41 +83,52 @@ def make_case(name, seed=0):
     return geo, mat, bcs, eff
 
 
-def cpu_baseline(geo, mat, bcs, eff, fourier_T, budget_s=20.0):
-    """Time the oracle's SuperLU path on a bounded sample: each worker factors
-    k systems (one direction, k MFP bins spread over the range) and then
-    repeats cached solves; updates/s = n * systems solved / wall seconds."""
+def host_cores():
+    """Host cores the CPU legs may use: this process's affinity set, capped at
+    16 (the host share of one GPU on a shared multi-GPU node)."""
+    aff = len(os.sched_getaffinity(0))
+    return min(aff, 16), aff
+
+
+def cpu_baseline(geo, mat, bcs, eff, fourier_T, budget_s=30.0):
+    """Time the oracle's SuperLU path on a bounded, stratified sample of the
+    workload's systems: every MFP bin once, at directions spread over the
+    whole set (bin m at direction (37 m) mod L), dealt round-robin to the
+    workers; each worker factorises its systems, then repeats cached solves.
+    Updates/s = n * systems solved / wall seconds (factor and solve timed
+    separately)."""
     import multiprocessing as mp
+    import scipy.sparse.linalg as spla
     from oracle import bte_oracle as bo
     ops = bo.assemble(geo, mat, bcs, eff)
-    cores = len(os.sched_getaffinity(0))
-    cores = min(cores, 16)
-    n = geo.n_elems
+    cores, aff = host_cores()
+    n, M, L = geo.n_elems, ops.n_mfp, ops.n_dirs
     TB = fourier_T[ops.i_diffuse].copy()
-    # probe one factorisation to size the sample
+    systems = [((37 * m) % L, m) for m in range(M)]
+    while len(systems) < cores:  # every worker gets at least one system
+        k = len(systems)
+        systems.append(((37 * k + 11) % L, k % M))
+    per_worker = [systems[w::cores] for w in range(cores)]
+    # probe one solve to size the cached-solve repetitions
+    lu = spla.splu(bo.system_matrix(ops, systems[0][0], systems[0][1]))
     t0 = time.perf_counter()
-    import scipy.sparse.linalg as spla
-    lu = spla.splu(bo.system_matrix(ops, 0, ops.n_mfp // 2))
-    t_fac = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    lu.solve(bo.rhs(ops, 0, ops.n_mfp // 2, fourier_T, TB))
+    lu.solve(bo.rhs(ops, systems[0][0], systems[0][1], fourier_T, TB))
     t_sol = time.perf_counter() - t0
-    k = int(max(1, min(ops.n_mfp, (0.4 * budget_s) // max(t_fac, 1e-3))))
-    reps = int(max(1, (0.5 * budget_s) // max(k * t_sol, 1e-4)))
-    ms = np.linspace(0, ops.n_mfp - 1, k).astype(int)
+    k_max = max(len(s) for s in per_worker)
+    reps = int(max(1, (0.5 * budget_s) // max(k_max * t_sol, 1e-4)))
     ctx = mp.get_context('fork')
     q = ctx.Queue()
 
     def work(w):
-        l = (w * 7) % ops.n_dirs
+        mine = per_worker[w]
         t0 = time.perf_counter()
-        lus = [spla.splu(bo.system_matrix(ops, l, m)) for m in ms]
+        lus = [spla.splu(bo.system_matrix(ops, l, m)) for l, m in mine]
         tf = time.perf_counter() - t0
         t0 = time.perf_counter()
         for _ in range(reps):
-            for m, f in zip(ms, lus):
+            for (l, m), f in zip(mine, lus):
                 f.solve(bo.rhs(ops, l, m, fourier_T, TB))
-        q.put((tf, time.perf_counter() - t0))
+        q.put((w, tf, time.perf_counter() - t0))
 
     procs = [ctx.Process(target=work, args=(w,)) for w in range(cores)]
     wall0 = time.perf_counter()
@@ -126,17 +138,66 @@ def cpu_baseline(geo, mat, bcs, eff, fourier_T, budget_s=20.0):
     for p in procs:
         p.join()
     wall = time.perf_counter() - wall0
-    tf = max(r[0] for r in res)
-    ts = max(r[1] for r in res)
-    steady = n * cores * k * reps / ts
-    first = n * cores * k / (tf + ts / reps)
+    tf = max(r[1] for r in res)
+    ts = max(r[2] for r in res)
+    n_sys = len(systems)
+    steady = n * n_sys * reps / ts
+    first = n * n_sys / (tf + ts / reps)
     return {
         'value': steady, 'unit': 'cell*mode updates/s', 'cores': cores, 'kind': 'port',
-        'sample': (f'{cores} workers x {k} (l,m) systems of the same mesh/material, SuperLU factor then '
-                   f'{reps} cached solves each; steady-state (cached LU) rate; first-iteration rate '
-                   f'(factor+solve) {first:.3e}; wall {wall:.1f}s'),
-        'first_iteration_value': first,
+        'sample': (f'{n_sys} (direction, MFP bin) systems of this workload covering all {M} MFP bins at '
+                   f'{len(set(l for l, _ in systems))} directions, dealt to {cores} forked workers '
+                   f'({aff} cores in the affinity set; at most 16 used, the host share of one GPU); '
+                   f'SuperLU factor, then {reps} cached solves each; value = steady-state (cached LU) rate; '
+                   f'first-iteration rate (factor + solve) {first:.3e}; wall {wall:.1f}s'),
+        'first_iteration_value': first, 'affinity_cores': aff,
     }
+
+
+def c1_oracle(workers):
+    """The CPU-runnable workload (2k-cell pore cell, 30 MFP x 48 directions)
+    solved by the oracle from its own Fourier guess until kappa changes by
+    less than 1e-6 relative between outer iterations; wall-clock timed."""
+    from oracle import bte_oracle as bo
+    from oracle import fourier_oracle as fo
+    geo, mat, bcs, eff = make_case('plain2k')
+    t0 = time.perf_counter()
+    f = fo.fourier(geo, mat.thermal_conductivity, bcs, eff)
+    ops = bo.assemble(geo, mat, bcs, eff)
+    T_F = np.asarray(f.variables['Temperature_Fourier']['data'], np.float64)
+    DeltaT, TB = T_F.copy(), T_F[ops.i_diffuse].copy()
+    kappas = []
+    with bo.OuterLoop(ops, workers) as loop:
+        while len(kappas) < 400:
+            DeltaT, TB, J, K, S = loop.step(DeltaT, TB)
+            kappas.append(float(K.sum()))
+            if len(kappas) > 1 and abs(kappas[-1] - kappas[-2]) <= 1e-6 * abs(kappas[-1]):
+                break
+    wall = time.perf_counter() - t0
+    n, M, L = geo.n_elems, ops.n_mfp, ops.n_dirs
+    return {'fourier_T': T_F, 'delta_t': DeltaT, 'kappa': kappas[-1], 'n_iter': len(kappas), 'seconds': wall,
+            'updates_per_s': n * M * L * len(kappas) / wall, 'workers': workers, 'n': n}
+
+
+def c1_gpu_check(c1):
+    """The same 2k-cell solve on the GPU, from the oracle's Fourier guess,
+    for the oracle's iteration count: kappa and temperature errors."""
+    import torch
+    from openbte_amd.bte import BTE_RTA
+    from openbte_amd.objects import SolverResults
+    geo, mat, bcs, eff = make_case('plain2k')
+    f = SolverResults(variables={'Temperature_Fourier': {'data': c1['fourier_T'], 'units': 'K'}})
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = BTE_RTA(geo, mat, bcs, fourier=f, effective_thermal_conductivity=eff, bte_max_iter=c1['n_iter'],
+                bte_rtol=0.0, verbose=False)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return {'kappa_gpu': r.kappa_eff, 'kappa_oracle': c1['kappa'],
+            'kappa_rel_err': abs(r.kappa_eff / c1['kappa'] - 1),
+            'temperature_max_abs_err_K': float(np.abs(r.variables['Temperature_BTE']['data'] - c1['delta_t']).max()),
+            'outer_iterations': c1['n_iter'], 'gpu_solve_s': wall, 'oracle_solve_s': c1['seconds'],
+            'oracle_workers': c1['workers'], 'oracle_updates_per_s': c1['updates_per_s']}
 
 
 def make_batch(name):
@@ -153,7 +214,7 @@ def make_batch(name):
     return geos, mat, bcs, eff
 
 
-def run_batch(args):
+def measure_batch(args):
     """Batched form vs the plain one-solve-at-a-time form on the same problems."""
     import torch
     from openbte_amd.batch import BTEBatch
@@ -245,7 +306,11 @@ def run_batch(args):
                    'batched_over_plain': dt_plain / dt_batch,
                    'max_rel_kappa_diff_batched_vs_plain': same},
     }
-    print(json.dumps(out))
+    return out
+
+
+def run_batch(args):
+    print(json.dumps(measure_batch(args)))
     return 0
 
 
@@ -278,7 +343,7 @@ def make_requests(name):
     return out
 
 
-def run_serve(args):
+def measure_serve(args):
     """Continuous-batching server vs the plain one-solve-at-a-time GPU form
     under the same open-loop Poisson request stream (lambda = 70% of the
     server's measured saturated capacity)."""
@@ -388,7 +453,11 @@ def run_serve(args):
                    'peak_live': st_sat.peak_live, 'peak_bytes': st_sat.peak_bytes,
                    'max_rel_kappa_diff_served_vs_alone': kdiff},
     }
-    print(json.dumps(out))
+    return out
+
+
+def run_serve(args):
+    print(json.dumps(measure_serve(args)))
     return 0
 
 
@@ -472,6 +541,73 @@ def run_inverse(args):
     return 0
 
 
+def time_to_solution(geo, mat, bcs, eff, reps=2):
+    """Wall time of a whole plain solve through the drop-in entries: Fourier
+    guess, then BTE_RTA to the reference's stop rule (|dT| < 1e-4), host
+    preparation and device upload included; best of ``reps``."""
+    import torch
+    from openbte_amd.bte import BTE_RTA
+    from openbte_amd.fourier import Fourier
+    best = None
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        f = Fourier(geo, mat.thermal_conductivity, bcs, eff, verbose=False)
+        t1 = time.perf_counter()
+        r = BTE_RTA(geo, mat, bcs, fourier=f, effective_thermal_conductivity=eff, verbose=False)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        rec = {'seconds': t2 - t0, 'fourier_s': t1 - t0, 'bte_rta_s': t2 - t1,
+               'bte_rta_breakdown_s': r.aux.get('timing'), 'outer_iterations': r.aux['n_iter'],
+               'kappa_eff': r.kappa_eff}
+        if best is None or rec['seconds'] < best['seconds']:
+            best = rec
+        del r, f
+        torch.cuda.empty_cache()
+    return best
+
+
+def north_star_legs(args, geo, mat, bcs, eff, c1, value):
+    """The rest of the headline metric beside the plain throughput: a whole
+    solve's latency, the kappa error against the oracle on the converged
+    2k-cell workload, and the batched and served forms against the plain
+    one-solve-at-a-time form (shorter runs of the batch64 / serve configs)."""
+    import torch
+    torch.cuda.empty_cache()
+    north = {'plain_updates_per_s': value}
+    tts = time_to_solution(geo, mat, bcs, eff)
+    north['plain200k_time_to_solution_s'] = tts['seconds']
+    north['plain200k_time_to_solution'] = tts
+    print(f'# time to solution {tts["seconds"]:.3f}s ({tts["outer_iterations"]} outer iterations)',
+          file=sys.stderr, flush=True)
+    if c1 is not None:
+        chk = c1_gpu_check(c1)
+        north['kappa_rel_err_vs_oracle'] = chk['kappa_rel_err']
+        north['converged_2k_parity'] = chk
+        print(f'# 2k converged: kappa rel err {chk["kappa_rel_err"]:.2e}, T err '
+              f'{chk["temperature_max_abs_err_K"]:.2e} K', file=sys.stderr, flush=True)
+    ba = argparse.Namespace(config='batch64', steps=5, warmup=1, mc=None, block=None,
+                            no_window=args.no_window, no_graphs=False)
+    b = measure_batch(ba)
+    north['batched_over_plain'] = b['config']['batched_over_plain']
+    north['batched'] = {'value': b['value'], 'ms_per_step': b['ms_per_step'], 'steps': b['steps'],
+                        'plain_value': b['config']['plain_value'], 'shape': [b['config']['mc'], b['config']['block']],
+                        'max_rel_kappa_diff_batched_vs_plain': b['config']['max_rel_kappa_diff_batched_vs_plain'],
+                        'workload': b['config']['workload']}
+    torch.cuda.empty_cache()
+    sv = measure_serve(argparse.Namespace(config='serve', mc=None, block=None))
+    c = sv['config']
+    north['served_over_plain'] = c['served_over_plain']
+    north['served_p50_s'] = c['served_p50_s']
+    north['served_p99_s'] = c['served_p99_s']
+    north['plain_p50_s'] = c['plain_p50_s']
+    north['plain_p99_s'] = c['plain_p99_s']
+    north['served'] = {k: c[k] for k in ('workload', 'lambda_per_s', 'plain_value', 'open_loop_updates_per_s',
+                                         'max_rel_kappa_diff_served_vs_alone', 'served_prep_p50_s', 'peak_live')}
+    north['served']['value'] = sv['value']
+    return north
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -486,7 +622,10 @@ def main(argv=None):
     ap.add_argument('--mc', type=int, default=None)
     ap.add_argument('--block', type=int, default=None)
     ap.add_argument('--no-cpu', action='store_true')
-    ap.add_argument('--cpu-budget', type=float, default=20.0)
+    ap.add_argument('--cpu-budget', type=float, default=30.0)
+    ap.add_argument('--no-extras', action='store_true',
+                    help='plain200k: skip the converged 2k-cell parity check, time-to-solution, batched and '
+                         'served legs (kernel profiling runs)')
     ap.add_argument('--copy-calibration', action='store_true',
                     help='run one 1 GiB device-to-device copy (known bytes) for HBM counter calibration')
     ap.add_argument('--no-fourier', action='store_true',
@@ -513,14 +652,26 @@ def main(argv=None):
     n, M, L = geo.n_elems, mat.grid[0], int(np.prod(mat.grid[1:]))
     print(f'# config {args.config}: n={n} M={M} L={L}', file=sys.stderr, flush=True)
     cpu = None
+    c1 = None
+    extras = args.config == 'plain200k' and not args.no_extras
     if args.config in CONFIGS3D and not args.no_cpu:
         # one SuperLU factorisation of a 3D system of this size takes minutes
         print('# no CPU leg for the 3D configuration (SuperLU fill-in)', file=sys.stderr, flush=True)
     elif not args.no_cpu:
-        # CPU baseline first, before this process touches the GPU (its workers fork)
+        # CPU legs first, before this process touches the GPU (their workers fork)
         from oracle import fourier_oracle as fo
+        if extras:
+            c1 = c1_oracle(host_cores()[0])
+            print(f'# oracle 2k-cell converged solve: {c1["n_iter"]} iterations, {c1["seconds"]:.1f}s, '
+                  f'kappa {c1["kappa"]:.10f}', file=sys.stderr, flush=True)
         T_cpu = fo.fourier(geo, mat.thermal_conductivity, bcs, eff).variables['Temperature_Fourier']['data']
         cpu = cpu_baseline(geo, mat, bcs, eff, T_cpu, budget_s=args.cpu_budget)
+        if c1 is not None:
+            cpu['converged_2k_value'] = c1['updates_per_s']
+            cpu['converged_2k_sample'] = (f'2k-cell workload ({c1["n"]} cells, 30 MFP x 48 directions) solved in '
+                                          f'full by the oracle on {c1["workers"]} forked workers: '
+                                          f'{c1["n_iter"]} outer iterations to a kappa change < 1e-6, '
+                                          f'{c1["seconds"]:.2f}s wall incl. Fourier and factorisations')
         print(f'# cpu baseline {cpu["value"]:.3e} updates/s on {cpu["cores"]} cores', file=sys.stderr, flush=True)
     t_setup = time.perf_counter()
     fourier_info = None
@@ -603,6 +754,7 @@ def main(argv=None):
         kernel = 'k_sweep final sweep (mode 2: every solution value written)'
     t_full = float(np.mean(by_mode[2]))
     achieved = sweep_bytes / t_full
+    window_positions, n_cut, kmax = dev.window_positions(), dev.n_cut, dev.cfg.kmax
     print(f'# {args.steps} steps in {dt:.3f}s: {value:.3e} updates/s; extensions per step {exts}; '
           f'sweep ms mean by mode {mean_ms}, total {total_ms}; kappa {kappa:.6f}',
           file=sys.stderr, flush=True)
@@ -615,8 +767,16 @@ def main(argv=None):
             f_, w_ = rec['fetch_kib_per_dispatch'], rec['write_kib_per_dispatch']
             traffic = 1024.0 * (f_['mode2'] + w_['mode2'])
 
+    north = None
+    if extras:
+        del dev
+        north = north_star_legs(args, geo, mat, bcs, eff, c1, value)
+    frac_counter = traffic / t_full / HBM_PEAK if traffic else None
     out = {
-        'metric': 'cell*mode updates/s (BTE outer iteration, all systems to direct-solver accuracy)',
+        'metric': BASELINE_METRIC,
+        'metric_detail': ('value: cell*mode updates/s of the plain solve, outer iterations 1..K from the Fourier '
+                          'guess, every system to direct-solver accuracy; latency, batched/served ratios and '
+                          'the kappa error vs the oracle in north_star'),
         'value': value,
         'unit': 'cell*mode updates/s',
         'n_gpus': 1,
@@ -633,20 +793,35 @@ def main(argv=None):
                                 if args.config in CONFIGS else
                                 f'{args.config}: periodic porous slab, n={n} tetrahedra, ')
                                + f'{M} MFP x {L} directions = {M * L} systems',
-                   'mc': cfg.mc, 'block': cfg.block, 'lds_window_positions': dev.window_positions(),
+                   'mc': cfg.mc, 'block': cfg.block, 'lds_window_positions': window_positions,
                    'kappa_eff_after_steps': kappa,
-                   'extensions_per_step': exts, 'n_cut_faces': dev.n_cut,
-                   'outer_error_per_step': errors, 'kmax': cfg.kmax, 'gcr_rtol': cfg.rtol,
+                   'extensions_per_step': exts, 'n_cut_faces': n_cut,
+                   'outer_error_per_step': errors, 'kmax': kmax, 'gcr_rtol': cfg.rtol,
                    'fourier': fourier_info, 'setup_s': t_setup},
         'roofline': {'bound': 'hbm', 'achieved': achieved / 1e9, 'peak': HBM_PEAK / 1e9, 'unit': 'GB/s',
                      'frac': achieved / HBM_PEAK, 'traffic': traffic,
-                     'traffic_source': 'rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE (profiles/)' if traffic else None,
+                     'achieved_model': (f'algorithmic bytes: {per_update:g} B per cell*mode update (source 8 + '
+                                        f'solution 8 + 8 per upwind gather) x n*M*L updates of one final-sweep '
+                                        f'launch / its mean launch time (HIP events on the launch stream)'),
+                     'traffic_source': ('stored rocprofv3 --pmc FETCH_SIZE + WRITE_SIZE per dispatch of this '
+                                        'kernel and shape (profiles/); not measured in this run')
+                     if traffic else None,
+                     'frac_counter': frac_counter,
+                     'frac_end_to_end': value * per_update / HBM_PEAK,
+                     'limiter': ('CU pipelines, not HBM: the windowed sweep keeps upwind values in LDS, so its '
+                                 'HBM traffic is below the model bytes; counters show VALU ~0.7 and the '
+                                 'texture data path ~0.9 busy'),
                      'kernel': kernel,
                      'bytes_per_launch': sweep_bytes, 'bytes_per_update': per_update,
                      'compulsory_bytes_per_launch': compulsory, 'mean_launch_s': t_full,
                      'sweep_ms_mean_by_mode': mean_ms, 'sweep_ms_total_by_mode': total_ms},
         'cpu_baseline': cpu,
     }
+    if north is not None:
+        if cpu is not None:
+            north['gpu_over_cpu_steady'] = value / cpu['value']
+            north['gpu_over_cpu_first_iteration'] = value / cpu['first_iteration_value']
+        out['north_star'] = north
     print(json.dumps(out))
     return 0
 

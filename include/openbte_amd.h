@@ -49,6 +49,10 @@ int obte_sched_create(int32_t n, int64_t n_faces, int32_t dim, const int32_t* i_
 int obte_sched_sizes(void* handle, int64_t* sizes);
 int obte_sched_export(void* handle, int32_t* order, int32_t* pos, int32_t* lvl_ptr, int32_t* in_ptr,
                       int32_t* in_src, int32_t* in_face, int32_t* cut_exit, int32_t* cut_face);
+/* Fixed-width inflow slots nb_src / nb_face ([L][kin][n] each, the layout of
+ * obte_bte_plan_t) from the same schedule; kin >= the most inflow faces of
+ * one cell, else OBTE_EINVAL. */
+int obte_sched_export_slots(void* handle, int32_t kin, int32_t* nb_src, int32_t* nb_face);
 int obte_sched_destroy(void* handle);
 
 /* ---- device plan of one BTE solve ----------------------------------- */

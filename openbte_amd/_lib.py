@@ -86,6 +86,7 @@ _SIGNATURES = {
     'obte_sched_create': [_i32, _i64, _i32, _p, _p, _p, _p, _i32, _p, _i32, C.POINTER(_p)],
     'obte_sched_sizes': [_p, _p],
     'obte_sched_export': [_p, _p, _p, _p, _p, _p, _p, _p, _p],
+    'obte_sched_export_slots': [_p, _i32, _p, _p],
     'obte_sched_destroy': [_p],
     'obte_bte_assemble': [C.POINTER(BtePlan), _p],
     'obte_bte_source': [C.POINTER(BtePlan), _p],

@@ -401,14 +401,22 @@ def BTE_RTA(geo, mat, bcs, fourier=None, effective_thermal_conductivity=None, he
     direct-solver accuracy.  Like the reference (openbte/bte.py:256) it
     prints kappa and the outer error every iteration; ``verbose=False``
     silences that."""
+    t0 = time.perf_counter()
     op = build_host_operator(geo, mat, bcs, effective_thermal_conductivity, heat_source)
+    t1 = time.perf_counter()
+    sched = build_schedule(op)
+    t2 = time.perf_counter()
     if fourier is None:
         dt0 = np.zeros(geo.n_elems)
     else:
         dt0 = np.asarray(fourier.variables['Temperature_Fourier']['data'], dtype=np.float64)
-    solver = BTEDevice(op, cfg=config or default_config([op]))
+    solver = BTEDevice(op, sched=sched, cfg=config or default_config([op]))
+    torch.cuda.synchronize(solver.dev)
+    t3 = time.perf_counter()
     n_iter = solver.solve(dt0, bte_max_iter=bte_max_iter, bte_rtol=bte_rtol, verbose=verbose, stats=stats)
     DeltaT, J, K, S = solver.results()
+    t4 = time.perf_counter()
+    timing = {'operator_s': t1 - t0, 'schedule_s': t2 - t1, 'device_setup_s': t3 - t2, 'iterations_s': t4 - t3}
     if op.contact_kind == 0:
         K = np.zeros_like(K)
         S = np.zeros_like(S)
@@ -416,4 +424,4 @@ def BTE_RTA(geo, mat, bcs, fourier=None, effective_thermal_conductivity=None, he
     variables = {'Temperature_BTE': {'data': DeltaT, 'units': 'K'},
                  'Flux_BTE': {'data': J * 1e9, 'units': 'W/m/m'}}
     return SolverResults(kappa_eff=kappa_eff, variables=variables,
-                         aux={'suppression': S, 'kappa_eff': kappa_eff, 'n_iter': n_iter})
+                         aux={'suppression': S, 'kappa_eff': kappa_eff, 'n_iter': n_iter, 'timing': timing})

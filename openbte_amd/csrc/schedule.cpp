@@ -259,6 +259,44 @@ int obte_sched_export(void* handle, int32_t* order, int32_t* pos, int32_t* lvl_p
   return OBTE_OK;
 }
 
+// Fixed-width inflow slots in position order, [L][kin][n] each: slot k of
+// position p holds p's k-th inflow entry as an upstream row (a position, or
+// n + cut index for a cut face) and its directed face; unused slots point at
+// p itself with face -1.  Directions are filled in parallel.
+int obte_sched_export_slots(void* handle, int32_t kin, int32_t* nb_src, int32_t* nb_face) {
+  if (!handle || !nb_src || !nb_face || kin <= 0)
+    return obte_fail(OBTE_EINVAL, "obte_sched_export_slots: invalid arguments");
+  auto* s = static_cast<Schedule*>(handle);
+  const int64_t n = s->n;
+  for (int32_t l = 0; l < s->n_dirs; ++l) {
+    const DirSchedule& d = s->dirs[l];
+    for (int64_t p = 0; p < n; ++p)
+      if (d.in_ptr[p + 1] - d.in_ptr[p] > kin)
+        return obte_fail(OBTE_EINVAL, "obte_sched_export_slots: more inflow faces than kin slots");
+  }
+#pragma omp parallel for schedule(static)
+  for (int32_t l = 0; l < s->n_dirs; ++l) {
+    const DirSchedule& d = s->dirs[l];
+    int32_t* src = nb_src + (int64_t)l * kin * n;
+    int32_t* face = nb_face + (int64_t)l * kin * n;
+    for (int64_t p = 0; p < n; ++p) {
+      const int32_t e0 = d.in_ptr[p], e1 = d.in_ptr[p + 1];
+      for (int32_t k = 0; k < kin; ++k) {
+        const int32_t e = e0 + k;
+        if (e < e1) {
+          const int32_t v = d.in_src[e];
+          src[k * n + p] = v >= 0 ? v : (int32_t)(n + ~v);
+          face[k * n + p] = d.in_face[e];
+        } else {
+          src[k * n + p] = (int32_t)p;
+          face[k * n + p] = -1;
+        }
+      }
+    }
+  }
+  return OBTE_OK;
+}
+
 int obte_sched_destroy(void* handle) {
   delete static_cast<Schedule*>(handle);
   return OBTE_OK;

@@ -71,6 +71,10 @@ class Schedule(NamedTuple):
     cut_exit: np.ndarray
     cut_face: np.ndarray
     n_levels: np.ndarray
+    # fixed-width inflow slots [L][kin][n] (see fixed_width_inflow), exported natively
+    kin: int = 0
+    nb_src: Optional[np.ndarray] = None
+    nb_face: Optional[np.ndarray] = None
 
 
 def n_directions(mat) -> int:
@@ -223,6 +227,10 @@ def build_schedule(op: HostOperator, dirs: Optional[np.ndarray] = None, cut_cand
         cut_face = np.zeros(max(1, int(ncut.sum())), np.int32)
         _lib.call('obte_sched_export', h, _lib.ptr(order), _lib.ptr(pos), _lib.ptr(lvl_ptr), _lib.ptr(in_ptr),
                   _lib.ptr(in_src), _lib.ptr(in_face), _lib.ptr(cut_exit), _lib.ptr(cut_face))
+        kin = max(2, int(np.diff(in_ptr.reshape(L, n + 1), axis=1).max(initial=0)))
+        nb_src = np.empty((L, kin, n), np.int32)
+        nb_face = np.empty((L, kin, n), np.int32)
+        _lib.call('obte_sched_export_slots', h, kin, _lib.ptr(nb_src), _lib.ptr(nb_face))
     finally:
         _lib.call('obte_sched_destroy', h)
     lvl_off = np.concatenate([[0], np.cumsum(nlev + 1)]).astype(np.int32)
@@ -230,7 +238,7 @@ def build_schedule(op: HostOperator, dirs: Optional[np.ndarray] = None, cut_cand
     cut_off = np.concatenate([[0], np.cumsum(ncut)]).astype(np.int32)
     return Schedule(order=order, pos=pos, lvl_off=lvl_off, lvl_ptr=lvl_ptr, in_ptr=in_ptr, in_off=in_off,
                     in_src=in_src, in_face=in_face, cut_off=cut_off, cut_exit=cut_exit, cut_face=cut_face,
-                    n_levels=nlev.astype(np.int64))
+                    n_levels=nlev.astype(np.int64), kin=kin, nb_src=nb_src, nb_face=nb_face)
 
 
 def level_pair_max(sched: Schedule) -> int:
@@ -258,6 +266,8 @@ def fixed_width_inflow(sched: Schedule, n: int, kin: int):
     (a position, or n + cut index for a cut face) and its directed face.
     Unused slots point at p itself with face -1 (their coefficient is 0).
     Every row lies in [0, n + cuts of the direction)."""
+    if sched.nb_src is not None and sched.kin == kin:
+        return sched.nb_src, sched.nb_face
     L = len(sched.n_levels)
     nb_src = np.empty((L, kin, n), np.int32)
     nb_face = np.full((L, kin, n), -1, np.int32)

@@ -61,6 +61,10 @@ def test_scheduler_topological_order(max_width):
     n = op.n
     kin = max(2, max_inflow(sc))
     src, face = fixed_width_inflow(sc, n, kin)
+    # the scheduler's native slot export equals the host construction from the CSR
+    assert sc.kin == kin and sc.nb_src is src
+    src_h, face_h = fixed_width_inflow(sc._replace(nb_src=None, nb_face=None, kin=0), n, kin)
+    assert np.array_equal(src, src_h) and np.array_equal(face, face_h)
     for l in range(op.n_dirs):
         lp = sc.lvl_ptr[sc.lvl_off[l]:sc.lvl_off[l + 1]]
         levelof = np.repeat(np.arange(len(lp) - 1), np.diff(lp))
