@@ -372,6 +372,16 @@ int obte_tim_face_grad(const obte_bte_plan_t* plan, const double* t_fwd, const d
                        const int32_t* i_face, const int32_t* j_face, const int32_t* side_of_face, const double* tb,
                        const double* jump0, const double* vol, int32_t n_faces, double* grad,
                        obte_stream_t stream);
+/* The same sum restricted to directions l0 <= l < l1, whose forward
+ * solutions t_fwd_dirs hold only those directions ([l1 - l0][n_chunks][n +
+ * ncut_max][mc]); accumulate != 0 adds into grad.  Lets a caller recompute the
+ * forward solutions a block of directions at a time instead of storing all of
+ * them (same reference counterpart as obte_tim_face_grad). */
+int obte_tim_face_grad_dirs(const obte_bte_plan_t* plan, const double* t_fwd_dirs, int32_t l0, int32_t l1,
+                            int32_t accumulate, const double* t_adj, const int32_t* lbar, const int32_t* i_face,
+                            const int32_t* j_face, const int32_t* side_of_face, const double* tb,
+                            const double* jump0, const double* vol, int32_t n_faces, double* grad,
+                            obte_stream_t stream);
 
 /* ---- material interpolation ------------------------------------------------
  *

@@ -104,6 +104,8 @@ _SIGNATURES = {
     'obte_batch_reduce': [C.POINTER(Batch), _p],
     'obte_batch_log_active': [C.POINTER(Batch), _p, _p],
     'obte_tim_face_grad': [C.POINTER(BtePlan), _p, _p, _p, _p, _p, _p, _p, _p, _p, C.c_int32, _p, _p],
+    'obte_tim_face_grad_dirs': [C.POINTER(BtePlan), _p, _i32, _i32, _i32, _p, _p, _p, _p, _p, _p, _p, _p,
+                                C.c_int32, _p, _p],
     'obte_rta_scatter': [C.c_int32, C.c_int32, _p, _p, _p, C.c_int32, _p, C.c_int32, _p, C.c_int32, _p, _p, _p, _p],
     'obte_bte_reduce': [C.POINTER(BtePlan), _p, _p, _p, _p, _p],
     'obte_fourier_solve': [C.POINTER(FourierPlan), _p, _p, _p, _p, _p, _f64, _i32, _i32,

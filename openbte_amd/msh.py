@@ -16,9 +16,14 @@ Semantics kept from the reference:
     (openbte/mesh.py:127-145);
   * a periodic pair is the side regions ``<name>_a`` / ``<name>_b``; the
     ``_b`` side is kept and the ``_a`` side discarded (openbte/mesh.py:238-257).
-    Sides are paired by the translation between the two regions, which is what
-    the node pairs of the ``$Periodic`` section encode for translational
-    periodicity; that section is therefore not needed and is skipped.
+    The reference pairs two sides when they hold the two nodes of a
+    ``$Periodic`` node pair and their centroids differ by that pair's offset
+    (openbte/mesh.py:205-236).  Here the offsets of a ``$Periodic`` entity's
+    node pairs give its translation (they must all agree: only translational
+    periodicity is read), sides are paired by that translation, and every
+    node pair must then join two sides so paired.  Without a ``$Periodic``
+    section the translation is the difference of the two regions' mean side
+    centroids.
 
 ``write_msh`` writes the same format (nodes, physical names, bulk and
 boundary elements with their physical tags) so that generated meshes can be
@@ -55,9 +60,69 @@ def _sections(text: str) -> Dict[str, List[str]]:
     return out
 
 
+def _periodic_entities(lines: List[str], dim: int):
+    """Entities of a ``$Periodic`` section: (slave entity, master entity,
+    node pairs [k][2] as (slave node id, master node id))."""
+    out = []
+    n_ent = int(lines[0].split()[0])
+    i = 1
+    for _ in range(n_ent):
+        d, slave, master = (int(x) for x in lines[i].split()[:3])
+        i += 1
+        if lines[i].strip().startswith('Affine'):
+            i += 1
+        k = int(lines[i].split()[0])
+        pairs = np.array(' '.join(lines[i + 1:i + 1 + k]).split(), dtype=np.int64).reshape(k, 2)
+        i += 1 + k
+        if d == dim - 1:
+            out.append((slave, master, pairs))
+    return out
+
+
+def _periodic_from_section(lines, dim, nodes, index, face_nodes, face_tag, face_entity, region_names, tags):
+    """Translation of every periodic region (``X_a`` onto ``X_b``) from the
+    node pairs of the ``$Periodic`` section."""
+    name_of_tag = dict(zip(tags, region_names))
+    periodic: Dict[str, np.ndarray] = {}
+    pairs_by_base: Dict[str, List[np.ndarray]] = {}
+    for slave, master, pairs in _periodic_entities(lines, dim):
+        ps = {name_of_tag[t] for t in face_tag[face_entity == slave].tolist()}
+        if len(ps) != 1:
+            raise ValueError(f'periodic entity {slave}: its faces carry {len(ps)} physical regions')
+        sname = ps.pop()
+        if not (sname.endswith('_a') or sname.endswith('_b')):
+            continue
+        off = nodes[index[pairs[:, 0]]] - nodes[index[pairs[:, 1]]]   # slave - master
+        shift = off.mean(axis=0)
+        if np.abs(off - shift).max() > 1e-9 * max(1.0, np.abs(nodes).max()):
+            raise ValueError(f'periodic entity {slave}: node pairs are not one translation')
+        base = sname[:-2]
+        t = shift if sname.endswith('_b') else -shift               # X_a onto X_b
+        if base in periodic and np.abs(periodic[base] - t).max() > 1e-9 * max(1.0, np.abs(nodes).max()):
+            raise ValueError(f'periodic region {base}: entities disagree on the translation')
+        periodic[base] = t
+        pairs_by_base.setdefault(base, []).append(pairs)
+    return periodic, pairs_by_base
+
+
+def _check_node_pairs(mesh: Mesh, nodes, index, pairs_by_base) -> None:
+    """Every node pair of the ``$Periodic`` section joins a kept side and its
+    discarded partner: the pairing the reference derives from the same pairs."""
+    for base, plist in pairs_by_base.items():
+        kept = np.asarray(mesh.periodic_sides.get(base, []), np.int64)
+        if len(kept) == 0:
+            raise ValueError(f'periodic region {base}: no side pairs')
+        kept_nodes = set(np.asarray(mesh.sides)[kept].reshape(-1).tolist())
+        for pairs in plist:
+            a, b = index[pairs[:, 0]], index[pairs[:, 1]]
+            if not all((x in kept_nodes) or (y in kept_nodes) for x, y in zip(a.tolist(), b.tolist())):
+                raise ValueError(f'periodic region {base}: a node pair joins no paired sides')
+
+
 def read_msh(filename: str, periodic: Optional[Dict[str, np.ndarray]] = None) -> Mesh:
     """Read a gmsh 2.2 ASCII mesh.  ``periodic`` optionally gives the
-    translation of each periodic region (``X_a`` onto ``X_b``); by default it is
+    translation of each periodic region (``X_a`` onto ``X_b``); by default it
+    comes from the ``$Periodic`` node pairs, or, without that section, from
     the difference of the two regions' mean side centroids."""
     if not filename.endswith('.msh'):
         filename = filename + '.msh'
@@ -98,6 +163,7 @@ def read_msh(filename: str, periodic: Optional[Dict[str, np.ndarray]] = None) ->
     face_nodes = (np.sort(index[np.array([r[3 + r[2]:] for r in face_rows])], axis=1)
                   if face_rows else np.zeros((0, dim), np.int64))
     face_tag = np.array([r[3] if r[2] > 0 else 0 for r in face_rows], np.int64)
+    face_entity = np.array([r[4] if r[2] > 1 else -1 for r in face_rows], np.int64)
 
     # boundary side regions by node tuple
     tags = sorted(set(face_tag.tolist()))
@@ -107,6 +173,10 @@ def read_msh(filename: str, periodic: Optional[Dict[str, np.ndarray]] = None) ->
     def side_region(side_nodes, centroids):
         return np.array([lookup.get(tuple(sorted(s)), -1) for s in np.asarray(side_nodes).tolist()], np.int64)
 
+    pairs_by_base = {}
+    if periodic is None and 'Periodic' in sec:
+        periodic, pairs_by_base = _periodic_from_section(sec['Periodic'], dim, nodes, index, face_nodes,
+                                                         face_tag, face_entity, region_names, tags)
     # periodic translations from the _a / _b regions
     if periodic is None:
         periodic = {}
@@ -120,7 +190,9 @@ def read_msh(filename: str, periodic: Optional[Dict[str, np.ndarray]] = None) ->
     elem_regions = {}
     for t in sorted(set(bulk_tag.tolist())):
         elem_regions[names.get(t, (dim, str(t)))[1]] = np.nonzero(bulk_tag == t)[0]
-    return build_mesh(nodes, elems, side_region, region_names, periodic=periodic, elem_regions=elem_regions)
+    mesh = build_mesh(nodes, elems, side_region, region_names, periodic=periodic, elem_regions=elem_regions)
+    _check_node_pairs(mesh, nodes, index, pairs_by_base)
+    return mesh
 
 
 def get_mesh(filename: str = 'mesh') -> Mesh:
@@ -186,3 +258,25 @@ def write_msh(mesh: Mesh, filename: str) -> None:
         for i, l in enumerate(lines):
             f.write(f'{i + 1} {l}\n')
         f.write('$EndElements\n')
+        # node pairs of every periodic region, slave = X_b (kept), master = X_a,
+        # entity tags = the physical tags written above
+        ent = []
+        tag_of = {name: k + 1 for k, (d, name, _) in enumerate(regions)}
+        for name, kept in mesh.periodic_sides.items():
+            kept = np.asarray(kept)
+            part = np.asarray(mesh.side_elem_map)[kept, 1]
+            a_sides = [k for k, (d, nm, _) in enumerate(regions) if nm == name + '_a'][0]
+            a_nodes = np.unique(np.asarray(mesh.sides)[regions[a_sides][2]].reshape(-1))
+            b_nodes = np.unique(np.asarray(mesh.sides)[kept].reshape(-1))
+            shift = mesh.side_centroids[kept].mean(axis=0) - \
+                mesh.side_centroids[regions[a_sides][2]].mean(axis=0) if len(part) else np.zeros(dim)
+            key = {tuple(np.round(mesh.nodes[i] + shift, 9).tolist()): i for i in a_nodes.tolist()}
+            pairs = [(i, key[tuple(np.round(mesh.nodes[i], 9).tolist())]) for i in b_nodes.tolist()]
+            ent.append((tag_of[name + '_b'], tag_of[name + '_a'], pairs))
+        if ent:
+            f.write(f'$Periodic\n{len(ent)}\n')
+            for slave, master, pairs in ent:
+                f.write(f'{dim - 1} {slave} {master}\n{len(pairs)}\n')
+                for i, j in pairs:
+                    f.write(f'{i + 1} {j + 1}\n')
+            f.write('$EndPeriodic\n')

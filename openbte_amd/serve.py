@@ -19,6 +19,14 @@ The device loop works in rounds.  Each round
 3. applies each problem's own stop rule and retires the finished ones, whose
    memory and slot go to the next requests.
 
+A request may carry a deadline (``timeout_s`` after submission) and may be
+cancelled (``BTEServer.cancel``) at any time.  Both take effect at the next
+round: the request ends with ``RequestTimeout`` or ``RequestCancelled`` in
+``error`` and no result, and its slot and memory are released; the problems
+that share the device with it are untouched (each problem's state and
+reductions are its own).  The reference has no such path: a worker that dies
+or hangs blocks its barrier forever (openbte/objects.py:21,88-101).
+
 A problem therefore follows exactly the iteration it would follow alone, and
 every kernel treats problems independently (fixed-order reductions, per-problem
 counters), so its result does not depend on what else is in the batch.
@@ -27,7 +35,7 @@ from __future__ import annotations
 
 import itertools
 import time
-from concurrent.futures import ThreadPoolExecutor
+from concurrent.futures import CancelledError, ThreadPoolExecutor
 from dataclasses import dataclass, field
 from typing import Any, Dict, List, Optional
 
@@ -55,6 +63,7 @@ class Request:
     bte_max_iter: int = 100
     bte_rtol: float = 1e-4
     tag: Any = None
+    timeout_s: Optional[float] = None   # deadline after submission (None: none)
     # filled in by the server
     id: int = -1
     t_submit: float = 0.0
@@ -65,10 +74,45 @@ class Request:
     updates: int = 0
     result: Optional[SolverResults] = None
     error: Optional[BaseException] = None
+    cancelled: bool = False
 
     @property
     def latency(self) -> float:
         return self.t_done - self.t_submit
+
+
+class RequestTimeout(TimeoutError):
+    """A request passed its ``timeout_s`` before its solve finished."""
+
+
+class RequestCancelled(CancelledError):
+    """A request was cancelled before its solve finished."""
+
+
+def ending_error(req: Request, now: float) -> Optional[BaseException]:
+    """The error that ends ``req`` now, if any: cancellation first, then its
+    deadline (``timeout_s`` after submission; 0 expires at the next round)."""
+    if req.cancelled:
+        return RequestCancelled(f'request {req.id} cancelled')
+    if req.timeout_s is not None and now - req.t_submit >= req.timeout_s:
+        return RequestTimeout(f'request {req.id} passed its {req.timeout_s:g} s deadline')
+    return None
+
+
+def plan_admission(sizes: List[int], n_free_slots: int, used: int, budget: int):
+    """First come, first served admission by device footprint: returns the
+    indices to admit now, those that can never fit (larger than the whole
+    budget) and those that wait for memory or a slot."""
+    admit, never, wait = [], [], []
+    for i, b in enumerate(sizes):
+        if b > budget:
+            never.append(i)
+        elif len(admit) < n_free_slots and used + b <= budget:
+            admit.append(i)
+            used += b
+        else:
+            wait.append(i)
+    return admit, never, wait
 
 
 @dataclass
@@ -93,6 +137,7 @@ class _Live:
 class ServerStats:
     submitted: int = 0
     completed: int = 0
+    failed: int = 0          # ended without a result (preparation error, memory, timeout, cancellation)
     rounds: int = 0
     updates: int = 0
     busy_s: float = 0.0
@@ -168,6 +213,11 @@ class BTEServer:
         req.t_ready = time.perf_counter()
         return _Prepared(req, op, sched, t0)
 
+    def cancel(self, req: Request) -> None:
+        """Cancel a submitted request; it ends at the next round (a request
+        still being prepared ends when its preparation returns)."""
+        req.cancelled = True
+
     def _collect_prepared(self) -> None:
         still = []
         for fut, req in self.preparing:
@@ -181,24 +231,54 @@ class BTEServer:
                 self.ready.append(p)
         self.preparing = still
 
+    def _reap(self) -> bool:
+        """End the prepared and admitted requests that were cancelled or
+        passed their deadline; returns True when a live problem ended."""
+        now = time.perf_counter()
+        keep = []
+        for p in self.ready:
+            err = ending_error(p.req, now)
+            if err is None:
+                keep.append(p)
+            else:
+                p.req.error = err
+                self._finish_failed(p.req)
+        self.ready = keep
+        keep, ended = [], False
+        for x in self.live:
+            err = ending_error(x.req, now)
+            if err is None:
+                keep.append(x)
+                continue
+            x.req.error = err
+            self._release(x)
+            self._finish_failed(x.req)
+            ended = True
+        self.live = keep
+        if ended:
+            self._dirty = True
+            self._batch = None  # drop the view's references to the released devices
+        return ended
+
     def _finish_failed(self, req: Request) -> None:
         req.t_done = time.perf_counter()
         self.done.append(req)
         self.stats.completed += 1
+        self.stats.failed += 1
 
     # -- admission -----------------------------------------------------------
     def _admit(self) -> None:
-        keep = []
         # first come first served, skipping a request that does not fit yet
-        for p in self.ready:
+        admit, never, wait = plan_admission([p.req.bytes for p in self.ready], len(self.free_slots),
+                                            self.used_bytes, self.hbm_budget)
+        for i in never:
+            req = self.ready[i].req
+            req.error = MemoryError(f'request needs {req.bytes} bytes, budget is {self.hbm_budget}')
+            self._finish_failed(req)
+        keep = [self.ready[i] for i in wait]
+        for i in admit:
+            p = self.ready[i]
             req = p.req
-            if req.bytes > self.hbm_budget:
-                req.error = MemoryError(f'request needs {req.bytes} bytes, budget is {self.hbm_budget}')
-                self._finish_failed(req)
-                continue
-            if not self.free_slots or self.used_bytes + req.bytes > self.hbm_budget:
-                keep.append(p)
-                continue
             slot = self.free_slots.pop(0)
             dev = BTEDevice(p.op, sched=p.sched, cfg=SweepConfig(**vars(self.cfg)), device=self.dev,
                             n_active=self.n_active[slot:slot + 1], norm2=self.norm2[slot:slot + 1])
@@ -223,6 +303,7 @@ class BTEServer:
         """Admit, run one outer iteration of every admitted problem, retire
         the finished ones; returns the requests completed in this round."""
         self._collect_prepared()
+        self._reap()
         self._admit()
         if not self.live:
             return []
@@ -249,6 +330,7 @@ class BTEServer:
             for x in finished:
                 self._retire(x)
             self._dirty = True
+            self._batch = None
         self.live = keep
         self.stats.busy_s += time.perf_counter() - t
         return [x.req for x in finished]
@@ -269,15 +351,19 @@ class BTEServer:
                                                 'Flux_BTE': {'data': J * 1e9, 'units': 'W/m/m'}},
                                      aux={'suppression': S, 'kappa_eff': kappa, 'n_iter': x.n_iter})
         x.req.t_done = time.perf_counter()
+        self._release(x)
+        self.done.append(x.req)
+        self.stats.completed += 1
+        self.stats.latencies.append(x.req.latency)
+
+    def _release(self, x: _Live) -> None:
+        """Give a live problem's slot and device memory back."""
         self.n_active[x.slot:x.slot + 1].zero_()
         self.norm2[x.slot:x.slot + 1].zero_()
         self.free_slots.append(x.slot)
         self.free_slots.sort()
         self.used_bytes -= x.req.bytes
         x.dev = None
-        self.done.append(x.req)
-        self.stats.completed += 1
-        self.stats.latencies.append(x.req.latency)
 
     def wait_prepared(self, poll_s: float = 0.001) -> None:
         """Block until every submitted request has finished host preparation."""

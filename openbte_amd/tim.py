@@ -115,10 +115,20 @@ class TIMSolver:
     openbte/bte.py:193-256 to a relative residual ``rtol`` (far tighter than
     the reference's stop rule, so that gradients and finite differences are
     meaningful), with the interface Krylov solve at ``cfg.rtol`` (default here
-    1e-13).  Successive solves start from the previous solution."""
+    1e-13).  Successive solves start from the previous solution.
+
+    The gradient needs every forward mode solution T_ml next to the adjoint
+    ones.  By default (``store_forward=True``) the forward solve keeps a copy
+    of its solution workspace (L x M x n doubles).  With
+    ``store_forward=False`` it keeps only its converged state (DeltaT, TB)
+    and interface values, and the gradient recomputes the forward and adjoint
+    solutions ``dir_block`` directions at a time by one final sweep each:
+    extra memory ``dir_block / L`` of a workspace instead of a whole one, for
+    two extra sweeps in total."""
 
     def __init__(self, geo, mat, bcs, eff, pixel_of_cell: np.ndarray, cfg: Optional[SweepConfig] = None,
-                 rtol: float = 1e-12, max_iter: int = 3000, restart: int = 80):
+                 rtol: float = 1e-12, max_iter: int = 3000, restart: int = 80, store_forward: bool = True,
+                 dir_block: int = 8):
         if not torch.cuda.is_available():
             raise RuntimeError('openbte_amd.TIMSolver needs a ROCm GPU; there is no CPU fallback')
         self.geo = geo
@@ -161,7 +171,10 @@ class TIMSolver:
         self.heat0 = torch.zeros_like(d.buf['heat'])
         self.nd = len(op.i_diffuse)
         self.restart = int(restart)
+        self.store_forward = bool(store_forward)
+        self.dir_block = max(1, int(dir_block))
         self.x_fwd = self.x_adj = None
+        self.t_fwd = self.y_fwd = None
         self.n_iter = 0
         self.kappa = None
         self.rho = None
@@ -265,7 +278,13 @@ class TIMSolver:
         self.x_fwd = self._solve(self.x_fwd)
         self.n_iter = self.evals
         self.kappa = float(d.buf['kappa'].sum().item())
-        self.t_fwd = d.buf['t'].clone()
+        if self.store_forward:
+            self.t_fwd = d.buf['t'].clone()
+        else:
+            # the final sweep's interface values: with the converged state they
+            # reproduce every forward solution exactly (obte_bte_sweep mode 2)
+            self.t_fwd = None
+            self.y_fwd = d.gbuf['y'].clone()
         self.tb_fwd = self.x_fwd[self.op.n:].clone()
         return self.kappa
 
@@ -292,15 +311,39 @@ class TIMSolver:
     def adjoint(self) -> None:
         d = self.dev
         P = d.plan
-        d.buf['pert'].copy_(self._adjoint_source())
+        self.adj_src = self._adjoint_source()
+        d.buf['pert'].copy_(self.adj_src)
         heat_ptr = P.heat
         P.heat = self.heat0.data_ptr()
         try:
             self.x_adj = self._solve(self.x_adj)
             self.n_iter_adj = self.evals
+            self.y_adj = d.gbuf['y'].clone()
         finally:
             P.heat = heat_ptr
             _lib.call('obte_bte_assemble', P, d.stream)     # restores the forward perturbation
+
+    def _resweep(self, x: torch.Tensor, y: torch.Tensor, dirs: np.ndarray, adjoint: bool) -> None:
+        """Recompute the final-sweep solutions of directions ``dirs`` (all MFP
+        chunks) into the workspace from a converged state ``x`` and interface
+        values ``y``; the other directions' rows are left as they are."""
+        d = self.dev
+        P = d.plan
+        active = torch.zeros(self.op.n_dirs, d.n_chunks, dtype=torch.int32)
+        active[torch.as_tensor(np.asarray(dirs, np.int64))] = 1
+        active = active.to(d.dev)
+        heat_ptr = P.heat
+        if adjoint:
+            d.buf['pert'].copy_(self.adj_src)
+            P.heat = self.heat0.data_ptr()
+        try:
+            self._set_state(x)
+            _lib.call('obte_bte_source', P, d.stream)
+            _lib.call('obte_bte_sweep', P, 2, y.data_ptr(), d.gbuf['x'].data_ptr(), active.data_ptr(), 0, d.stream)
+        finally:
+            if adjoint:
+                P.heat = heat_ptr
+                _lib.call('obte_bte_assemble', P, d.stream)
 
     def gradient(self) -> np.ndarray:
         """d kappa_eff / d rho for the densities of the last ``forward``."""
@@ -310,26 +353,47 @@ class TIMSolver:
         d, op = self.dev, self.op
         F = len(op.i_mat)
         grad_f = torch.zeros(F, dtype=torch.float64, device=d.dev)
-        _lib.call('obte_tim_face_grad', d.plan, self.t_fwd.data_ptr(), d.buf['t'].data_ptr(),
-                  self.lbar_d.data_ptr(), self.i_face.data_ptr(), self.j_face.data_ptr(),
-                  self.side_of_face.data_ptr(), self.tb_fwd.data_ptr(), self.jump0_d.data_ptr(),
-                  self.vol.data_ptr(), F, grad_f.data_ptr(), d.stream)
+        args = (self.lbar_d.data_ptr(), self.i_face.data_ptr(), self.j_face.data_ptr(), self.side_of_face.data_ptr(),
+                self.tb_fwd.data_ptr(), self.jump0_d.data_ptr(), self.vol.data_ptr(), F, grad_f.data_ptr(), d.stream)
+        if self.store_forward:
+            _lib.call('obte_tim_face_grad', d.plan, self.t_fwd.data_ptr(), d.buf['t'].data_ptr(), *args)
+            contact = self._contact_term(self.t_fwd, 0, op.n_dirs)
+        else:
+            # forward and adjoint solutions recomputed a block of directions at
+            # a time: forward rows of the block are copied out, then the
+            # adjoint rows of the reversed block are swept in place
+            L = op.n_dirs
+            rows = (op.n + d.ncut_max) * d.n_chunks * d.cfg.mc
+            tw = d.buf['t'].view(L, rows)
+            blk = torch.empty(min(self.dir_block, L), rows, dtype=torch.float64, device=d.dev)
+            contact = np.zeros(F // 2)
+            for l0 in range(0, L, self.dir_block):
+                l1 = min(L, l0 + self.dir_block)
+                self._resweep(self.x_fwd, self.y_fwd, np.arange(l0, l1), adjoint=False)
+                blk[:l1 - l0].copy_(tw[l0:l1])
+                self._resweep(self.x_adj, self.y_adj, self.lbar[l0:l1], adjoint=True)
+                _lib.call('obte_tim_face_grad_dirs', d.plan, blk.data_ptr(), l0, l1, int(l0 > 0),
+                          d.buf['t'].data_ptr(), *args)
+                contact += self._contact_term(blk[:l1 - l0], l0, l1)
         gf = grad_f.cpu().numpy()
         nint = F // 2
         g_side = gf[:nint] + gf[nint:]
-        g_side += self._contact_term()
+        g_side += contact
         grad = np.zeros(self.n_pixels)
         np.add.at(grad, self.pa, g_side * self.dt_da)
         np.add.at(grad, self.pb, g_side * self.dt_db)
         return grad
 
-    def _contact_term(self) -> np.ndarray:
-        """d kappa / d t of the contact sides through the flux formula itself."""
+    def _contact_term(self, t_fwd: torch.Tensor, l0: int, l1: int) -> np.ndarray:
+        """d kappa / d t of the contact sides through the flux formula itself,
+        from the forward solutions of directions l0 <= l < l1 (``t_fwd`` holds
+        those directions only)."""
         op, geo, d = self.op, self.geo, self.dev
-        n, L, M, mc, nch = op.n, op.n_dirs, op.n_mfp, d.cfg.mc, d.n_chunks
+        n, M, mc, nch = op.n, op.n_mfp, d.cfg.mc, d.n_chunks
+        L = l1 - l0
         rows = n + d.ncut_max
-        T = self.t_fwd.view(L, nch, rows, mc)
-        pos = torch.as_tensor(self.sched.pos.reshape(L, n).astype(np.int64), device=d.dev)
+        T = t_fwd.reshape(L, nch, rows, mc)
+        pos = torch.as_tensor(self.sched.pos.reshape(op.n_dirs, n)[l0:l1].astype(np.int64), device=d.dev)
         e1 = torch.as_tensor(op.ce1.astype(np.int64), device=d.dev)
         e2 = torch.as_tensor(op.ce2.astype(np.int64), device=d.dev)
         li = torch.arange(L, device=d.dev)[:, None]
@@ -338,7 +402,7 @@ class TIMSolver:
         T1 = T1.reshape(L, len(op.ce1), nch * mc)[:, :, :M].cpu().numpy()
         T2 = T2.reshape(L, len(op.ce1), nch * mc)[:, :, :M].cpu().numpy()
         na = np.asarray(geo.normal_areas)[self.c_sides]
-        sn = np.einsum('mlj,cj->lcm', op.sigma, na)      # (L, nc, M)
+        sn = np.einsum('mlj,cj->lcm', op.sigma[:, l0:l1], na)      # (L, nc, M)
         per = (np.clip(sn, 0, None) * T1 + np.clip(sn, None, 0) * (T2 + op.contact_jump)).sum(axis=(0, 2))
         out = np.zeros(len(op.i_mat) // 2)
         side_index = {s: k for k, s in enumerate(np.asarray(geo.internal))}

@@ -211,3 +211,45 @@ def dense_kappa(T: TimOps) -> float:
             o = idx(l, m)
             kappa += _kappa_ml(T, l, m, x[o:o + n])
     return kappa
+
+
+def reduced_kappa(T: TimOps) -> float:
+    """Converged kappa of the TIM model from an exact solve of the outer
+    iteration's fixed point reduced to its coupling variables x = (DeltaT,
+    TB): every mode's solution is T_ml = A_ml^-1 (rhs_ml(0) + R_ml x), so
+    x = M x + f with M and f formed from one multi-right-hand-side SuperLU
+    solve per mode, and (I - M) x = f is solved densely (least squares: the
+    periodic problem fixes DeltaT only up to a constant, which kappa does not
+    see).  No iteration error; scales to full mode grids on small meshes
+    (pinned against ``dense_kappa`` on tiny ones)."""
+    ops = T.ops
+    n, M, L = ops.n_elems, ops.n_mfp, ops.n_dirs
+    nd = len(ops.i_diffuse)
+    N = n + nd
+    Mx = np.zeros((N, N))
+    f = np.zeros(N)
+    zero_n, zero_d = np.zeros(n), np.zeros(nd)
+    lus = {}
+    for l in range(L):
+        for m in range(M):
+            lu = lus[(l, m)] = spla.splu(bo.system_matrix(ops, l, m))
+            lam = ops.mfps[m]
+            R = np.zeros((n, N))
+            R[:, :n] = np.eye(n)
+            if nd:
+                # d rhs / d TB_s = lam * w_s * (-Gbm2_ls) at cell i_d[s] (accumulated)
+                np.add.at(R, (ops.i_diffuse, n + np.arange(nd)), lam * (-ops.Gbm2[l] * T.weight))
+            X = lu.solve(R)
+            x0 = lu.solve(_rhs(T, l, m, zero_n, zero_d))
+            Mx[:n] += ops.t_coeff[m, l] * X
+            f[:n] += ops.t_coeff[m, l] * x0
+            if nd:
+                Mx[n:] += -ops.GG[m, l][:, None] * X[ops.i_diffuse]
+                f[n:] += -ops.GG[m, l] * x0[ops.i_diffuse]
+    x = np.linalg.lstsq(np.eye(N) - Mx, f, rcond=None)[0]
+    DeltaT, TB = x[:n], x[n:]
+    kappa = 0.0
+    for l in range(L):
+        for m in range(M):
+            kappa += _kappa_ml(T, l, m, lus[(l, m)].solve(_rhs(T, l, m, DeltaT, TB)))
+    return kappa

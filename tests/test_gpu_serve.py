@@ -199,3 +199,36 @@ def test_full_size_mix_served_equals_alone():
         assert r.result.kappa_eff == a.kappa_eff
         assert np.array_equal(r.result.variables['Temperature_BTE']['data'], a.variables['Temperature_BTE']['data'])
         assert np.array_equal(r.result.variables['Flux_BTE']['data'], a.variables['Flux_BTE']['data'])
+
+
+def test_cancel_timeout_and_failure_leave_others_unchanged(alone):
+    """Host-forced endings: one request cancelled while it runs, one whose
+    deadline passes while it runs (timeout_s set to 0 after admission), one
+    whose preparation fails (unknown contact name); the other two finish with
+    the results they have alone, and every slot and byte is released."""
+    from openbte_amd.bte import SweepConfig
+    from openbte_amd.objects import EffectiveThermalConductivity
+    from openbte_amd.serve import BTEServer, Request, RequestCancelled, RequestTimeout
+    probs, fs, cfg, ref = alone
+    srv = BTEServer(cfg=SweepConfig(**vars(cfg)), max_slots=8)
+    reqs = [Request(g, m, b, e, fourier=f, bte_max_iter=12, bte_rtol=1e-6) for (g, m, b, e), f in zip(probs, fs)]
+    g, m, b, _ = probs[0]
+    bad = Request(g, m, b, EffectiveThermalConductivity(contact='NoSuchRegion', normalization=-1.0), fourier=fs[0])
+    for r in reqs + [bad]:
+        srv.submit(r)
+    srv.wait_prepared()
+    srv.step()
+    srv.step()
+    assert len(srv.live) == 4
+    srv.cancel(reqs[1])
+    reqs[2].timeout_s = 0.0
+    srv.run_until_idle()
+    srv.close()
+    assert isinstance(reqs[1].error, RequestCancelled) and reqs[1].result is None
+    assert isinstance(reqs[2].error, RequestTimeout) and reqs[2].result is None
+    assert bad.error is not None and bad.result is None
+    for k in (0, 3):
+        assert reqs[k].error is None
+        _same(reqs[k].result, ref[k])
+    assert srv.used_bytes == 0 and len(srv.free_slots) == 8
+    assert srv.stats.failed == 3 and srv.stats.completed == 5

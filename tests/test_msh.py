@@ -73,3 +73,125 @@ $EndElements
     s = g.periodic_sides['Periodic_x'][0]
     assert np.isclose(g.side_centroids[s][0], 1.0)
     assert g.side_elem_map[s][0] != g.side_elem_map[s][1]
+
+
+_PERIODIC_SQUARE = """$MeshFormat
+2.2 0 8
+$EndMeshFormat
+$PhysicalNames
+4
+1 10 "Periodic_x_a"
+1 11 "Periodic_x_b"
+1 12 "Boundary"
+2 20 "Bulk"
+$EndPhysicalNames
+$Nodes
+6
+1 0 0 0
+2 1 0 0
+3 1 1 0
+4 0 1 0
+5 0 2 0
+6 1 2 0
+$EndNodes
+$Elements
+10
+1 1 2 10 101 4 1
+2 1 2 10 101 5 4
+3 1 2 11 102 2 3
+4 1 2 11 102 3 6
+5 1 2 12 103 1 2
+6 1 2 12 104 6 5
+7 2 2 20 1 1 2 3
+8 2 2 20 1 1 3 4
+9 2 2 20 1 4 3 6
+10 2 2 20 1 4 6 5
+$EndElements
+$Periodic
+1
+1 102 101
+3
+2 1
+3 4
+6 5
+$EndPeriodic
+"""
+
+
+def _reference_pairs(text):
+    """The reference's pairing rule (openbte/mesh.py:222-257) restated on the
+    file's data: for a node pair (slave, master) with offset L, a side holding
+    the slave node and one holding the master node are partners when their
+    centroids differ by L; the side of the _b region is kept."""
+    import re
+    nodes = {int(l.split()[0]): np.array(l.split()[1:3], float)
+             for l in text.split('$Nodes\n')[1].split('$EndNodes')[0].strip().splitlines()[1:]}
+    faces = []
+    for l in text.split('$Elements\n')[1].split('$EndElements')[0].strip().splitlines()[1:]:
+        r = [int(x) for x in l.split()]
+        if r[1] == 1:
+            faces.append((r[3], frozenset(r[5:7])))
+    name = {10: 'a', 11: 'b'}
+    per = text.split('$Periodic\n')[1].split('$EndPeriodic')[0].strip().splitlines()
+    pairs = [tuple(int(x) for x in l.split()) for l in per[3:]]
+    out = set()
+    cen = lambda f: np.mean([nodes[i] for i in f], axis=0)  # noqa: E731
+    for n0, n1 in pairs:
+        L = nodes[n0] - nodes[n1]
+        for t1, f1 in faces:
+            for t2, f2 in faces:
+                if n0 in f1 and n1 in f2 and np.allclose(cen(f1), cen(f2) + L):
+                    kept, disc = (f1, f2) if name.get(t1) == 'b' else (f2, f1)
+                    out.add((kept, disc))
+    return out
+
+
+def test_periodic_section_pairs_sides_like_the_reference(tmp_path):
+    """A hand-written file with a $Periodic section (gmsh entity tags, node
+    pairs slave -> master): the kept sides and their partner elements are
+    those the reference's node-pair rule gives, and the translation comes from
+    the node pairs."""
+    fn = tmp_path / 'per.msh'
+    fn.write_text(_PERIODIC_SQUARE)
+    g = read_msh(str(fn))
+    expect = _reference_pairs(_PERIODIC_SQUARE)
+    assert len(expect) == 2
+    kept = np.asarray(g.periodic_sides['Periodic_x'])
+    assert len(kept) == 2
+    elems = [frozenset((np.asarray(e) + 1).tolist()) for e in g.elems]
+    for s in kept:
+        ks = frozenset((np.asarray(g.sides[s]) + 1).tolist())
+        (disc,) = [d for k, d in expect if k == ks]
+        e0, e1 = g.side_elem_map[s]
+        assert ks <= elems[e0] and disc <= elems[e1]
+    assert np.allclose(g.dists[kept][:, 0], g.elem_centroids[g.side_elem_map[kept, 1], 0] + 1.0
+                       - g.elem_centroids[g.side_elem_map[kept, 0], 0])
+
+
+def test_periodic_section_errors(tmp_path):
+    """Node pairs that are not one translation, or a translation that pairs
+    no sides, are refused."""
+    bad = _PERIODIC_SQUARE.replace('6 5\n$EndPeriodic', '6 4\n$EndPeriodic')
+    fn = tmp_path / 'bad.msh'
+    fn.write_text(bad)
+    with pytest.raises(ValueError):
+        read_msh(str(fn))
+    shifted = _PERIODIC_SQUARE.replace('2 1\n3 4\n6 5', '3 1\n6 4\n3 1')
+    fn.write_text(shifted)
+    with pytest.raises(ValueError):
+        read_msh(str(fn))
+
+
+@pytest.mark.parametrize("geo", [pore_cell_2d(10.0, 0.25, 800, seed=1), pore_slab_3d(10.0, 3.0, 0.2, 900, seed=2, nz=3)])
+def test_written_periodic_section(tmp_path, geo):
+    """write_msh writes the node pairs of every periodic region; reading the
+    file with them and with the section removed gives the same mesh."""
+    fn = str(tmp_path / 'm.msh')
+    write_msh(geo, fn)
+    text = open(fn).read()
+    assert '$Periodic' in text
+    stripped = tmp_path / 'n.msh'
+    stripped.write_text(text.split('$Periodic')[0])
+    g1, g2 = read_msh(fn), read_msh(str(stripped))
+    for f in FIELDS:
+        assert np.array_equal(np.asarray(getattr(g1, f)), np.asarray(getattr(g2, f))), f

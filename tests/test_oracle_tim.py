@@ -92,3 +92,17 @@ def test_energy_conservation_across_interfaces():
     # orient normals along +x
     f_inner = plane_flux(inner, 0.0) * np.sign(geo.normal_areas[inner[0], 0])
     assert abs(f_inner / f_contact - 1) < 1e-9
+
+
+def test_reduced_solve_equals_dense_solve():
+    """The reduced (DeltaT, TB) fixed-point solve equals the dense coupled
+    solve and the converged Jacobi iteration (so it can stand in for them at
+    mode counts the dense solve cannot reach)."""
+    geo, mat, bcs, eff, pix, rho = _case(N=3, M=2, L=8, seed=5)
+    t, _, _, _, _, itf = to.side_transmission(geo, pix, rho)
+    T = to.assemble_tim(geo, mat, bcs, eff, t, itf)
+    kd = to.dense_kappa(T)
+    kr = to.reduced_kappa(T)
+    assert abs(kr / kd - 1) < 1e-11
+    k, _, _, _ = to.iterate(T, np.zeros(geo.n_elems), 2000, 1e-15)
+    assert abs(kr / k - 1) < 1e-10
