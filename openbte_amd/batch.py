@@ -310,10 +310,16 @@ class BTEBatch:
 
 
 def BTE_RTA_batch(problems, bte_max_iter: int = 100, bte_rtol: float = 1e-4,
-                  config: Optional[SweepConfig] = None) -> List[SolverResults]:
+                  config: Optional[SweepConfig] = None, autotune: bool = True,
+                  graphs: bool = True) -> List[SolverResults]:
     """Solve several problems together.  ``problems`` is a sequence of dicts
     with the ``BTE_RTA`` arguments (geo, mat, bcs, fourier,
-    effective_thermal_conductivity, heat_source)."""
+    effective_thermal_conductivity, heat_source).  Without ``config`` the
+    launch shape is picked by ``tune.tune`` on these problems (memoised per
+    problem signature; ``autotune=False``: the single-solve default, which
+    makes every result bitwise that of ``BTE_RTA`` alone); ``graphs`` replays
+    the outer iterations from hipGraphs."""
+    from .operator import build_schedule
     ops, t0 = [], []
     for pr in problems:
         geo = pr['geo']
@@ -322,6 +328,15 @@ def BTE_RTA_batch(problems, bte_max_iter: int = 100, bte_rtol: float = 1e-4,
         f = pr.get('fourier')
         t0.append(np.zeros(geo.n_elems) if f is None else
                   np.asarray(f.variables['Temperature_Fourier']['data'], np.float64))
-    batch = BTEBatch(ops, cfg=config or default_config(ops))
+    scheds = [build_schedule(o) for o in ops]
+    if config is None:
+        if autotune:
+            from . import tune
+            config = tune.tune(ops, scheds)
+        else:
+            config = default_config(ops)
+    batch = BTEBatch(ops, cfg=config, scheds=scheds)
+    if graphs:
+        batch.enable_graphs(True)
     batch.solve(t0, bte_max_iter=bte_max_iter, bte_rtol=bte_rtol)
     return [batch.results(p) for p in range(len(problems))]

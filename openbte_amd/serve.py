@@ -155,11 +155,13 @@ class BTEServer:
     ``hbm_budget`` (bytes) bounds the summed device footprint of admitted
     problems (default: 85% of the free device memory at start); ``max_slots``
     bounds their number.  ``cfg`` fixes the launch shape (mc, block) shared by
-    every problem.
+    every problem.  With ``graphs`` the launches of a round are replayed from
+    hipGraphs captured once per resident set (the set changes on admission
+    and retirement).
     """
 
     def __init__(self, cfg: Optional[SweepConfig] = None, hbm_budget: Optional[int] = None, max_slots: int = 64,
-                 prep_workers: int = 2, device=None):
+                 prep_workers: int = 2, device=None, graphs: bool = True):
         if not torch.cuda.is_available():
             raise RuntimeError('openbte_amd.BTEServer needs a ROCm GPU; there is no CPU fallback')
         self.dev = device or torch.device('cuda', torch.cuda.current_device())
@@ -182,6 +184,7 @@ class BTEServer:
         self._pool = ThreadPoolExecutor(max_workers=max(1, prep_workers))
         self._batch: Optional[BTEBatch] = None
         self._dirty = True
+        self.graphs = bool(graphs)  # replay each round's launches from hipGraphs (per resident set)
 
     # -- intake --------------------------------------------------------------
     def submit(self, req: Request) -> Request:
@@ -296,6 +299,8 @@ class BTEServer:
         if self._dirty or self._batch is None:
             self._batch = BTEBatch(cfg=self.cfg, device=self.dev, devices=[x.dev for x in self.live],
                                    n_active=self.n_active, norm2=self.norm2, slots=[x.slot for x in self.live])
+            if self.graphs:
+                self._batch.enable_graphs(True)
             self._dirty = False
         return self._batch
 

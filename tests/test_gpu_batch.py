@@ -26,7 +26,7 @@ def test_batch_equals_single_and_oracle():
     for pr in probs:
         pr['fourier'] = fo.fourier(pr['geo'], pr['mat'].thermal_conductivity, pr['bcs'],
                                    pr['effective_thermal_conductivity'])
-    outs = BTE_RTA_batch(probs, bte_max_iter=8)
+    outs = BTE_RTA_batch(probs, bte_max_iter=8, autotune=False)
     for pr, out in zip(probs, outs):
         single = BTE_RTA(pr['geo'], pr['mat'], pr['bcs'], fourier=pr['fourier'],
                          effective_thermal_conductivity=pr['effective_thermal_conductivity'], bte_max_iter=8)
@@ -58,7 +58,7 @@ def test_batch_mixed_boundary_kinds(isothermal_case, two_wall_case):
     for pr in probs:
         pr['fourier'] = fo.fourier(pr['geo'], pr['mat'].thermal_conductivity, pr['bcs'],
                                    pr['effective_thermal_conductivity'])
-    outs = BTE_RTA_batch(probs, bte_max_iter=6)
+    outs = BTE_RTA_batch(probs, bte_max_iter=6, autotune=False)
     for pr, out in zip(probs, outs):
         single = BTE_RTA(pr['geo'], pr['mat'], pr['bcs'], fourier=pr['fourier'],
                          effective_thermal_conductivity=pr['effective_thermal_conductivity'],
@@ -75,7 +75,7 @@ def test_batch_different_iteration_counts():
     from openbte_amd.bte import BTE_RTA
     from openbte_amd.batch import BTE_RTA_batch
     probs = [_problem(21, 400), _problem(22, 400)]
-    outs = BTE_RTA_batch(probs, bte_max_iter=30, bte_rtol=1e-3)
+    outs = BTE_RTA_batch(probs, bte_max_iter=30, bte_rtol=1e-3, autotune=False)
     for pr, out in zip(probs, outs):
         single = BTE_RTA(pr['geo'], pr['mat'], pr['bcs'],
                          effective_thermal_conductivity=pr['effective_thermal_conductivity'],
@@ -137,3 +137,25 @@ def test_tuner_picks_a_candidate_and_memoizes():
         ks.append([float(d.buf['kappa'].sum().item()) for d in b.devs])
         del b
     np.testing.assert_allclose(ks[0], ks[1], rtol=1e-9)
+
+
+def test_batch_default_tuned_graphs_match_oracle():
+    """The public entry with its defaults -- launch shape from the tuner,
+    hipGraph-replayed outer iterations -- against the oracle (kappa 1e-6,
+    T 1e-8 K, same iteration counts)."""
+    from oracle import bte_oracle as bo, fourier_oracle as fo
+    from openbte_amd.batch import BTE_RTA_batch
+    probs = [_problem(41, 600), _problem(42, 1200), _problem(43, 300)]
+    for pr in probs:
+        pr['fourier'] = fo.fourier(pr['geo'], pr['mat'].thermal_conductivity, pr['bcs'],
+                                   pr['effective_thermal_conductivity'])
+    outs = BTE_RTA_batch(probs, bte_max_iter=10, bte_rtol=1e-5)
+    for pr, out in zip(probs, outs):
+        ref = bo.bte_rta(pr['geo'], pr['mat'], pr['bcs'], fourier=pr['fourier'],
+                         effective_thermal_conductivity=pr['effective_thermal_conductivity'], bte_max_iter=10,
+                         bte_rtol=1e-5)
+        print('tuned batch', out.kappa_eff, ref.kappa_eff, out.aux['n_iter'], flush=True)
+        assert out.aux['n_iter'] == ref.aux['n_iter']
+        assert abs(out.kappa_eff / ref.kappa_eff - 1) < 1e-6
+        assert np.abs(out.variables['Temperature_BTE']['data']
+                      - ref.variables['Temperature_BTE']['data']).max() < 1e-8
