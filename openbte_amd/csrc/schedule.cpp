@@ -117,13 +117,21 @@ void build_direction(int32_t n, int64_t n_faces, int dim, const int32_t* i_mat, 
       indeg[best] = 0;
       front.push_back(best);
     }
-    std::sort(front.begin(), front.end());
-    cur.assign(carry.begin(), carry.end());
-    cur.insert(cur.end(), front.begin(), front.end());
-    size_t take = cur.size();
-    if (max_width > 0 && take > (size_t)max_width) take = (size_t)max_width;
-    carry.assign(cur.begin() + take, cur.end());
-    cur.resize(take);
+    if (max_width <= 0 || carry.size() + front.size() <= (size_t)max_width) {
+      // nothing is deferred: the level is carry and front whole (one sort,
+      // same result)
+      cur.assign(carry.begin(), carry.end());
+      cur.insert(cur.end(), front.begin(), front.end());
+      carry.clear();
+    } else {
+      std::sort(front.begin(), front.end());
+      cur.assign(carry.begin(), carry.end());
+      cur.insert(cur.end(), front.begin(), front.end());
+      size_t take = cur.size();
+      if (max_width > 0 && take > (size_t)max_width) take = (size_t)max_width;
+      carry.assign(cur.begin() + take, cur.end());
+      cur.resize(take);
+    }
     std::sort(cur.begin(), cur.end());
     out.lvl_ptr.push_back(done);
     next.clear();
@@ -150,27 +158,29 @@ void build_direction(int32_t n, int64_t n_faces, int dim, const int32_t* i_mat, 
 
   // inflow CSR in position order; cut faces numbered in position order
   out.in_ptr.assign(n + 1, 0);
-  out.in_src.clear();
-  out.in_face.clear();
-  out.cut_exit.clear();
-  out.cut_face.clear();
-  out.in_src.reserve(cnt[n]);
-  out.in_face.reserve(cnt[n]);
+  out.in_src.resize(cnt[n]);
+  out.in_face.resize(cnt[n]);
+  int32_t ncut = 0;
+  for (int32_t e = 0; e < cnt[n]; ++e) ncut += is_cut[row_faces[e]];
+  out.cut_exit.resize(ncut);
+  out.cut_face.resize(ncut);
+  int32_t w = 0, q = 0;
   for (int32_t p = 0; p < n; ++p) {
-    int32_t c = out.order[p];
-    for (int32_t e = cnt[c]; e < cnt[c + 1]; ++e) {
-      int32_t f = row_faces[e];
+    const int32_t c = out.order[p];
+    for (int32_t e = cnt[c]; e < cnt[c + 1]; ++e, ++w) {
+      const int32_t f = row_faces[e];
+      const int32_t up = out.pos[j_mat[f]];
       if (is_cut[f]) {
-        int32_t q = (int32_t)out.cut_face.size();
-        out.cut_face.push_back(f);
-        out.cut_exit.push_back(out.pos[j_mat[f]]);
-        out.in_src.push_back(~q);
+        out.cut_face[q] = f;
+        out.cut_exit[q] = up;
+        out.in_src[w] = ~q;
+        ++q;
       } else {
-        out.in_src.push_back(out.pos[j_mat[f]]);
+        out.in_src[w] = up;
       }
-      out.in_face.push_back(f);
+      out.in_face[w] = f;
     }
-    out.in_ptr[p + 1] = (int32_t)out.in_src.size();
+    out.in_ptr[p + 1] = w;
   }
 }
 
@@ -239,22 +249,28 @@ int obte_sched_export(void* handle, int32_t* order, int32_t* pos, int32_t* lvl_p
   if (!handle) return obte_fail(OBTE_EINVAL, "obte_sched_export: null handle");
   auto* s = static_cast<Schedule*>(handle);
   const int64_t n = s->n;
-  int64_t lo = 0, eo = 0, co = 0;
-  for (int32_t l = 0; l < s->n_dirs; ++l) {
+  const int32_t L = s->n_dirs;
+  // offsets of every direction's blocks, then the copies in parallel
+  std::vector<int64_t> lo_off(L + 1, 0), eo_off(L + 1, 0), co_off(L + 1, 0);
+  for (int32_t l = 0; l < L; ++l) {
+    lo_off[l + 1] = lo_off[l] + (int64_t)s->dirs[l].lvl_ptr.size();
+    eo_off[l + 1] = eo_off[l] + (int64_t)s->dirs[l].in_src.size();
+    co_off[l + 1] = co_off[l] + (int64_t)s->dirs[l].cut_face.size();
+  }
+#pragma omp parallel for schedule(static)
+  for (int32_t l = 0; l < L; ++l) {
     const DirSchedule& d = s->dirs[l];
+    const int64_t lo = lo_off[l], eo = eo_off[l], co = co_off[l];
     std::memcpy(order + l * n, d.order.data(), n * sizeof(int32_t));
     std::memcpy(pos + l * n, d.pos.data(), n * sizeof(int32_t));
     std::memcpy(lvl_ptr + lo, d.lvl_ptr.data(), d.lvl_ptr.size() * sizeof(int32_t));
-    lo += (int64_t)d.lvl_ptr.size();
     std::memcpy(in_ptr + l * (n + 1), d.in_ptr.data(), (n + 1) * sizeof(int32_t));
     std::memcpy(in_src + eo, d.in_src.data(), d.in_src.size() * sizeof(int32_t));
     std::memcpy(in_face + eo, d.in_face.data(), d.in_face.size() * sizeof(int32_t));
-    eo += (int64_t)d.in_src.size();
     if (!d.cut_face.empty()) {
       std::memcpy(cut_exit + co, d.cut_exit.data(), d.cut_exit.size() * sizeof(int32_t));
       std::memcpy(cut_face + co, d.cut_face.data(), d.cut_face.size() * sizeof(int32_t));
     }
-    co += (int64_t)d.cut_face.size();
   }
   return OBTE_OK;
 }

@@ -25,6 +25,7 @@ import numpy as np
 import torch
 
 from . import _lib
+from .cache import FOURIER, content_key
 from .objects import SolverResults
 
 PINNED_CELL = 10  # openbte/fourier.py:43
@@ -78,31 +79,33 @@ def build_fourier_host(geo, kappa, bcs, effective_thermal_conductivity=None, hea
     v_orth, v_non = decomposed_directions(geo, kappa)
     inn = np.asarray(geo.internal, np.int64)
     a, b = sem[inn, 0], sem[inn, 1]
-    vo = v_orth[inn]
-    diag = np.zeros(n)
-    np.add.at(diag, a, vo)
-    np.add.at(diag, b, vo)
-    rows = np.concatenate([a, b])
-    cols = np.concatenate([b, a])
-    ws = np.concatenate([vo, vo])
-    o = np.lexsort((cols, rows))
-    rows, cols, ws = rows[o], cols[o], ws[o]
-    row_ptr = np.zeros(n + 1, np.int64)
-    np.add.at(row_ptr, rows + 1, 1)
-    row_ptr = np.cumsum(row_ptr)
+    # per (cell, side slot): the neighbour across an internal side; a cell's
+    # row of the operator is its internal slots in slot order (a kept
+    # periodic side sits in the slots of both its cells)
+    is_int = np.zeros(len(geo.sides), bool)
+    is_int[inn] = True
+    cells = np.repeat(np.arange(n), nv)
+    sd = esm.reshape(-1)
+    first = sem[sd, 0] == cells
+    other = np.where(first, sem[sd, 1], sem[sd, 0])
+    intr = is_int[sd]
+    row_ptr = np.concatenate([[0], np.cumsum(intr.reshape(n, nv).sum(axis=1))])
+    cols = other[intr]
+    ws = v_orth[sd[intr]]
+    diag = np.bincount(cells[intr], weights=ws, minlength=n)
     B = np.zeros(n)
     for key, contact in bcs.mixed.items():
         sides = np.asarray(geo.side_physical_regions[key], np.int64)
         ii = sem[sides, 0]
         h = contact['boundary_conductance']
         tmp = v_orth[sides] * h / (h + v_orth[sides] / geo.side_areas[sides])
-        np.add.at(diag, ii, tmp)
+        diag += np.bincount(ii, weights=tmp, minlength=n)
         B[ii] = contact['value'] * tmp / vol[ii]  # assignment (openbte/fourier.py:86)
     for region, jump in bcs.periodic.items():
         sides = np.asarray(geo.periodic_sides[region], np.int64)
         ii, jj = sem[sides, 0], sem[sides, 1]
-        np.add.at(B, ii, jump * v_orth[sides] / vol[ii])
-        np.add.at(B, jj, -jump * v_orth[sides] / vol[jj])
+        B += np.bincount(ii, weights=jump * v_orth[sides] / vol[ii], minlength=n)
+        B -= np.bincount(jj, weights=jump * v_orth[sides] / vol[jj], minlength=n)
     if len(heat_source) != 0:
         B += (np.asarray(heat_source) * 1e-18) if not modified_fourier else np.asarray(heat_source)
     pin = -1
@@ -113,8 +116,6 @@ def build_fourier_host(geo, kappa, bcs, effective_thermal_conductivity=None, hea
         B[pin] = 0.0
 
     # gradient slots (openbte/objects.py:322-405)
-    is_int = np.zeros(len(geo.sides), bool)
-    is_int[inn] = True
     int_index = -np.ones(len(geo.sides), np.int64)
     int_index[inn] = np.arange(len(inn))
     side_jump = np.zeros(len(geo.sides))
@@ -123,10 +124,6 @@ def build_fourier_host(geo, kappa, bcs, effective_thermal_conductivity=None, hea
     mixed_val = np.full(len(geo.sides), np.nan)
     for key, value in bcs.mixed.items():
         mixed_val[np.asarray(geo.side_physical_regions[key], np.int64)] = float(value['value'])
-    cells = np.repeat(np.arange(n), nv)
-    sd = esm.reshape(-1)
-    first = sem[sd, 0] == cells
-    other = np.where(first, sem[sd, 1], sem[sd, 0])
     mode = np.zeros(n * nv, np.int32)
     val = np.zeros(n * nv)
     nb = -np.ones(n * nv, np.int64)
@@ -256,16 +253,48 @@ def _blas_threads(k: int):
         return contextlib.nullcontext()
 
 
+def prepare_host(host: FourierHost):
+    """The host system as the device solves it: renumbered along a Morton
+    curve with its coarse inverse when the two-level preconditioner applies
+    (returns (host, perm or None, coarse inverse or None))."""
+    if COARSE_MIN_CELLS <= host.n <= COARSE_MAX_CELLS and host.xy is not None:
+        perm = morton_order(host.xy)
+        host = permute_host(host, perm)
+        return host, perm, coarse_inverse(host)
+    return host, None, None
+
+
+def prepare(geometry, kappa, bcs, eff=None, heat_source=(), modified_fourier=False, global_temperature=0.0):
+    """``prepare_host`` of the Fourier system of these inputs, cached by
+    content (a solve on a mesh seen before skips the host work)."""
+    g = geometry
+    dict_arrays = []
+    for dct in (g.periodic_sides, g.side_physical_regions):
+        for k in sorted(dct):
+            dict_arrays += [k, np.asarray(dct[k])]
+    key = content_key('fourier', g.n_elems, g.dim, np.asarray(g.elem_side_map), np.asarray(g.side_elem_map),
+                      np.asarray(g.normal_areas), np.asarray(g.elem_volumes), np.asarray(g.side_areas),
+                      np.asarray(g.dists), np.asarray(g.gradient_weights), np.asarray(g.interp_weights),
+                      np.asarray(g.elem_centroids), np.asarray(g.internal), *dict_arrays,
+                      np.asarray(kappa, np.float64), repr(dict(bcs._asdict()) if hasattr(bcs, '_asdict') else bcs),
+                      repr(eff), np.asarray(heat_source, np.float64), bool(modified_fourier),
+                      float(global_temperature), COARSE_MIN_CELLS, COARSE_MAX_CELLS)
+
+    def build():
+        host = build_fourier_host(geometry, kappa, bcs, eff, heat_source, modified_fourier)
+        if modified_fourier:
+            host = host._replace(B=host.B + global_temperature)
+        return prepare_host(host)
+    return FOURIER.get_or_build(key, build)
+
+
 class FourierDevice:
     """Device-resident Fourier operator and correction loop state."""
 
-    def __init__(self, host: FourierHost, device=None):
+    def __init__(self, host: FourierHost, device=None, prepared=None):
         if not torch.cuda.is_available():
             raise RuntimeError('openbte_amd.Fourier needs a ROCm GPU; there is no CPU fallback')
-        self.perm = None
-        if COARSE_MIN_CELLS <= host.n <= COARSE_MAX_CELLS and host.xy is not None:
-            self.perm = morton_order(host.xy)
-            host = permute_host(host, self.perm)
+        host, self.perm, cinv = prepared if prepared is not None else prepare_host(host)
         self.h = host
         self.dev = dev = device or torch.device('cuda', torch.cuda.current_device())
         f64, i32 = torch.float64, torch.int32
@@ -298,8 +327,8 @@ class FourierDevice:
             setattr(P, name, b[name].data_ptr())
         P.n_contact, P.contact_kind = len(h.ce1), h.contact_kind
         P.c_value, P.normalization = h.c_value, h.normalization
-        if self.perm is not None:
-            b['coarse_inv'] = T(coarse_inverse(h), f64)
+        if cinv is not None:
+            b['coarse_inv'] = T(cinv, f64)
             P.n_coarse, P.coarse_inv = (n + 255) // 256, b['coarse_inv'].data_ptr()
         self.stream = torch.cuda.current_stream(dev).cuda_stream
         self.cg_iters = []
@@ -343,11 +372,9 @@ def Fourier(geometry, thermal_conductivity, boundary_conditions, effective_therm
             fourier_rtol=1e-8, verbose=True, cg_rtol=1e-13) -> SolverResults:
     """Drop-in for ``openbte.fourier.Fourier`` (openbte/fourier.py:15-24)."""
     kappa = np.asarray(thermal_conductivity, np.float64)[:geometry.dim, :geometry.dim]
-    host = build_fourier_host(geometry, kappa, boundary_conditions, effective_thermal_conductivity,
-                              heat_source, modified_fourier)
-    if modified_fourier:
-        host = host._replace(B=host.B + global_temperature)
-    dev = FourierDevice(host)
+    prepared = prepare(geometry, kappa, boundary_conditions, effective_thermal_conductivity, heat_source,
+                       modified_fourier, global_temperature)
+    dev = FourierDevice(prepared[0], prepared=prepared)
     norm_C_old = 0.0
     error, n_iter, kappa_eff = 1.0, 0, 0.0
     while error > fourier_rtol and n_iter < fourier_max_iter:
@@ -364,7 +391,7 @@ def Fourier(geometry, thermal_conductivity, boundary_conditions, effective_therm
     # the reference's ``while ... else`` resets kappa_eff (openbte/fourier.py:149,179-180)
     kappa_diffusive = kappa_eff
     T = dev.in_cell_order(dev.b['T'].cpu().numpy().copy())
-    grad = dev.in_cell_order(dev.b['grad'].view(host.n, host.dim).cpu().numpy())
+    grad = dev.in_cell_order(dev.b['grad'].view(dev.h.n, dev.h.dim).cpu().numpy())
     J = -grad @ kappa.T
     variables = {'Temperature_Fourier': {'data': T, 'units': 'K'},
                  'Flux_Fourier': {'data': J * 1e9, 'units': 'W/m/m'}}

@@ -15,6 +15,7 @@ from typing import NamedTuple, Optional
 import numpy as np
 
 from . import _lib
+from .cache import SCHEDULES, content_key
 
 
 class HostOperator(NamedTuple):
@@ -204,6 +205,14 @@ def build_schedule(op: HostOperator, dirs: Optional[np.ndarray] = None, cut_cand
     ``max_width`` caps the cells of one level (0: longest-path levels)."""
     dirs = op.dirs if dirs is None else np.ascontiguousarray(dirs, dtype=np.float64)
     cut = op.cut_candidate if cut_candidate is None else np.ascontiguousarray(cut_candidate, dtype=np.uint8)
+    # the schedule depends on the faces, the cut set and the directions only:
+    # solves that reuse a mesh reuse it (openbte_amd.cache)
+    key = content_key('schedule', op.n, op.dim, np.asarray(op.i_mat), np.asarray(op.j_mat),
+                      np.asarray(op.k_face), np.asarray(cut), np.asarray(dirs), int(max_width))
+    return SCHEDULES.get_or_build(key, lambda: _build_schedule(op, dirs, cut, max_width))
+
+
+def _build_schedule(op: HostOperator, dirs: np.ndarray, cut: np.ndarray, max_width: int) -> Schedule:
     L = len(dirs)
     n = op.n
     i_mat = np.ascontiguousarray(op.i_mat, np.int32)
