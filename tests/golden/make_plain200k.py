@@ -13,7 +13,10 @@ semantics): DeltaT after each iteration, the per-mode kappa K and
 suppression S, the wall temperatures TB, and the flux J on every 50th cell
 plus its Frobenius norm.  Run from the repository root:
 
-    python tests/golden/make_plain200k.py [workers] [iterations]
+    python tests/golden/make_plain200k.py [workers] [iterations] [--resume]
+
+--resume continues from the last iteration stored in the fixture (same mesh
+hash): its DeltaT and TB are the state the next iteration starts from.
 """
 import hashlib
 import multiprocessing as mp
@@ -64,8 +67,10 @@ def _one_direction(l):
 
 def main():
     global _OPS, _STATE
-    workers = int(sys.argv[1]) if len(sys.argv) > 1 else max(1, len(os.sched_getaffinity(0)) - 1)
-    n_iter = int(sys.argv[2]) if len(sys.argv) > 2 else 2
+    args = [a for a in sys.argv[1:] if a != '--resume']
+    resume = '--resume' in sys.argv[1:]
+    workers = int(args[0]) if len(args) > 0 else max(1, len(os.sched_getaffinity(0)) - 1)
+    n_iter = int(args[1]) if len(args) > 1 else 2
     geo, mat, bcs, eff = case()
     t0 = time.perf_counter()
     f = fo.fourier(geo, mat.thermal_conductivity, bcs, eff)
@@ -78,7 +83,18 @@ def main():
     out = {'mesh_sha256': np.array(mesh_hash(geo)), 'n_elems': geo.n_elems, 'n_mfp': ops.n_mfp,
            'n_dirs': ops.n_dirs, 'fourier_T_norm': np.linalg.norm(T_F), 'fourier_T_sum': T_F.sum(),
            'j_stride': J_STRIDE}
-    for it in range(1, n_iter + 1):
+    first = 1
+    if resume and os.path.exists(OUT):
+        with np.load(OUT) as z:
+            old = {k: z[k] for k in z.files}
+        if str(old['mesh_sha256']) != str(out['mesh_sha256']):
+            raise SystemExit('stored fixture is for another mesh')
+        out = old
+        first = int(old['n_iter']) + 1
+        DeltaT = old[f'delta_t_{first - 1}'].copy()
+        TB = old[f'tb_{first - 1}'].copy()
+        print(f'resuming after iteration {first - 1}', flush=True)
+    for it in range(first, n_iter + 1):
         t0 = time.perf_counter()
         _STATE = (DeltaT, TB)
         parts = [None] * ops.n_dirs
