@@ -392,7 +392,7 @@ def measure_serve(args):
           file=sys.stderr, flush=True)
 
     def serve(arrivals):
-        srv = BTEServer(cfg=SweepConfig(**vars(cfg)), max_slots=64, prep_workers=4)
+        srv = BTEServer(cfg=SweepConfig(**vars(cfg)), max_slots=64, prep_workers=getattr(args, 'prep_workers', 4))
         reqs = [Request(g, m, b, e, fourier='auto', bte_max_iter=maxit, bte_rtol=rtol) for g, m, b, e in probs]
         order = np.argsort(arrivals, kind='stable')
         start = time.perf_counter()
@@ -605,7 +605,7 @@ def north_star_legs(args, geo, mat, bcs, eff, c1, value):
                         'max_rel_kappa_diff_batched_vs_plain': b['config']['max_rel_kappa_diff_batched_vs_plain'],
                         'workload': b['config']['workload']}
     torch.cuda.empty_cache()
-    sv = measure_serve(argparse.Namespace(config='serve', mc=None, block=None))
+    sv = measure_serve(argparse.Namespace(config='serve', mc=None, block=None, prep_workers=args.prep_workers))
     c = sv['config']
     north['served_over_plain'] = c['served_over_plain']
     north['served_p50_s'] = c['served_p50_s']
@@ -633,6 +633,7 @@ def main(argv=None):
     ap.add_argument('--block', type=int, default=None)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--cpu-budget', type=float, default=30.0)
+    ap.add_argument("--prep-workers", type=int, default=8, help='serve: host preparation threads')
     ap.add_argument('--no-extras', action='store_true',
                     help='plain200k: skip the converged 2k-cell parity check, time-to-solution, batched and '
                          'served legs (kernel profiling runs)')

@@ -267,6 +267,13 @@ typedef struct obte_bte_batch {
   const int32_t* task_local;     /* [n_tasks] l * n_chunks + chunk */
   const int32_t* queue_off;      /* [9] task range of each XCD queue */
   int32_t* queue_cnt;            /* [8] queue counters (cleared by each sweep launch) */
+  /* Tasks run by a single wave (problems whose adjacent levels are narrow);
+   * the waves of a workgroup take them once the workgroup queues are drained.
+   * NULL wave_off: none.  Their window holds min_window(wave_pair_max)
+   * positions per wave (block / 64 windows within the 48 KiB budget). */
+  const int32_t* wave_off;       /* [9] task range of each XCD wave queue, or NULL */
+  int32_t* wave_cnt;             /* [8] wave queue counters (cleared by each sweep launch) */
+  int32_t wave_pair_max;         /* max lvl_pair_max of the wave tasks' plans */
 } obte_bte_batch_t;
 
 int obte_batch_assemble(const obte_bte_batch_t* b, obte_stream_t stream);

@@ -63,7 +63,7 @@ class Batch(C.Structure):
         ('max_elems', _i32), ('max_dirs', _i32), ('max_cells', _i32), ('max_walls', _i32),
         ('lvl_pair_max', _i32), ('sweep_window', _i32), ('part_sums', _i32),
         ('plans', _p), ('gcrs', _p), ('io', _p), ('rio', _p), ('live', _p), ('task_pid', _p), ('task_local', _p),
-        ('queue_off', _p), ('queue_cnt', _p),
+        ('queue_off', _p), ('queue_cnt', _p), ('wave_off', _p), ('wave_cnt', _p), ('wave_pair_max', _i32),
     ]
 
 

@@ -83,34 +83,69 @@ class BTEBatch:
         self.norm_old = np.zeros(P)
 
     # -- task table --------------------------------------------------------
+    def wave_pair_limit(self) -> int:
+        """Largest ``lvl_pair_max`` of a problem whose tasks run one per wave
+        (0: none): its window, a power of two, must fit block / 64 times in
+        the 48 KiB window budget."""
+        cfg = self.cfg
+        if not (cfg.wave_tasks and cfg.window) or cfg.mc > 16 or cfg.block < 128:
+            return 0
+        waves = cfg.block // 64
+        cap = 64
+        while waves * ((2 * cap) * cfg.mc // 2 + 1) * 16 <= 48 * 1024:
+            cap *= 2
+        return cap if waves * (cap * cfg.mc // 2 + 1) * 16 <= 48 * 1024 else 0
+
     def _build_tasks(self):
-        items = []  # (cost, pid, l, nch)
+        """Task table: workgroup tasks in 8 queues, then wave tasks (problems
+        whose adjacent levels hold at most ``wave_pair_limit()`` cells) in 8
+        more; each set balanced greedily by cost, longest first."""
+        lim = self.wave_pair_limit()
+        self.wave_problem = np.array([0 < d.lvl_pair_max <= lim for d in self.devs], dtype=bool)
+        sets = ([], [])  # (cost, pid, l, nch) of workgroup tasks, of wave tasks
         for pid, d in enumerate(self.devs):
             lev = d.sched.n_levels
+            wave = bool(self.wave_problem[pid])
             for l in range(d.op.n_dirs):
-                # a direction's chunks run at about levels x (cells / block) passes each
-                cost = float(lev[l]) * max(1.0, d.op.n / d.cfg.block) * d.n_chunks
-                items.append((cost, pid, l, d.n_chunks))
-        items.sort(key=lambda t: -t[0])
-        load = np.zeros(N_QUEUES)
-        queues = [[] for _ in range(N_QUEUES)]
-        for cost, pid, l, nch in items:
-            q = int(np.argmin(load))
-            load[q] += cost
-            queues[q].append((cost, pid, l, nch))
-        tp, tl, off = [], [], [0]
-        for q in queues:
-            for cost, pid, l, nch in q:
-                for c in range(nch):
-                    tp.append(pid)
-                    tl.append(l * nch + c)
-            off.append(len(tp))
+                # passes per level: cells / (cells of one pass), at least one
+                per_pass = 32 if wave else d.cfg.block
+                cost = float(lev[l]) * max(1.0, d.op.n / (float(lev[l]) * per_pass)) * d.n_chunks
+                sets[wave].append((cost, pid, l, d.n_chunks))
+        tp, tl = [], []
+        offs, loads = [], []
+        for items in sets:
+            items.sort(key=lambda t: -t[0])
+            load = np.zeros(N_QUEUES)
+            queues = [[] for _ in range(N_QUEUES)]
+            for cost, pid, l, nch in items:
+                q = int(np.argmin(load))
+                load[q] += cost
+                queues[q].append((cost, pid, l, nch))
+            off = [len(tp)]
+            for q in queues:
+                for cost, pid, l, nch in q:
+                    for c in range(nch):
+                        tp.append(pid)
+                        tl.append(l * nch + c)
+                off.append(len(tp))
+            offs.append(off)
+            loads.append(load)
         self.task_pid = torch.as_tensor(np.asarray(tp, np.int32)).to(self.dev)
         self.task_local = torch.as_tensor(np.asarray(tl, np.int32)).to(self.dev)
-        self.queue_off = torch.as_tensor(np.asarray(off, np.int32)).to(self.dev)
+        self.queue_off = torch.as_tensor(np.asarray(offs[0], np.int32)).to(self.dev)
         self.queue_cnt = torch.zeros(N_QUEUES, dtype=torch.int32, device=self.dev)
         self.n_tasks = len(tp)
-        self.queue_load = load
+        self.n_block_tasks = offs[0][-1]
+        self.n_wave_tasks = self.n_tasks - self.n_block_tasks
+        if self.n_wave_tasks:
+            self.wave_off = torch.as_tensor(np.asarray(offs[1], np.int32)).to(self.dev)
+            self.wave_cnt = torch.zeros(N_QUEUES, dtype=torch.int32, device=self.dev)
+            self.wave_pair_max = max(d.lvl_pair_max for d, w in zip(self.devs, self.wave_problem) if w)
+        else:
+            self.wave_off = self.wave_cnt = None
+            self.wave_pair_max = 0
+        self.queue_load = loads[0]
+        self.wave_load = loads[1]
 
     def _build_structs(self, groups, assemble: bool = True):
         devs = self.devs
@@ -147,7 +182,9 @@ class BTEBatch:
         B.mc, B.block, B.kin = self.cfg.mc, self.cfg.block, self.kin
         props = torch.cuda.get_device_properties(self.dev)
         cus = getattr(props, 'multi_processor_count', 256)
-        B.groups = int(groups or min(self.n_tasks, cus * max(1, 1024 // self.cfg.block)))
+        waves = self.cfg.block // 64
+        work = self.n_block_tasks + (self.n_wave_tasks + waves - 1) // waves
+        B.groups = int(groups or min(work, cus * max(1, 1024 // self.cfg.block)))
         B.max_elems = max(d.op.n * d.op.n_dirs for d in devs)
         B.max_dirs = max(d.op.n_dirs for d in devs)
         B.max_cells = max(d.op.n for d in devs)
@@ -160,6 +197,9 @@ class BTEBatch:
         B.live = self.live.data_ptr()
         B.task_pid, B.task_local = self.task_pid.data_ptr(), self.task_local.data_ptr()
         B.queue_off, B.queue_cnt = self.queue_off.data_ptr(), self.queue_cnt.data_ptr()
+        if self.wave_off is not None:
+            B.wave_off, B.wave_cnt = self.wave_off.data_ptr(), self.wave_cnt.data_ptr()
+            B.wave_pair_max = self.wave_pair_max
         if assemble:
             _lib.call('obte_batch_assemble', self.b, self.stream)
 

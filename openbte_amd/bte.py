@@ -43,6 +43,7 @@ class SweepConfig:
     max_extend: int = 64     # cap on basis extensions per outer iteration
     basis_bytes: float = 24e9  # HBM budget of the recycled basis (u and c)
     window: bool = True      # keep recent levels of a sweep in LDS where they fit
+    wave_tasks: bool = True  # batches: one wave per task for problems with narrow levels
 
 
 class InterfaceNotConverged(UserWarning):

@@ -319,11 +319,36 @@ constexpr int kLvlStage = 2048;  // level pointers staged in LDS per refill
 
 template <int MC>
 struct alignas(16) SweepShared {
-  double red[kSweepBlock];
+  static constexpr int kStage = kLvlStage;
   double sig[3 * MC];      // sigma of the chunk's bins, [bin][3] (final sweeps)
   double wk[4][MC];        // mode-sum weights t, sigma_x, sigma_y, sigma_z per bin (final sweeps)
   int lvl[kLvlStage + 2];  // staged level starts of the running direction
 };
+
+// Shared state of a task run by one wave (a batched sweep runs up to four of
+// them per workgroup side by side: problems whose levels are narrow): the
+// same fields, fewer level starts staged per refill.
+template <int MC>
+struct alignas(16) WaveShared {
+  static constexpr int kStage = 510;
+  double sig[3 * MC];
+  double wk[4][MC];
+  int lvl[kStage + 2];
+};
+
+template <int MC>
+union BatchShared {
+  SweepShared<MC> blk;
+  WaveShared<MC> wav[4];
+};
+
+// LDS ordering point of a task run by one wave: the wave's LDS operations
+// execute in order, and no other wave takes part (no s_barrier)
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 // Per-chunk mode sums of one cell from its row of MC values (G = MC / 2
 // double2 pairs): e[0] = sum over the chunk's bins of t_m T_m, e[1 + d] =
@@ -423,12 +448,14 @@ __device__ __forceinline__ void load_win(WinRegs<U, NP, KIN>& R, int base, int p
   }
 }
 
-// One (problem pid, task lt = l * n_chunks + chunk) sweep by the whole block;
+// One (problem pid, task lt = l * n_chunks + chunk) sweep by the whole block,
+// or (WAVE) by one wave of it: then tid and nthr count the wave's lanes, the
+// wave synchronises alone and owns window slot range wave * (cap * G + 1).
 // WIN: with an LDS window of cap positions (dynamic shared memory).
 // MODE: 0 source sweep, 1 transfer sweep, 2 final sweep, 3 final sweep that
 // also forms the per-chunk mode sums from its LDS window (windowed only).
-template <class S, int MC, int MODE, int KIN, bool WIN, int U>
-__device__ void sweep_task(const S& src, int pid, int lt, SweepShared<MC>& sh, int cap) {
+template <class S, int MC, int MODE, int KIN, bool WIN, int U, bool WAVE = false, class SH = SweepShared<MC>>
+__device__ void sweep_task(const S& src, int pid, int lt, SH& sh, int cap) {
   constexpr bool kFinal = MODE >= 2;
   constexpr bool kFuse = MODE == 3;
   static_assert(WIN || !kFuse, "the fused mode sums read the LDS window");
@@ -442,8 +469,13 @@ __device__ void sweep_task(const S& src, int pid, int lt, SweepShared<MC>& sh, i
   const int chunk = lt - l * P.n_chunks;
   const int n = P.n;
   const int dim = P.dim;
-  const int tid = threadIdx.x;
-  const int nthr = blockDim.x;
+  const int tid = WAVE ? (int)(threadIdx.x & 63) : (int)threadIdx.x;
+  const int nthr = WAVE ? 64 : (int)blockDim.x;
+  auto sync = [] {
+    if constexpr (WAVE) wave_sync();
+    else __syncthreads();
+  };
+  double2* const win = win_lds + (WAVE ? (int)(threadIdx.x >> 6) * (cap * G + 1) : 0);
   const int jl = tid % G;          // bin pair of this lane
   const int cslot = tid / G;       // cell slot of this lane
   const int cps = nthr / G;        // cells per pass
@@ -490,19 +522,19 @@ __device__ void sweep_task(const S& src, int pid, int lt, SweepShared<MC>& sh, i
   int* const lvl = sh.lvl;
   // level starts from `from` on; past the last level every level is empty
   auto stage = [&](int from) {
-    for (int i = tid; i < kLvlStage + 2; i += nthr) lvl[i] = lvl_g[from + i <= nlev ? from + i : nlev];
+    for (int i = tid; i < SH::kStage + 2; i += nthr) lvl[i] = lvl_g[from + i <= nlev ? from + i : nlev];
   };
   if constexpr (WIN) {
     // unused inflow slots read the window with a zero coefficient: no
     // uninitialised (possibly NaN) patterns in it
-    for (int i = tid; i <= cap * G; i += nthr) win_lds[i] = make_double2(0.0, 0.0);
+    for (int i = tid; i <= cap * G; i += nthr) win[i] = make_double2(0.0, 0.0);
   }
   stage(0);
-  __syncthreads();
+  sync();
   // Three register sets rotate (current, next, prefetch target) without
   // copies; the level count is padded to a multiple of 3 with empty levels so
   // the unrolled loop has no exits.  Level starts are staged kSeg at a time.
-  constexpr int kSeg = (kLvlStage - 2) / 3 * 3;
+  constexpr int kSeg = (SH::kStage - 2) / 3 * 3;
   const int nlev3 = (nlev + 2) / 3 * 3;
 
   if constexpr (WIN) {
@@ -551,7 +583,7 @@ __device__ void sweep_task(const S& src, int pid, int lt, SweepShared<MC>& sh, i
         double a0 = 0.0, a1 = 0.0;
 #pragma unroll
         for (int kk = 0; kk < KIN; ++kk) {
-          const double2 w = win_lds[C.woff[u][kk]];
+          const double2 w = win[C.woff[u][kk]];
           a0 += cf[kk] * (w.x + C.far[u][kk].x);
           a1 += cf[kk] * (w.y + C.far[u][kk].y);
         }
@@ -561,7 +593,7 @@ __device__ void sweep_task(const S& src, int pid, int lt, SweepShared<MC>& sh, i
         // go to HBM, unless the final sweep stores them all
         const bool keep = store_all || __double2hiint(rec_get<GQ, NP, 7>(C.pc[u])) != 0;
         buf_store2(t_rs, keep ? C.p[u] * (MC * 8) + jl * 16 : kBufOob, tv);
-        win_lds[(C.p[u] & cmask) * G + jl] = tv;
+        win[(C.p[u] & cmask) * G + jl] = tv;
       }
     };
     // mode sums of a finished level (positions q0 .. q1), read back from the
@@ -571,7 +603,7 @@ __device__ void sweep_task(const S& src, int pid, int lt, SweepShared<MC>& sh, i
     // level's sums are one contiguous range: consecutive positions).
     auto reduce_level = [&](int q0, int q1) {
       for (int p = q0 + tid; p < q1; p += nthr) {
-        const double2* row = win_lds + (p & cmask) * G;
+        const double2* row = win + (p & cmask) * G;
         double e[4] = {0.0, 0.0, 0.0, 0.0};
         chunk_sums<G, true>([&](int j) { return row[j]; },
                             [&](int k, int j) { return *reinterpret_cast<const double2*>(&sh.wk[k][2 * j]); }, dim,
@@ -609,7 +641,7 @@ __device__ void sweep_task(const S& src, int pid, int lt, SweepShared<MC>& sh, i
         rq0 = p0;
         rq1 = p1;
       }
-      __syncthreads();
+      sync();
     };
     WR A, B, C;
     load_win<U, NP, KIN>(A, lvl[0] + cslot, lvl[1], cps, jq, rec_rs);
@@ -620,7 +652,7 @@ __device__ void sweep_task(const S& src, int pid, int lt, SweepShared<MC>& sh, i
     for (int seg = 0; seg < nlev3; seg += kSeg) {
       if (seg > 0) {
         stage(seg);
-        __syncthreads();
+        sync();
       }
       const int kend = min(kSeg, nlev3 - seg);
       for (int kk = 0; kk < kend; kk += 3) {
@@ -694,7 +726,7 @@ __device__ void sweep_task(const S& src, int pid, int lt, SweepShared<MC>& sh, i
       gather(C, g);
       finish(C, g);
     }
-    __syncthreads();
+    sync();
   };
 
   Regs A, B, C;
@@ -705,7 +737,7 @@ __device__ void sweep_task(const S& src, int pid, int lt, SweepShared<MC>& sh, i
   for (int seg = 0; seg < nlev3; seg += kSeg) {
     if (seg > 0) {
       stage(seg);
-      __syncthreads();
+      sync();
     }
     const int kend = min(kSeg, nlev3 - seg);
     for (int kk = 0; kk < kend; kk += 3) {
@@ -742,13 +774,16 @@ __device__ void sweep_task(const S& src, int pid, int lt, SweepShared<MC>& sh, i
       }
       P.part_tb[(int64_t)lt * P.n_wall + sd] = acc;
     }
-    // kappa and suppression of each mode (openbte/bte.py:232-243)
-    if (P.contact_kind != 0) {
+    // kappa and suppression of each mode (openbte/bte.py:232-243), summed
+    // by the task's first wave in a fixed order (lane = contact side mod 64,
+    // then an xor tree), so a task run by one wave or by a workgroup gives
+    // the same bits
+    if (P.contact_kind != 0 && tid < 64) {
 #pragma unroll 1
       for (int m = 0; m < MC; ++m) {
         double kap = 0.0, sup = 0.0;
         const double s0 = sh.sig[3 * m], s1 = sh.sig[3 * m + 1], s2 = sh.sig[3 * m + 2];
-        for (int sc = tid; sc < P.n_contact; sc += nthr) {
+        for (int sc = tid; sc < P.n_contact; sc += 64) {
           const double* na = P.c_na + (int64_t)sc * dim;
           double sn = s0 * na[0] + s1 * na[1];
           double area2 = na[0] * na[0] + na[1] * na[1];
@@ -764,23 +799,12 @@ __device__ void sweep_task(const S& src, int pid, int lt, SweepShared<MC>& sh, i
           kap += P.c_t != nullptr ? P.c_t[sc] * kc : kc;
           sup += t1 * sqrt(area2);
         }
-        // fixed-order block reductions
-        sh.red[tid] = kap;
-        __syncthreads();
-        for (int o = nthr / 2; o > 0; o >>= 1) {
-          if (tid < o) sh.red[tid] += sh.red[tid + o];
-          __syncthreads();
+        double ksum = kap, ssum = sup;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+          ksum += __shfl_xor(ksum, o, 64);
+          ssum += __shfl_xor(ssum, o, 64);
         }
-        const double ksum = sh.red[0];
-        __syncthreads();
-        sh.red[tid] = sup;
-        __syncthreads();
-        for (int o = nthr / 2; o > 0; o >>= 1) {
-          if (tid < o) sh.red[tid] += sh.red[tid + o];
-          __syncthreads();
-        }
-        const double ssum = sh.red[0];
-        __syncthreads();
         const int mg = chunk * MC + m;
         if (tid == 0 && mg < P.n_mfp) {
           P.kappa[(int64_t)mg * L + l] = ksum * P.normalization;
@@ -789,7 +813,7 @@ __device__ void sweep_task(const S& src, int pid, int lt, SweepShared<MC>& sh, i
       }
     }
   }
-  __syncthreads();  // shared state is reused by the next task of this block
+  sync();  // shared state is reused by the next task of this block
 }
 
 // Blocks are dealt to the 8 XCDs round-robin; give all chunks of one
@@ -822,48 +846,71 @@ void k_sweep(One src, int ntask, int cap) {
 }
 
 // batch: persistent workgroups draining 8 XCD-local task queues (tasks of one
-// (problem, direction) share a queue; a drained queue steals from the next)
-// the batch kernel's fused final sweep on tetrahedra at MC = 8 lands seven
+// (problem, direction) share a queue; a drained queue steals from the next).
+// Workgroup tasks come first; then every wave of the workgroup drains the 8
+// queues of wave tasks on its own (tasks of problems whose levels are narrow:
+// one wave covers a level in a pass or two, where a workgroup would leave most
+// of its lanes idle), with its own slice of the shared state and window.
+// The batch kernel's fused final sweep on tetrahedra at MC = 8 lands seven
 // registers over the three-wave budget unasked (three waves: -2% per outer
 // iteration of a batch of tetrahedral slabs)
 template <int MC, int MODE, int KIN, bool WIN>
-constexpr int batch_min_waves() { return (MC == 8 && KIN == 3 && MODE == 3 && WIN) ? 3 : 1; }
+constexpr int batch_min_waves() { return (MC == 8 && KIN == 3 && MODE != 2 && WIN) ? 3 : 1; }
+
+// next task of an XCD's queue set, stealing from the other queues once its own
+// is drained (*q: queues already found empty); -1 when all are drained
+__device__ __forceinline__ int take_task(const int32_t* __restrict__ off, int32_t* __restrict__ cnt, int xcd, int* q) {
+  for (; *q < 8; ++*q) {
+    const int qq = (xcd + *q) & 7;
+    const int i = atomicAdd(&cnt[qq], 1) + off[qq];
+    if (i < off[qq + 1]) return i;
+  }
+  return -1;
+}
 
 template <int MC, int MODE, int KIN, bool WIN>
 __global__ __launch_bounds__(kSweepBlock) __attribute__((amdgpu_waves_per_eu(batch_min_waves<MC, MODE, KIN, WIN>())))
-void k_sweep_batch(Many src, const int32_t* __restrict__ qoff, int32_t* __restrict__ qcnt, int cap) {
-  __shared__ SweepShared<MC> sh;
+void k_sweep_batch(Many src, const int32_t* __restrict__ qoff, int32_t* __restrict__ qcnt, int cap,
+                   const int32_t* __restrict__ woff, int32_t* __restrict__ wcnt, int wcap) {
+  __shared__ BatchShared<MC> sh;
   __shared__ int s_task;
   const int xcd = blockIdx.x % 8;
   int q = 0;
   for (;;) {
-    if (threadIdx.x == 0) {
-      int t = -1;
-      for (; q < 8; ++q) {
-        const int qq = (xcd + q) & 7;
-        const int i = atomicAdd(&qcnt[qq], 1) + qoff[qq];
-        if (i < qoff[qq + 1]) {
-          t = i;
-          break;
-        }
-      }
-      s_task = t;
-    }
+    if (threadIdx.x == 0) s_task = take_task(qoff, qcnt, xcd, &q);
     __syncthreads();
     const int t = s_task;
     __syncthreads();
+    if (t < 0) break;
+    const int pid = src.pid_of(t);
+    const int lt = src.local_of(t);
+    if (!src.live(pid)) continue;
+    const obte_sweep_io_t& io = src.sio(pid);
+    if (io.active != nullptr && io.active[lt] == 0) continue;
+    sweep_task<Many, MC, MODE, KIN, WIN, units_batch<MC, WIN>()>(src, pid, lt, sh.blk, cap);
+  }
+  if (woff == nullptr) return;
+  // wave tasks: no workgroup barrier from here on (the last workgroup task
+  // ended with one, so the shared state is free)
+  const int w = threadIdx.x >> 6;
+  q = 0;
+  for (;;) {
+    int t = -1;
+    if ((threadIdx.x & 63) == 0) t = take_task(woff, wcnt, xcd, &q);
+    t = __builtin_amdgcn_readfirstlane(t);
     if (t < 0) return;
     const int pid = src.pid_of(t);
     const int lt = src.local_of(t);
     if (!src.live(pid)) continue;
     const obte_sweep_io_t& io = src.sio(pid);
     if (io.active != nullptr && io.active[lt] == 0) continue;
-    sweep_task<Many, MC, MODE, KIN, WIN, units_batch<MC, WIN>()>(src, pid, lt, sh, cap);
+    sweep_task<Many, MC, MODE, KIN, WIN, units_batch<MC, WIN>(), true, WaveShared<MC>>(src, pid, lt, sh.wav[w], wcap);
   }
 }
 
-__global__ void k_zero_queues(int32_t* q) {
+__global__ void k_zero_queues(int32_t* q, int32_t* wq) {
   if (threadIdx.x < 8) q[threadIdx.x] = 0;
+  else if (threadIdx.x < 16 && wq != nullptr) wq[threadIdx.x - 8] = 0;
 }
 
 // Block-wide per-lane sum over the slots (fixed order); result broadcast.
@@ -1469,29 +1516,43 @@ struct SweepOne {
   }
 };
 
+// Where the batch kernel finds its task queues and windows.
+struct BatchQueues {
+  const int32_t* qoff;  // workgroup tasks
+  int32_t* qcnt;
+  int cap;              // their window positions (0: no window)
+  const int32_t* woff;  // wave tasks (null: none)
+  int32_t* wcnt;
+  int wcap;             // window positions of each wave
+};
+
 template <int MC, int KIN>
 struct SweepMany {
   template <bool WIN>
-  static void go(const Many& src, int mode, int groups, int block, const int32_t* qoff, int32_t* qcnt, int cap,
-                 hipStream_t st) {
+  static void go(const Many& src, int mode, int groups, int block, const BatchQueues& Q, hipStream_t st) {
     dim3 grid(groups), blk(block);
-    const size_t lds = WIN ? ((size_t)cap * MC / 2 + 1) * sizeof(double2) : 0;  // window + zero slot
-    if (mode == 0) hipLaunchKernelGGL((k_sweep_batch<MC, 0, KIN, WIN>), grid, blk, lds, st, src, qoff, qcnt, cap);
-    else if (mode == 1)
-      hipLaunchKernelGGL((k_sweep_batch<MC, 1, KIN, WIN>), grid, blk, lds, st, src, qoff, qcnt, cap);
+    // the workgroup's window or the waves' windows (each with its zero slot)
+    const size_t blk_win = (size_t)Q.cap * MC / 2 + 1;
+    const size_t wav_win = Q.woff ? (size_t)(block / 64) * ((size_t)Q.wcap * MC / 2 + 1) : 0;
+    const size_t lds = WIN ? (blk_win > wav_win ? blk_win : wav_win) * sizeof(double2) : 0;
+    const int cap = WIN ? Q.cap : 0, wcap = WIN ? Q.wcap : 0;
+#define OBTE_BATCH_LAUNCH(M, W) \
+  hipLaunchKernelGGL((k_sweep_batch<MC, M, KIN, W>), grid, blk, lds, st, src, Q.qoff, Q.qcnt, cap, Q.woff, Q.wcnt, wcap)
+    if (mode == 0) OBTE_BATCH_LAUNCH(0, WIN);
+    else if (mode == 1) OBTE_BATCH_LAUNCH(1, WIN);
     else if constexpr (WIN) {
-      if (mode == 3) hipLaunchKernelGGL((k_sweep_batch<MC, 3, KIN, true>), grid, blk, lds, st, src, qoff, qcnt, cap);
-      else hipLaunchKernelGGL((k_sweep_batch<MC, 2, KIN, true>), grid, blk, lds, st, src, qoff, qcnt, cap);
+      if (mode == 3) OBTE_BATCH_LAUNCH(3, true);
+      else OBTE_BATCH_LAUNCH(2, true);
     } else {
-      hipLaunchKernelGGL((k_sweep_batch<MC, 2, KIN, false>), grid, blk, lds, st, src, qoff, qcnt, cap);
+      OBTE_BATCH_LAUNCH(2, false);
     }
+#undef OBTE_BATCH_LAUNCH
   }
-  static void run(const Many& src, int mode, int groups, int block, const int32_t* qoff, int32_t* qcnt, int cap,
-                  hipStream_t st) {
+  static void run(const Many& src, int mode, int groups, int block, const BatchQueues& Q, hipStream_t st) {
     if constexpr (MC <= 16) {
-      if (cap > 0) return go<true>(src, mode, groups, block, qoff, qcnt, cap, st);
+      if (Q.cap > 0) return go<true>(src, mode, groups, block, Q, st);
     }
-    go<false>(src, mode == 3 ? 2 : mode, groups, block, qoff, qcnt, 0, st);
+    go<false>(src, mode == 3 ? 2 : mode, groups, block, Q, st);
   }
 };
 
@@ -1550,6 +1611,12 @@ int check_batch(const obte_bte_batch_t* B, const char* who) {
   if (B->kin != 2 && B->kin != 3) return obte_fail(OBTE_EINVAL, "batch: kin must be 2 or 3");
   if (B->groups <= 0 || B->max_elems <= 0 || B->max_dirs <= 0 || B->max_cells <= 0)
     return obte_fail(OBTE_EINVAL, "batch: invalid launch sizes");
+  if (B->wave_off != nullptr) {
+    if (!B->wave_cnt || B->wave_pair_max <= 0 || B->wave_pair_max > B->lvl_pair_max)
+      return obte_fail(OBTE_EINVAL, "batch: wave tasks need wave_cnt and 0 < wave_pair_max <= lvl_pair_max");
+    const int64_t wav = (int64_t)(B->block / 64) * ((int64_t)min_window(B->wave_pair_max) * B->mc / 2 + 1) * 16;
+    if (wav > kWindowMaxBytes) return obte_fail(OBTE_EINVAL, "batch: the waves' windows exceed the LDS budget");
+  }
   return OBTE_OK;
 }
 
@@ -1696,10 +1763,15 @@ int obte_batch_sweep(const obte_bte_batch_t* b, int32_t mode, obte_stream_t stre
   if (rc) return rc;
   if (mode < 0 || mode > 2) return obte_fail(OBTE_EINVAL, "obte_batch_sweep: mode must be 0, 1 or 2");
   hipStream_t st = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_zero_queues, dim3(1), dim3(64), 0, st, b->queue_cnt);
-  const int cap = window_cap(b->sweep_window, b->lvl_pair_max, b->mc);
-  const int m = mode == 2 && cap > 0 && b->part_sums ? 3 : (int)mode;
-  dispatch<SweepMany>(b->mc, b->kin, make_many(b, mode), m, b->groups, b->block, b->queue_off, b->queue_cnt, cap, st);
+  BatchQueues Q{b->queue_off, b->queue_cnt, window_cap(b->sweep_window, b->lvl_pair_max, b->mc), nullptr, nullptr, 0};
+  if (b->wave_off != nullptr) {
+    Q.woff = b->wave_off;
+    Q.wcnt = b->wave_cnt;
+    Q.wcap = min_window(b->wave_pair_max);
+  }
+  hipLaunchKernelGGL(k_zero_queues, dim3(1), dim3(64), 0, st, b->queue_cnt, b->wave_cnt);
+  const int m = mode == 2 && Q.cap > 0 && b->part_sums ? 3 : (int)mode;
+  dispatch<SweepMany>(b->mc, b->kin, make_many(b, mode), m, b->groups, b->block, Q, st);
   return launch_status("obte_batch_sweep: launch failed");
 }
 

@@ -70,6 +70,37 @@ def test_batch_mixed_boundary_kinds(isothermal_case, two_wall_case):
         np.testing.assert_array_equal(out.variables['Flux_BTE']['data'], single.variables['Flux_BTE']['data'])
 
 
+@pytest.mark.parametrize('mc', [8, 4])
+def test_batch_wave_tasks_beside_workgroup_tasks(mc):
+    """A problem whose adjacent levels exceed the wave window (8000 cells, 152
+    cells at mc 8) runs one task per workgroup; the narrow ones run one task
+    per wave in the same launches (at mc 4 all three do).  Each result is
+    bitwise its standalone solve, and the same with wave tasks switched off."""
+    from openbte_amd.bte import BTE_RTA, SweepConfig
+    from openbte_amd.batch import BTEBatch, BTE_RTA_batch
+    from openbte_amd.operator import build_host_operator
+    probs = [_problem(41, 8000), _problem(42, 300), _problem(43, 900)]
+    ops = [build_host_operator(p['geo'], p['mat'], p['bcs'], p['effective_thermal_conductivity']) for p in probs]
+    b = BTEBatch(ops, cfg=SweepConfig(mc=mc, block=256))
+    expect = [mc == 4, True, True]
+    assert b.wave_problem.tolist() == expect, (b.wave_pair_limit(), [d.lvl_pair_max for d in b.devs])
+    assert b.n_wave_tasks > 0 and (b.n_block_tasks > 0) == (not all(expect))
+    del b
+    outs = BTE_RTA_batch(probs, bte_max_iter=4, config=SweepConfig(mc=mc, block=256))
+    blocks = BTE_RTA_batch(probs, bte_max_iter=4, config=SweepConfig(mc=mc, block=256, wave_tasks=False))
+    for pr, out, blk in zip(probs, outs, blocks):
+        single = BTE_RTA(pr['geo'], pr['mat'], pr['bcs'],
+                         effective_thermal_conductivity=pr['effective_thermal_conductivity'],
+                         bte_max_iter=4, config=SweepConfig(mc=mc, block=256))
+        for o in (out, blk):
+            assert o.aux['n_iter'] == single.aux['n_iter']
+            assert o.kappa_eff == single.kappa_eff
+            np.testing.assert_array_equal(o.variables['Temperature_BTE']['data'],
+                                          single.variables['Temperature_BTE']['data'])
+            np.testing.assert_array_equal(o.variables['Flux_BTE']['data'], single.variables['Flux_BTE']['data'])
+            np.testing.assert_array_equal(o.aux['suppression'], single.aux['suppression'])
+
+
 def test_batch_different_iteration_counts():
     """Problems stop at their own iteration; the early ones stay frozen."""
     from openbte_amd.bte import BTE_RTA
