@@ -422,6 +422,23 @@ __device__ __forceinline__ double2 buf_load2(__amdgpu_buffer_rsrc_t rs, int off)
   const f64x2 v = __builtin_bit_cast(f64x2, __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0));
   return make_double2(v.x, v.y);
 }
+// a pointer every lane holds the same value of, moved to scalar registers (a
+// buffer descriptor built from it then needs no per-lane handling)
+template <class T>
+__device__ __forceinline__ T* uniform_ptr(T* p) {
+  const uint64_t v = reinterpret_cast<uint64_t>(p);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return reinterpret_cast<T*>(((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ int64_t uniform_i64(int64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)v >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ double buf_load1(__amdgpu_buffer_rsrc_t rs, int off) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, 0));
+}
 __device__ __forceinline__ void buf_store2(__amdgpu_buffer_rsrc_t rs, int off, double2 t) {
   const f64x2 v = {t.x, t.y};
   __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, v), rs, off, 0, 0);
@@ -976,37 +993,102 @@ constexpr int kGsBlock = 8;  // basis vectors per block of the Gram-Schmidt pass
 
 // cn -= sum_j (c_j . cn) c_j and un -= the same combination of u_j over the
 // first kb basis vectors, JB vectors at a time (dot products of a block
-// together, blocks in sequence).
+// together, blocks in sequence).  Each thread walks its faces f = slot,
+// slot + cpp, ... in order; the loads of R consecutive faces are issued
+// together before their arithmetic (the loop is bound by memory latency: a
+// task's interface is a few hundred faces per thread group), which keeps the
+// per-thread operation order and so the results of a face-at-a-time loop.
 template <int MC>
-__device__ void gs_pass(const obte_gcr_t& G, int64_t stride, int64_t base, int ncut, int kb, double* cn,
+__device__ void gs_pass(const obte_gcr_t& G, int64_t stride_, int64_t base, int ncut_, int kb_, double* cn,
                         double* un, double* red, double* bc) {
+  // uniform over the block (a batch reads them from memory per task)
+  const int64_t stride = uniform_i64(stride_);
+  const int ncut = __builtin_amdgcn_readfirstlane(ncut_);
+  const int kb = __builtin_amdgcn_readfirstlane(kb_);
   constexpr int JB = kGsBlock;
+  constexpr int RD = 4;  // faces in flight per thread, dot products
+  constexpr int RU = 2;  // faces in flight per thread, update
   const int lane = threadIdx.x % MC;
   const int slot = threadIdx.x / MC;
   const int cpp = blockDim.x / MC;
+  const int64_t vbytes = (int64_t)ncut * MC * 8;  // this task's slice of one basis vector
+  const double* const gcb = uniform_ptr(G.c + base);
+  const double* const gub = uniform_ptr(G.u + base);
   for (int j0 = 0; j0 < kb; j0 += JB) {
+    // one buffer per basis vector of the block (vectors past kb: empty
+    // buffers, which read as zero and are not used)
+    __amdgpu_buffer_rsrc_t crs[JB], urs[JB];
+#pragma unroll
+    for (int jj = 0; jj < JB; ++jj) {
+      const int64_t vo = (int64_t)(j0 + jj) * stride;
+      crs[jj] = make_rsrc(gcb + vo, j0 + jj < kb ? vbytes : 0);
+      urs[jj] = make_rsrc(gub + vo, j0 + jj < kb ? vbytes : 0);
+    }
     double d[JB];
 #pragma unroll
     for (int jj = 0; jj < JB; ++jj) d[jj] = 0.0;
-    for (int f = slot; f < ncut; f += cpp) {
-      const int64_t o = (int64_t)f * MC + lane;
-      const double c0 = cn[o];
+    int f = slot;
+    for (; f + (RD - 1) * cpp < ncut; f += RD * cpp) {
+      double c0[RD], cc[RD][JB];
+#pragma unroll
+      for (int r = 0; r < RD; ++r) {
+        const int e = (f + r * cpp) * MC + lane;
+        c0[r] = cn[e];
+#pragma unroll
+        for (int jj = 0; jj < JB; ++jj) cc[r][jj] = buf_load1(crs[jj], e * 8);
+      }
+#pragma unroll
+      for (int r = 0; r < RD; ++r)
+#pragma unroll
+        for (int jj = 0; jj < JB; ++jj)
+          if (j0 + jj < kb) d[jj] += cc[r][jj] * c0[r];
+    }
+    for (; f < ncut; f += cpp) {
+      const int e = f * MC + lane;
+      const double c0 = cn[e];
 #pragma unroll
       for (int jj = 0; jj < JB; ++jj)
-        if (j0 + jj < kb) d[jj] += G.c[(j0 + jj) * stride + base + o] * c0;
+        if (j0 + jj < kb) d[jj] += buf_load1(crs[jj], e * 8) * c0;
     }
     block_lane_sums<MC, JB>(d, red, bc);
-    for (int f = slot; f < ncut; f += cpp) {
-      const int64_t o = (int64_t)f * MC + lane;
-      double cv = cn[o], uv = un[o];
+    f = slot;
+    for (; f + (RU - 1) * cpp < ncut; f += RU * cpp) {
+      double cv[RU], uv[RU], cc[RU][JB], uu[RU][JB];
+#pragma unroll
+      for (int r = 0; r < RU; ++r) {
+        const int e = (f + r * cpp) * MC + lane;
+        cv[r] = cn[e];
+        uv[r] = un[e];
+#pragma unroll
+        for (int jj = 0; jj < JB; ++jj) {
+          cc[r][jj] = buf_load1(crs[jj], e * 8);
+          uu[r][jj] = buf_load1(urs[jj], e * 8);
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < RU; ++r) {
+#pragma unroll
+        for (int jj = 0; jj < JB; ++jj)
+          if (j0 + jj < kb) {
+            cv[r] -= d[jj] * cc[r][jj];
+            uv[r] -= d[jj] * uu[r][jj];
+          }
+        const int e = (f + r * cpp) * MC + lane;
+        cn[e] = cv[r];
+        un[e] = uv[r];
+      }
+    }
+    for (; f < ncut; f += cpp) {
+      const int e = f * MC + lane;
+      double cv = cn[e], uv = un[e];
 #pragma unroll
       for (int jj = 0; jj < JB; ++jj)
         if (j0 + jj < kb) {
-          cv -= d[jj] * G.c[(j0 + jj) * stride + base + o];
-          uv -= d[jj] * G.u[(j0 + jj) * stride + base + o];
+          cv -= d[jj] * buf_load1(crs[jj], e * 8);
+          uv -= d[jj] * buf_load1(urs[jj], e * 8);
         }
-      cn[o] = cv;
-      un[o] = uv;
+      cn[e] = cv;
+      un[e] = uv;
     }
   }
 }
