@@ -225,6 +225,19 @@ int obte_gcr_project(const obte_bte_plan_t* plan, const obte_gcr_t* g, obte_stre
 int obte_gcr_begin_extend(const obte_bte_plan_t* plan, const obte_gcr_t* g, obte_stream_t stream);
 /* c = u - x, orthogonalise against the basis, normalise, update y and r. */
 int obte_gcr_end_extend(const obte_bte_plan_t* plan, const obte_gcr_t* g, obte_stream_t stream);
+/* The whole extension loop in one launch: every active (direction, chunk)
+ * task repeats begin_extend, the transfer sweep (obte_bte_sweep mode 1 with
+ * y = g->w, x = g->x, active = g->active, lane_active = g->lane_active) and
+ * end_extend until it converges or has made max_extend extensions, without
+ * waiting for the other tasks.  Each task takes the steps of the host loop
+ * "while any task is active and fewer than max_extend rounds: begin, sweep,
+ * end" (openbte/bte.py:217-220 solves the same systems directly), so the
+ * results are those of that loop.  On return *g->n_active (device) counts the
+ * tasks still active and *ext_max (device int) holds the largest number of
+ * extensions a task made.  No counterpart in the reference (its systems are
+ * factorised, not iterated). */
+int obte_gcr_extend(const obte_bte_plan_t* plan, const obte_gcr_t* g, int32_t max_extend, int32_t* ext_max,
+                    obte_stream_t stream);
 /* Clear the basis (k = 0); y = the plan's current delta_t at each cut's exit
  * cell (use_delta_t != 0) or 0. */
 int obte_gcr_reset(const obte_bte_plan_t* plan, const obte_gcr_t* g, int32_t use_delta_t, obte_stream_t stream);

@@ -94,6 +94,7 @@ _SIGNATURES = {
     'obte_gcr_project': [C.POINTER(BtePlan), C.POINTER(Gcr), _p],
     'obte_gcr_begin_extend': [C.POINTER(BtePlan), C.POINTER(Gcr), _p],
     'obte_gcr_end_extend': [C.POINTER(BtePlan), C.POINTER(Gcr), _p],
+    'obte_gcr_extend': [C.POINTER(BtePlan), C.POINTER(Gcr), _i32, _p, _p],
     'obte_gcr_reset': [C.POINTER(BtePlan), C.POINTER(Gcr), _i32, _p],
     'obte_sweep_window': [_i32, _i32, _i32],
     'obte_batch_assemble': [C.POINTER(Batch), _p],

@@ -44,6 +44,7 @@ class SweepConfig:
     basis_bytes: float = 24e9  # HBM budget of the recycled basis (u and c)
     window: bool = True      # keep recent levels of a sweep in LDS where they fit
     wave_tasks: bool = True  # batches: one wave per task for problems with narrow levels
+    fused_extend: bool = True  # one launch runs every task's whole Krylov extension loop
 
 
 class InterfaceNotConverged(UserWarning):
@@ -227,6 +228,7 @@ class BTEDevice:
         # counters may live in a batch-wide array (one readback per iteration)
         g['n_active'] = n_active if n_active is not None else torch.zeros(1, dtype=i32, device=dev)
         g['rnorm'] = torch.zeros(M * L, dtype=f64, device=dev)
+        self.ext_max = torch.zeros(1, dtype=i32, device=dev)  # extensions of the fused loop
         self.norm2 = norm2 if norm2 is not None else torch.zeros(1, dtype=f64, device=dev)
         self.cur = 0
         self.plan = self._make_plan()
@@ -318,7 +320,19 @@ class BTEDevice:
         max_extend = self.cfg.max_extend if max_extend is None else max_extend
         _lib.call('obte_bte_source', P, st)
         ext = 0
-        if self.n_cut > 0:
+        if self.n_cut > 0 and self.cfg.fused_extend:
+            self._sweep(0, g['y'].data_ptr(), 0, timer)
+            _lib.call('obte_gcr_project', P, G, st)
+            if timer is not None:
+                ev0 = torch.cuda.Event(enable_timing=True)
+                ev1 = torch.cuda.Event(enable_timing=True)
+                ev0.record()
+            _lib.call('obte_gcr_extend', P, G, max_extend, self.ext_max.data_ptr(), st)
+            if timer is not None:
+                ev1.record()
+                timer.append((1, ev0, ev1))
+            ext = -1  # read once the rest of the iteration is queued
+        elif self.n_cut > 0:
             self._sweep(0, g['y'].data_ptr(), 0, timer)
             _lib.call('obte_gcr_project', P, G, st)
             while ext < max_extend and self._n_active() > 0:
@@ -336,6 +350,10 @@ class BTEDevice:
         self.cur = nxt
         P.delta_t = b['dt'][nxt].data_ptr()
         P.tb = b['tb'][nxt].data_ptr()
+        if ext < 0:
+            ext = int(self.ext_max.item())
+            if ext >= max_extend and self._n_active() > 0:
+                warn_unconverged(self._n_active(), max_extend)
         return ext
 
     def solve(self, delta_t0: np.ndarray, bte_max_iter: int = 100, bte_rtol: float = 1e-4,

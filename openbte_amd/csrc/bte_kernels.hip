@@ -1113,9 +1113,11 @@ __device__ __forceinline__ GcrView gcr_view(const obte_bte_plan_t& P, int l, int
 }
 
 // Convergence test of one lane; updates flags.
+// count: add the task to *G.n_active when it stays active (the fused
+// extension loop counts once, at its end).
 template <int MC>
 __device__ void gcr_flags(const obte_bte_plan_t& P, const obte_gcr_t& G, int task, int l, int chunk,
-                          const GcrView& v, double rn2, double ymax, double* red, double* bc) {
+                          const GcrView& v, double rn2, double ymax, double* red, double* bc, bool count) {
   __shared__ int blk_any;
   const int lane = threadIdx.x % MC;
   const int slot = threadIdx.x / MC;
@@ -1135,20 +1137,14 @@ __device__ void gcr_flags(const obte_bte_plan_t& P, const obte_gcr_t& G, int tas
   __syncthreads();
   if (threadIdx.x == 0) {
     G.active[task] = blk_any;
-    if (blk_any) atomicAdd(G.n_active, 1);
+    if (blk_any && count) atomicAdd(G.n_active, 1);
   }
 }
 
-// r = x - y; MGS projection of r on the recycled basis; y += U a.
-template <class S, int MC>
-__global__ __launch_bounds__(kSweepBlock) void k_gcr_project(S src) {
-  __shared__ double red[kGsBlock * kSweepBlock];
-  __shared__ double bc[kGsBlock * 64];
-  const int pid = src.pid_of(blockIdx.x);
-  if (!src.live(pid)) return;
-  const int task = src.local_of(blockIdx.x);
-  const obte_bte_plan_t& P = src.plan(pid);
-  const obte_gcr_t& G = src.gcr(pid);
+// r = x - y; MGS projection of r on the recycled basis; y += U a.  One task
+// by the whole block; red, bc: kGsBlock * blockDim and kGsBlock * 64 doubles.
+template <int MC>
+__device__ void gcr_project_task(const obte_bte_plan_t& P, const obte_gcr_t& G, int task, double* red, double* bc) {
   const int l = task / P.n_chunks;
   const int chunk = task - l * P.n_chunks;
   const int lane = threadIdx.x % MC;
@@ -1181,18 +1177,21 @@ __global__ __launch_bounds__(kSweepBlock) void k_gcr_project(S src) {
   }
   const double rn2 = block_lane_sum<MC>(rp, red, bc);
   const double ymax = block_lane_max<MC>(ym, red, bc);
-  gcr_flags<MC>(P, G, task, l, chunk, v, rn2, ymax, red, bc);
+  gcr_flags<MC>(P, G, task, l, chunk, v, rn2, ymax, red, bc, true);
+}
+
+template <class S, int MC>
+__global__ __launch_bounds__(kSweepBlock) void k_gcr_project(S src) {
+  __shared__ double red[kGsBlock * kSweepBlock];
+  __shared__ double bc[kGsBlock * 64];
+  const int pid = src.pid_of(blockIdx.x);
+  if (!src.live(pid)) return;
+  gcr_project_task<MC>(src.plan(pid), src.gcr(pid), src.local_of(blockIdx.x), red, bc);
 }
 
 // w = r on unconverged lanes, 0 elsewhere (inflow of the transfer sweep).
-template <class S, int MC>
-__global__ __launch_bounds__(kSweepBlock) void k_gcr_begin(S src) {
-  const int pid = src.pid_of(blockIdx.x);
-  if (!src.live(pid)) return;
-  const int task = src.local_of(blockIdx.x);
-  const obte_bte_plan_t& P = src.plan(pid);
-  const obte_gcr_t& G = src.gcr(pid);
-  if (G.active[task] == 0) return;
+template <int MC>
+__device__ void gcr_begin_task(const obte_bte_plan_t& P, const obte_gcr_t& G, int task) {
   const int l = task / P.n_chunks;
   const int chunk = task - l * P.n_chunks;
   const int lane = threadIdx.x % MC;
@@ -1208,18 +1207,22 @@ __global__ __launch_bounds__(kSweepBlock) void k_gcr_begin(S src) {
   }
 }
 
-// c = w - x (= (I - T_op) w); orthogonalise against the basis, normalise,
-// append, then y += a u, r -= a c with a = c . r.
 template <class S, int MC>
-__global__ __launch_bounds__(kSweepBlock) void k_gcr_end(S src) {
-  __shared__ double red[kGsBlock * kSweepBlock];
-  __shared__ double bc[kGsBlock * 64];
+__global__ __launch_bounds__(kSweepBlock) void k_gcr_begin(S src) {
   const int pid = src.pid_of(blockIdx.x);
   if (!src.live(pid)) return;
   const int task = src.local_of(blockIdx.x);
-  const obte_bte_plan_t& P = src.plan(pid);
   const obte_gcr_t& G = src.gcr(pid);
   if (G.active[task] == 0) return;
+  gcr_begin_task<MC>(src.plan(pid), G, task);
+}
+
+// c = w - x (= (I - T_op) w); orthogonalise against the basis, normalise,
+// append, then y += a u, r -= a c with a = c . r.  One active task by the
+// whole block (red, bc as for gcr_project_task).
+template <int MC>
+__device__ void gcr_end_task(const obte_bte_plan_t& P, const obte_gcr_t& G, int task, double* red, double* bc,
+                             bool count) {
   const int l = task / P.n_chunks;
   const int chunk = task - l * P.n_chunks;
   const int lane = threadIdx.x % MC;
@@ -1287,7 +1290,67 @@ __global__ __launch_bounds__(kSweepBlock) void k_gcr_end(S src) {
   const double rn2 = block_lane_sum<MC>(rp, red, bc);
   const double ymax = block_lane_max<MC>(ym, red, bc);
   if (threadIdx.x == 0) G.k[task] = kb + 1;
-  gcr_flags<MC>(P, G, task, l, chunk, v, rn2, ymax, red, bc);
+  gcr_flags<MC>(P, G, task, l, chunk, v, rn2, ymax, red, bc, count);
+}
+
+template <class S, int MC>
+__global__ __launch_bounds__(kSweepBlock) void k_gcr_end(S src) {
+  __shared__ double red[kGsBlock * kSweepBlock];
+  __shared__ double bc[kGsBlock * 64];
+  const int pid = src.pid_of(blockIdx.x);
+  if (!src.live(pid)) return;
+  const int task = src.local_of(blockIdx.x);
+  const obte_gcr_t& G = src.gcr(pid);
+  if (G.active[task] == 0) return;
+  gcr_end_task<MC>(src.plan(pid), G, task, red, bc, true);
+}
+
+// Bytes of dynamic LDS the fused extension loop takes: the sweep window, or
+// the Gram-Schmidt reduction buffers, whichever is larger (they take turns).
+constexpr size_t kGsLdsBytes = (size_t)(kGsBlock * kSweepBlock + kGsBlock * 64) * sizeof(double);
+
+// The interface Krylov extension loop of one problem in one launch: each
+// active (direction, chunk) task runs begin, transfer sweep and end in turn
+// until it converges or has made max_extend extensions, without waiting for
+// the other tasks (the same steps, in the same order per task, as separate
+// begin / sweep / end launches).  At the end *G.n_active counts the tasks
+// still active and *ext_max is raised to this task's extension count.
+template <int MC, int KIN, bool WIN>
+__global__ __launch_bounds__(kSweepBlock) __attribute__((amdgpu_waves_per_eu(3)))
+void k_extend(One src, int ntask, int cap, int max_extend, int32_t* ext_max) {
+  __shared__ SweepShared<MC> sh;
+  const int t = task_of_block(blockIdx.x, ntask);
+  if (t < 0) return;
+  const obte_bte_plan_t& P = src.P;
+  const obte_gcr_t& G = src.G;
+  if (G.active[t] == 0) return;  // uniform over the block
+  double* const red = reinterpret_cast<double*>(win_lds);
+  double* const bc = red + kGsBlock * blockDim.x;
+  int e = 0;
+  while (e < max_extend) {
+    // the task index passes through an empty asm statement, so nothing the
+    // steps derive from it is hoisted out of the loop and kept live across
+    // the other steps (register pressure of the sweep)
+    int tt = t;
+    asm volatile("" : "+s"(tt));
+    gcr_begin_task<MC>(P, G, tt);
+    __syncthreads();  // w (and the flags) seen by every thread of the sweep
+    sweep_task<One, MC, 1, KIN, WIN, units_of<MC, WIN>()>(src, 0, tt, sh, cap);  // ends with a barrier
+    asm volatile("" : "+s"(tt));
+    gcr_end_task<MC>(P, G, tt, red, bc, false);
+    ++e;
+    __syncthreads();  // the task's flag, written by thread 0
+    if (G.active[t] == 0) break;
+  }
+  if (threadIdx.x == 0) {
+    if (G.active[t] != 0) atomicAdd(G.n_active, 1);
+    atomicMax(ext_max, e);
+  }
+}
+
+__global__ void k_zero_extend(int32_t* n_active, int32_t* ext_max) {
+  if (threadIdx.x == 0) *n_active = 0;
+  else if (threadIdx.x == 1) *ext_max = 0;
 }
 
 // zero the active-task counters of every problem (gridDim.x = problems)
@@ -1609,6 +1672,22 @@ struct BatchQueues {
 };
 
 template <int MC, int KIN>
+struct ExtendOne {
+  static void run(const One& src, int ntask, int block, int cap, int max_extend, int32_t* ext_max, hipStream_t st) {
+    dim3 grid(((ntask + 7) / 8) * 8), blk(block);
+    const size_t win = cap > 0 ? ((size_t)cap * MC / 2 + 1) * sizeof(double2) : 0;  // window + zero slot
+    const size_t lds = win > kGsLdsBytes ? win : kGsLdsBytes;
+    if constexpr (MC <= 16) {
+      if (cap > 0) {
+        hipLaunchKernelGGL((k_extend<MC, KIN, true>), grid, blk, lds, st, src, ntask, cap, max_extend, ext_max);
+        return;
+      }
+    }
+    hipLaunchKernelGGL((k_extend<MC, KIN, false>), grid, blk, lds, st, src, ntask, 0, max_extend, ext_max);
+  }
+};
+
+template <int MC, int KIN>
 struct SweepMany {
   template <bool WIN>
   static void go(const Many& src, int mode, int groups, int block, const BatchQueues& Q, hipStream_t st) {
@@ -1784,6 +1863,26 @@ int obte_gcr_begin_extend(const obte_bte_plan_t* plan, const obte_gcr_t* g, obte
 
 int obte_gcr_end_extend(const obte_bte_plan_t* plan, const obte_gcr_t* g, obte_stream_t stream) {
   return gcr_one(plan, g, 2, stream, "obte_gcr_end_extend");
+}
+
+int obte_gcr_extend(const obte_bte_plan_t* plan, const obte_gcr_t* g, int32_t max_extend, int32_t* ext_max,
+                    obte_stream_t stream) {
+  int rc = check_plan(plan, "obte_gcr_extend: null plan");
+  if (rc || (rc = check_gcr(g))) return rc;
+  if (max_extend < 0 || !ext_max) return obte_fail(OBTE_EINVAL, "obte_gcr_extend: max_extend < 0 or null ext_max");
+  One src = make_one(plan, g);
+  src.io.y = g->w;
+  src.io.x = g->x;
+  src.io.active = g->active;
+  src.io.lane_active = g->lane_active;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_zero_extend, dim3(1), dim3(64), 0, st, g->n_active, ext_max);
+  if (max_extend > 0) {
+    const int cap = window_cap(plan->sweep_window, plan->lvl_pair_max, plan->mc);
+    dispatch<ExtendOne>(plan->mc, plan->kin, src, plan->n_dirs * plan->n_chunks, plan->block, cap, (int)max_extend,
+                        ext_max, st);
+  }
+  return launch_status("obte_gcr_extend: launch failed");
 }
 
 int obte_gcr_reset(const obte_bte_plan_t* plan, const obte_gcr_t* g, int32_t use_delta_t, obte_stream_t stream) {
