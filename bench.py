@@ -241,6 +241,7 @@ def measure_batch(args):
         cfg.window = not args.no_window
         tuned = tune.last_results(ops)
         print(f'# autotuned mc={cfg.mc} block={cfg.block}', file=sys.stderr, flush=True)
+    cfg.fused_extend = not args.no_fused
     K = args.steps
     updates = float(ncells.sum()) * M * L * K
 
@@ -281,8 +282,8 @@ def measure_batch(args):
         return dt, ks
     # the plain form at the plain path's default shape, at mc 4 (faster for
     # small meshes alone) and at the batch's shape; the fastest counts
-    plain_runs = {(8, 256): plain(SweepConfig(window=not args.no_window)),
-                  (4, 256): plain(SweepConfig(mc=4, window=not args.no_window))}
+    plain_runs = {(8, 256): plain(SweepConfig(window=not args.no_window, fused_extend=not args.no_fused)),
+                  (4, 256): plain(SweepConfig(mc=4, window=not args.no_window, fused_extend=not args.no_fused))}
     if (cfg.mc, cfg.block) not in plain_runs:
         plain_runs[(cfg.mc, cfg.block)] = plain(cfg)
     best_plain = min(plain_runs, key=lambda k: plain_runs[k][0])
@@ -297,7 +298,7 @@ def measure_batch(args):
         'dtype': 'f64', 'data': 'synthetic (64 seeded multi-pore cells, seeded lognormal Si-like material)',
         'config': {'workload': f'{args.config}: {len(geos)} periodic multi-pore cells, {int(ncells.sum())} cells '
                                f'({int(ncells.min())}-{int(ncells.max())}), {M} MFP x {L} directions',
-                   'mc': cfg.mc, 'block': cfg.block, 'hipgraph': not args.no_graphs,
+                   'mc': cfg.mc, 'block': cfg.block, 'hipgraph': not args.no_graphs, 'fused_extend': cfg.fused_extend,
                    'extensions_per_step': exts,
                    'autotune': tuned,
                    'plain_value': updates / dt_plain, 'plain_ms_per_step': dt_plain / K * 1e3,
@@ -628,6 +629,8 @@ def main(argv=None):
                     + sorted(INVERSE_CONFIGS))
     ap.add_argument('--no-graphs', action='store_true', help='batched configs: launch without hipGraphs')
     ap.add_argument('--no-window', action='store_true', help='sweeps read every upwind value from HBM')
+    ap.add_argument('--no-fused', action='store_true',
+                    help='batched configs: host-driven Krylov extension rounds instead of one launch per iteration')
     ap.add_argument('--kmax', type=int, default=0, help='recycled interface basis capacity (vectors)')
     ap.add_argument('--mc', type=int, default=None)
     ap.add_argument('--block', type=int, default=None)

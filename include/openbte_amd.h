@@ -295,6 +295,16 @@ int obte_batch_sweep(const obte_bte_batch_t* b, int32_t mode, obte_stream_t stre
 /* op 0 project, 1 begin_extend, 2 end_extend (as the obte_gcr_* calls) */
 int obte_batch_gcr(const obte_bte_batch_t* b, int32_t op, obte_stream_t stream);
 int obte_batch_gcr_reset(const obte_bte_batch_t* b, int32_t use_delta_t, obte_stream_t stream);
+/* obte_gcr_extend for every live problem of the batch in one persistent
+ * launch: workgroups drain the task queues (workgroup tasks, then wave tasks,
+ * each run by a whole workgroup here) and loop begin_extend, transfer sweep
+ * and end_extend per task until it converges or has made max_extend
+ * extensions, without waiting for other tasks (openbte/bte.py:217-220 solves
+ * the same systems directly).  Clears and then sets *g.n_active of every
+ * problem (tasks still active) and *ext_max (device int: the largest number
+ * of extensions a task made).  The results are those of the round-by-round
+ * obte_batch_gcr / obte_batch_sweep loop. */
+int obte_batch_extend(const obte_bte_batch_t* b, int32_t max_extend, int32_t* ext_max, obte_stream_t stream);
 /* *out (device int) = number of active GCR tasks over all problems; used to
  * record the rounds of a graph-captured iteration. */
 int obte_batch_log_active(const obte_bte_batch_t* b, int32_t* out, obte_stream_t stream);

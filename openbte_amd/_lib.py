@@ -102,6 +102,7 @@ _SIGNATURES = {
     'obte_batch_sweep': [C.POINTER(Batch), _i32, _p],
     'obte_batch_gcr': [C.POINTER(Batch), _i32, _p],
     'obte_batch_gcr_reset': [C.POINTER(Batch), _i32, _p],
+    'obte_batch_extend': [C.POINTER(Batch), _i32, _p, _p],
     'obte_batch_reduce': [C.POINTER(Batch), _p],
     'obte_batch_log_active': [C.POINTER(Batch), _p, _p],
     'obte_tim_face_grad': [C.POINTER(BtePlan), _p, _p, _p, _p, _p, _p, _p, _p, _p, C.c_int32, _p, _p],

@@ -78,6 +78,7 @@ class BTEBatch:
         self.live = torch.ones(P, dtype=torch.int32, device=self.dev)
         self._build_tasks()
         self._build_structs(groups, assemble)
+        self.ext_max = torch.zeros(1, dtype=torch.int32, device=self.dev)
         self.n_iter = np.zeros(P, dtype=np.int64)
         self.error = np.ones(P)
         self.norm_old = np.zeros(P)
@@ -246,9 +247,36 @@ class BTEBatch:
         _lib.call('obte_batch_sweep', b, 2, st)
         _lib.call('obte_batch_reduce', b, st)
 
+    def _fused(self, max_extend: int):
+        """The whole outer iteration with the extension loop in one launch
+        (``obte_batch_extend``)."""
+        b, st = self.b, torch.cuda.current_stream(self.dev).cuda_stream
+        _lib.call('obte_batch_source', b, st)
+        _lib.call('obte_batch_sweep', b, 0, st)
+        _lib.call('obte_batch_gcr', b, 0, st)
+        _lib.call('obte_batch_extend', b, max_extend, self.ext_max.data_ptr(), st)
+        self._tail()
+
+    def _fused_done(self, max_extend: int) -> int:
+        ext = int(self.ext_max.item())
+        n_left = int(self.n_active.sum().item())
+        if n_left > 0:
+            warn_unconverged(n_left, max_extend)
+        return ext
+
     def outer_iteration(self, max_extend: Optional[int] = None) -> int:
-        """One outer iteration of every live problem; returns the extensions."""
+        """One outer iteration of every live problem; returns the extensions
+        (the largest count of a task)."""
         max_extend = self.cfg.max_extend if max_extend is None else max_extend
+        if self.cfg.fused_extend:
+            if self.use_graphs:
+                g = self._graphs.get(('fused', max_extend))
+                if g is None:
+                    g = self._graphs[('fused', max_extend)] = self._capture(self._fused, max_extend)
+                g.replay()
+            else:
+                self._fused(max_extend)
+            return self._fused_done(max_extend)
         if self.use_graphs:
             return self._outer_iteration_graph(max_extend)
         self._head(0)
@@ -265,12 +293,13 @@ class BTEBatch:
     use_graphs = False
 
     def enable_graphs(self, on: bool = True):
-        """Replay the outer iteration from hipGraphs: graph A = source, first
-        sweep, projection and E extension rounds (rounds of converged tasks
-        are no-ops inside the kernels); graph B = final sweep and reduction.
-        E follows the previous iteration's extension count; a residual left
-        after graph A is finished by host-driven rounds before graph B, so the
-        solve stays exact."""
+        """Replay the outer iteration from hipGraphs.  With the fused
+        extension loop (``SweepConfig.fused_extend``) one graph holds the whole
+        iteration.  Otherwise graph A = source, first sweep, projection and E
+        extension rounds (rounds of converged tasks are no-ops inside the
+        kernels); graph B = final sweep and reduction.  E follows the previous
+        iteration's extension count; a residual left after graph A is finished
+        by host-driven rounds before graph B, so the solve stays exact."""
         self.use_graphs = on
         self._graphs = {}
         self._tail_graph = None

@@ -62,18 +62,24 @@ struct One {
   __device__ int local_of(int t) const { return t; }
 };
 
+// The batch's struct arrays and task table are read through the constant
+// address space: nothing writes them while a kernel runs, so the loads of a
+// uniform problem index are scalar loads and the fields stay in scalar
+// registers (through generic pointers every field was a vector load).
+#define OBTE_CONST __attribute__((address_space(4)))
+
 struct Many {
-  const obte_bte_plan_t* P;
-  const obte_gcr_t* G;
-  const obte_sweep_io_t* io;
-  const obte_reduce_io_t* rio;
-  const int32_t* livef;
-  const int32_t* tpid;
-  const int32_t* tloc;
-  __device__ const obte_bte_plan_t& plan(int p) const { return P[p]; }
-  __device__ const obte_gcr_t& gcr(int p) const { return G[p]; }
-  __device__ const obte_sweep_io_t& sio(int p) const { return io[p]; }
-  __device__ const obte_reduce_io_t& red(int p) const { return rio[p]; }
+  const OBTE_CONST obte_bte_plan_t* P;
+  const OBTE_CONST obte_gcr_t* G;
+  const OBTE_CONST obte_sweep_io_t* io;
+  const OBTE_CONST obte_reduce_io_t* rio;
+  const OBTE_CONST int32_t* livef;
+  const OBTE_CONST int32_t* tpid;
+  const OBTE_CONST int32_t* tloc;
+  __device__ const OBTE_CONST obte_bte_plan_t& plan(int p) const { return P[p]; }
+  __device__ const OBTE_CONST obte_gcr_t& gcr(int p) const { return G[p]; }
+  __device__ const OBTE_CONST obte_sweep_io_t& sio(int p) const { return io[p]; }
+  __device__ const OBTE_CONST obte_reduce_io_t& red(int p) const { return rio[p]; }
   __device__ bool live(int p) const { return livef[p] != 0; }
   __device__ int pid_of(int t) const { return tpid[t]; }
   __device__ int local_of(int t) const { return tloc[t]; }
@@ -87,7 +93,7 @@ struct Many {
 template <class S>
 __global__ void k_assemble(S src) {
   const int pid = blockIdx.y;
-  const obte_bte_plan_t& P = src.plan(pid);
+  const auto& P = src.plan(pid);
   const int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int64_t total = (int64_t)P.n_dirs * P.n;
   if (gid >= total) return;
@@ -152,7 +158,7 @@ __host__ __device__ inline int min_window(int pair_max) {
 template <class S>
 __global__ void k_mark_keep(S src) {
   const int pid = blockIdx.y;
-  const obte_bte_plan_t& P = src.plan(pid);
+  const auto& P = src.plan(pid);
   const int cap = min_window(P.lvl_pair_max);
   const int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (cap == 0 || gid >= (int64_t)P.n_dirs * P.n) return;
@@ -180,7 +186,7 @@ __global__ void k_mark_keep(S src) {
 template <class S>
 __global__ void k_mark_exits(S src) {
   const int pid = blockIdx.y;
-  const obte_bte_plan_t& P = src.plan(pid);
+  const auto& P = src.plan(pid);
   if (min_window(P.lvl_pair_max) == 0) return;
   int32_t* words = reinterpret_cast<int32_t*>(P.rec);
   for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < P.n_cut; f += gridDim.x * blockDim.x) {
@@ -200,7 +206,7 @@ __global__ void k_mark_exits(S src) {
 template <class S>
 __global__ void k_mark_cells(S src) {
   const int pid = blockIdx.y;
-  const obte_bte_plan_t& P = src.plan(pid);
+  const auto& P = src.plan(pid);
   if (min_window(P.lvl_pair_max) == 0) return;
   const int nc = P.contact_kind == 0 ? 0 : P.n_contact;
   const int per = P.n_wall + (P.contact_kind == 1 ? 2 * nc : nc);
@@ -222,7 +228,7 @@ template <class S>
 __global__ void k_source(S src) {
   const int pid = blockIdx.y;
   if (!src.live(pid)) return;
-  const obte_bte_plan_t& P = src.plan(pid);
+  const auto& P = src.plan(pid);
   const int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int64_t total = (int64_t)P.n_dirs * P.n;
   if (gid >= total) return;
@@ -472,15 +478,15 @@ __device__ __forceinline__ void load_win(WinRegs<U, NP, KIN>& R, int base, int p
 // MODE: 0 source sweep, 1 transfer sweep, 2 final sweep, 3 final sweep that
 // also forms the per-chunk mode sums from its LDS window (windowed only).
 template <class S, int MC, int MODE, int KIN, bool WIN, int U, bool WAVE = false, class SH = SweepShared<MC>>
-__device__ void sweep_task(const S& src, int pid, int lt, SH& sh, int cap) {
+__device__ __forceinline__ void sweep_task(const S& src, int pid, int lt, SH& sh, int cap) {
   constexpr bool kFinal = MODE >= 2;
   constexpr bool kFuse = MODE == 3;
   static_assert(WIN || !kFuse, "the fused mode sums read the LDS window");
   constexpr int G = MC / 2;       // lanes per cell
   constexpr int GQ = G < 4 ? G : 4;  // lanes of a cell inside one quad
   constexpr int NP = 4 / GQ;         // record pieces per lane
-  const obte_bte_plan_t& P = src.plan(pid);
-  const obte_sweep_io_t& io = src.sio(pid);
+  const auto& P = src.plan(pid);
+  const auto& io = src.sio(pid);
   const int L = P.n_dirs;
   const int l = lt / P.n_chunks;
   const int chunk = lt - l * P.n_chunks;
@@ -902,7 +908,7 @@ void k_sweep_batch(Many src, const int32_t* __restrict__ qoff, int32_t* __restri
     const int pid = src.pid_of(t);
     const int lt = src.local_of(t);
     if (!src.live(pid)) continue;
-    const obte_sweep_io_t& io = src.sio(pid);
+    const auto& io = src.sio(pid);
     if (io.active != nullptr && io.active[lt] == 0) continue;
     sweep_task<Many, MC, MODE, KIN, WIN, units_batch<MC, WIN>()>(src, pid, lt, sh.blk, cap);
   }
@@ -919,7 +925,7 @@ void k_sweep_batch(Many src, const int32_t* __restrict__ qoff, int32_t* __restri
     const int pid = src.pid_of(t);
     const int lt = src.local_of(t);
     if (!src.live(pid)) continue;
-    const obte_sweep_io_t& io = src.sio(pid);
+    const auto& io = src.sio(pid);
     if (io.active != nullptr && io.active[lt] == 0) continue;
     sweep_task<Many, MC, MODE, KIN, WIN, units_batch<MC, WIN>(), true, WaveShared<MC>>(src, pid, lt, sh.wav[w], wcap);
   }
@@ -998,8 +1004,8 @@ constexpr int kGsBlock = 8;  // basis vectors per block of the Gram-Schmidt pass
 // together before their arithmetic (the loop is bound by memory latency: a
 // task's interface is a few hundred faces per thread group), which keeps the
 // per-thread operation order and so the results of a face-at-a-time loop.
-template <int MC>
-__device__ void gs_pass(const obte_gcr_t& G, int64_t stride_, int64_t base, int ncut_, int kb_, double* cn,
+template <int MC, class GT>
+__device__ void gs_pass(const GT& G, int64_t stride_, int64_t base, int ncut_, int kb_, double* cn,
                         double* un, double* red, double* bc) {
   // uniform over the block (a batch reads them from memory per task)
   const int64_t stride = uniform_i64(stride_);
@@ -1104,7 +1110,8 @@ struct GcrView {
   int ncut;
 };
 
-__device__ __forceinline__ GcrView gcr_view(const obte_bte_plan_t& P, int l, int chunk, int MC) {
+template <class PT>
+__device__ __forceinline__ GcrView gcr_view(const PT& P, int l, int chunk, int MC) {
   GcrView v;
   v.stride = (int64_t)P.n_chunks * P.n_cut * MC;
   v.base = ((int64_t)chunk * P.n_cut + P.cut_off[l]) * MC;
@@ -1115,8 +1122,8 @@ __device__ __forceinline__ GcrView gcr_view(const obte_bte_plan_t& P, int l, int
 // Convergence test of one lane; updates flags.
 // count: add the task to *G.n_active when it stays active (the fused
 // extension loop counts once, at its end).
-template <int MC>
-__device__ void gcr_flags(const obte_bte_plan_t& P, const obte_gcr_t& G, int task, int l, int chunk,
+template <int MC, class PT, class GT>
+__device__ void gcr_flags(const PT& P, const GT& G, int task, int l, int chunk,
                           const GcrView& v, double rn2, double ymax, double* red, double* bc, bool count) {
   __shared__ int blk_any;
   const int lane = threadIdx.x % MC;
@@ -1143,8 +1150,8 @@ __device__ void gcr_flags(const obte_bte_plan_t& P, const obte_gcr_t& G, int tas
 
 // r = x - y; MGS projection of r on the recycled basis; y += U a.  One task
 // by the whole block; red, bc: kGsBlock * blockDim and kGsBlock * 64 doubles.
-template <int MC>
-__device__ void gcr_project_task(const obte_bte_plan_t& P, const obte_gcr_t& G, int task, double* red, double* bc) {
+template <int MC, class PT, class GT>
+__device__ __forceinline__ void gcr_project_task(const PT& P, const GT& G, int task, double* red, double* bc) {
   const int l = task / P.n_chunks;
   const int chunk = task - l * P.n_chunks;
   const int lane = threadIdx.x % MC;
@@ -1190,8 +1197,8 @@ __global__ __launch_bounds__(kSweepBlock) void k_gcr_project(S src) {
 }
 
 // w = r on unconverged lanes, 0 elsewhere (inflow of the transfer sweep).
-template <int MC>
-__device__ void gcr_begin_task(const obte_bte_plan_t& P, const obte_gcr_t& G, int task) {
+template <int MC, class PT, class GT>
+__device__ __forceinline__ void gcr_begin_task(const PT& P, const GT& G, int task) {
   const int l = task / P.n_chunks;
   const int chunk = task - l * P.n_chunks;
   const int lane = threadIdx.x % MC;
@@ -1212,7 +1219,7 @@ __global__ __launch_bounds__(kSweepBlock) void k_gcr_begin(S src) {
   const int pid = src.pid_of(blockIdx.x);
   if (!src.live(pid)) return;
   const int task = src.local_of(blockIdx.x);
-  const obte_gcr_t& G = src.gcr(pid);
+  const auto& G = src.gcr(pid);
   if (G.active[task] == 0) return;
   gcr_begin_task<MC>(src.plan(pid), G, task);
 }
@@ -1220,8 +1227,8 @@ __global__ __launch_bounds__(kSweepBlock) void k_gcr_begin(S src) {
 // c = w - x (= (I - T_op) w); orthogonalise against the basis, normalise,
 // append, then y += a u, r -= a c with a = c . r.  One active task by the
 // whole block (red, bc as for gcr_project_task).
-template <int MC>
-__device__ void gcr_end_task(const obte_bte_plan_t& P, const obte_gcr_t& G, int task, double* red, double* bc,
+template <int MC, class PT, class GT>
+__device__ __forceinline__ void gcr_end_task(const PT& P, const GT& G, int task, double* red, double* bc,
                              bool count) {
   const int l = task / P.n_chunks;
   const int chunk = task - l * P.n_chunks;
@@ -1300,7 +1307,7 @@ __global__ __launch_bounds__(kSweepBlock) void k_gcr_end(S src) {
   const int pid = src.pid_of(blockIdx.x);
   if (!src.live(pid)) return;
   const int task = src.local_of(blockIdx.x);
-  const obte_gcr_t& G = src.gcr(pid);
+  const auto& G = src.gcr(pid);
   if (G.active[task] == 0) return;
   gcr_end_task<MC>(src.plan(pid), G, task, red, bc, true);
 }
@@ -1348,6 +1355,68 @@ void k_extend(One src, int ntask, int cap, int max_extend, int32_t* ext_max) {
   }
 }
 
+// The extension loop of every live problem of a batch in one persistent
+// launch: workgroups drain the workgroup task queues, then the wave task
+// queues (here each of those tasks is run by the whole workgroup as well), and
+// run every active task's begin, transfer sweep and end steps as k_extend
+// does, so no task waits for the tasks of other problems or directions.
+// *ext_max is raised to the largest extension count of a task.
+template <int MC, int KIN, bool WIN>
+__global__ __launch_bounds__(kSweepBlock) __attribute__((amdgpu_waves_per_eu(3)))
+void k_extend_batch(Many src, const int32_t* __restrict__ qoff, int32_t* __restrict__ qcnt,
+                    const int32_t* __restrict__ woff, int32_t* __restrict__ wcnt, int cap, int max_extend,
+                    int32_t* ext_max) {
+  __shared__ SweepShared<MC> sh;
+  __shared__ int s_task;
+  double* const red = reinterpret_cast<double*>(win_lds);
+  double* const bc = red + kGsBlock * blockDim.x;
+  const int xcd = blockIdx.x % 8;
+  for (int set = 0; set < 2; ++set) {
+    const int32_t* off = set == 0 ? qoff : woff;
+    int32_t* cnt = set == 0 ? qcnt : wcnt;
+    if (off == nullptr) break;
+    int q = 0;
+    for (;;) {
+      if (threadIdx.x == 0) s_task = take_task(off, cnt, xcd, &q);
+      __syncthreads();
+      const int t = __builtin_amdgcn_readfirstlane(s_task);  // uniform: scalar registers
+      __syncthreads();
+      if (t < 0) break;
+      const int pid = __builtin_amdgcn_readfirstlane(src.pid_of(t));
+      const int lt = __builtin_amdgcn_readfirstlane(src.local_of(t));
+      if (!src.live(pid)) continue;
+      const auto& G = src.gcr(pid);
+      if (G.active[lt] == 0) continue;  // uniform over the block
+      int e = 0;
+      while (e < max_extend) {
+        // the task passes through an empty asm statement, so nothing derived
+        // from it is kept live across the steps (register pressure)
+        int tt = lt, pp = pid;
+        asm volatile("" : "+s"(tt), "+s"(pp));
+        gcr_begin_task<MC>(src.plan(pp), src.gcr(pp), tt);
+        __syncthreads();  // w (and the flags) seen by every thread of the sweep
+        sweep_task<Many, MC, 1, KIN, WIN, units_batch<MC, WIN>()>(src, pp, tt, sh, cap);  // ends with a barrier
+        asm volatile("" : "+s"(tt), "+s"(pp));
+        gcr_end_task<MC>(src.plan(pp), src.gcr(pp), tt, red, bc, false);
+        ++e;
+        __syncthreads();  // the task's flag, written by thread 0
+        if (src.gcr(pid).active[lt] == 0) break;
+      }
+      if (threadIdx.x == 0) {
+        if (src.gcr(pid).active[lt] != 0) atomicAdd(src.gcr(pid).n_active, 1);
+        atomicMax(ext_max, e);
+      }
+    }
+  }
+}
+
+// queue counters (workgroup and wave sets) and the extension count of a batched extension loop
+__global__ void k_zero_batch_extend(int32_t* q, int32_t* wq, int32_t* ext_max) {
+  if (threadIdx.x < 8) q[threadIdx.x] = 0;
+  else if (threadIdx.x < 16 && wq != nullptr) wq[threadIdx.x - 8] = 0;
+  else if (threadIdx.x == 16) *ext_max = 0;
+}
+
 __global__ void k_zero_extend(int32_t* n_active, int32_t* ext_max) {
   if (threadIdx.x == 0) *n_active = 0;
   else if (threadIdx.x == 1) *ext_max = 0;
@@ -1365,8 +1434,8 @@ template <class S>
 __global__ void k_gcr_reset(S src, int use_dt) {
   const int pid = blockIdx.y;
   if (!src.live(pid)) return;
-  const obte_bte_plan_t& P = src.plan(pid);
-  const obte_gcr_t& G = src.gcr(pid);
+  const auto& P = src.plan(pid);
+  const auto& G = src.gcr(pid);
   const int l = blockIdx.x;
   if (l >= P.n_dirs) return;
   const int cb = P.cut_off[l];
@@ -1396,7 +1465,8 @@ __global__ void k_gcr_reset(S src, int use_dt) {
 constexpr int kModesumMaxM = 1024;
 
 // sum over the bins of sigma_ml (fixed order), by one thread
-__device__ __forceinline__ void sigma_total(const obte_bte_plan_t& P, int l, double* s) {
+template <class PT>
+__device__ __forceinline__ void sigma_total(const PT& P, int l, double* s) {
   const int L = P.n_dirs, dim = P.dim;
   double a0 = 0.0, a1 = 0.0, a2 = 0.0;
   for (int m = 0; m < P.n_mfp; ++m) {
@@ -1410,7 +1480,8 @@ __device__ __forceinline__ void sigma_total(const obte_bte_plan_t& P, int l, dou
   s[2] = a2;
 }
 
-__device__ __forceinline__ void modesum_store(const obte_bte_plan_t& P, int l, int p, const double (&acc)[4],
+template <class PT>
+__device__ __forceinline__ void modesum_store(const PT& P, int l, int p, const double (&acc)[4],
                                               const double* stot) {
   const int64_t gid = (int64_t)l * P.n + p;
   const double dt = P.delta_t[P.order[gid]];
@@ -1426,7 +1497,7 @@ __global__ __launch_bounds__(256) void k_modesum(S src) {
   __shared__ double stot[4];
   const int pid = blockIdx.z;
   if (!src.live(pid)) return;
-  const obte_bte_plan_t& P = src.plan(pid);
+  const auto& P = src.plan(pid);
   const int l = blockIdx.y;
   const int L = P.n_dirs, M = P.n_mfp, dim = P.dim, nch = P.n_chunks, n = P.n;
   if (l >= L || (int64_t)blockIdx.x * blockDim.x >= n) return;  // uniform over the block
@@ -1466,7 +1537,7 @@ __global__ __launch_bounds__(256) void k_modesum_part(S src) {
   __shared__ double stot[4];
   const int pid = blockIdx.z;
   if (!src.live(pid)) return;
-  const obte_bte_plan_t& P = src.plan(pid);
+  const auto& P = src.plan(pid);
   const int l = blockIdx.y;
   const int n = P.n, dim = P.dim, nch = P.n_chunks;
   if (l >= P.n_dirs || (int64_t)blockIdx.x * blockDim.x >= n) return;  // uniform over the block
@@ -1492,8 +1563,8 @@ __global__ void k_reduce_cells(S src) {
   __shared__ double red[256];
   const int pid = blockIdx.y;
   if (!src.live(pid)) return;
-  const obte_bte_plan_t& P = src.plan(pid);
-  const obte_reduce_io_t& R = src.red(pid);
+  const auto& P = src.plan(pid);
+  const auto& R = src.red(pid);
   if ((int64_t)blockIdx.x * blockDim.x >= P.n) return;  // uniform
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   const int dim = P.dim;
@@ -1531,8 +1602,8 @@ __global__ __launch_bounds__(256) void k_reduce_walls(S src) {
   __shared__ double part[16][kWallsPerBlock + 1];
   const int pid = blockIdx.y;
   if (!src.live(pid)) return;
-  const obte_bte_plan_t& P = src.plan(pid);
-  const obte_reduce_io_t& R = src.red(pid);
+  const auto& P = src.plan(pid);
+  const auto& R = src.red(pid);
   if ((int64_t)blockIdx.x * kWallsPerBlock >= P.n_wall) return;  // uniform
   const int w = threadIdx.x % kWallsPerBlock;
   const int g = threadIdx.x / kWallsPerBlock;
@@ -1556,7 +1627,7 @@ __global__ void k_reduce_final(S src) {
   __shared__ double red[1024];
   const int pid = blockIdx.x;
   if (!src.live(pid)) return;
-  const obte_bte_plan_t& P = src.plan(pid);
+  const auto& P = src.plan(pid);
   const int nblk = (P.n + 255) / 256;
   double s = 0.0;
   for (int i = threadIdx.x; i < nblk; i += blockDim.x) s += P.scratch[i];
@@ -1583,8 +1654,8 @@ __global__ void k_log_active(Many src, int n_problems, int32_t* out) {
 __global__ void k_commit(Many src) {
   const int pid = blockIdx.y;
   if (!src.live(pid)) return;
-  const obte_bte_plan_t& P = src.plan(pid);
-  const obte_reduce_io_t& R = src.red(pid);
+  const auto& P = src.plan(pid);
+  const auto& R = src.red(pid);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < P.n) const_cast<double*>(P.delta_t)[i] = R.dt_new[i];
   if (i < P.n_wall) const_cast<double*>(P.tb)[i] = R.tb_new[i];
@@ -1688,6 +1759,25 @@ struct ExtendOne {
 };
 
 template <int MC, int KIN>
+struct ExtendMany {
+  static void run(const Many& src, int groups, int block, const BatchQueues& Q, int max_extend, int32_t* ext_max,
+                  hipStream_t st) {
+    dim3 grid(groups), blk(block);
+    const size_t win = Q.cap > 0 ? ((size_t)Q.cap * MC / 2 + 1) * sizeof(double2) : 0;  // window + zero slot
+    const size_t lds = win > kGsLdsBytes ? win : kGsLdsBytes;
+    if constexpr (MC <= 16) {
+      if (Q.cap > 0) {
+        hipLaunchKernelGGL((k_extend_batch<MC, KIN, true>), grid, blk, lds, st, src, Q.qoff, Q.qcnt, Q.woff, Q.wcnt,
+                           Q.cap, max_extend, ext_max);
+        return;
+      }
+    }
+    hipLaunchKernelGGL((k_extend_batch<MC, KIN, false>), grid, blk, lds, st, src, Q.qoff, Q.qcnt, Q.woff, Q.wcnt, 0,
+                       max_extend, ext_max);
+  }
+};
+
+template <int MC, int KIN>
 struct SweepMany {
   template <bool WIN>
   static void go(const Many& src, int mode, int groups, int block, const BatchQueues& Q, hipStream_t st) {
@@ -1783,13 +1873,13 @@ int check_batch(const obte_bte_batch_t* B, const char* who) {
 
 Many make_many(const obte_bte_batch_t* B, int mode) {
   Many m{};
-  m.P = B->plans;
-  m.G = B->gcrs;
-  m.io = B->io + (int64_t)mode * B->n_problems;
-  m.rio = B->rio;
-  m.livef = B->live;
-  m.tpid = B->task_pid;
-  m.tloc = B->task_local;
+  m.P = (const OBTE_CONST obte_bte_plan_t*)B->plans;
+  m.G = (const OBTE_CONST obte_gcr_t*)B->gcrs;
+  m.io = (const OBTE_CONST obte_sweep_io_t*)(B->io + (int64_t)mode * B->n_problems);
+  m.rio = (const OBTE_CONST obte_reduce_io_t*)B->rio;
+  m.livef = (const OBTE_CONST int32_t*)B->live;
+  m.tpid = (const OBTE_CONST int32_t*)B->task_pid;
+  m.tloc = (const OBTE_CONST int32_t*)B->task_local;
   return m;
 }
 
@@ -1965,6 +2055,23 @@ int obte_batch_gcr(const obte_bte_batch_t* b, int32_t op, obte_stream_t stream) 
   if (op != 1) hipLaunchKernelGGL((k_zero_active<Many>), dim3(b->n_problems), dim3(64), 0, st, src);
   dispatch<GcrL>(b->mc, 2, src, (int)op, b->n_tasks, b->block, st);
   return launch_status("obte_batch_gcr: launch failed");
+}
+
+int obte_batch_extend(const obte_bte_batch_t* b, int32_t max_extend, int32_t* ext_max, obte_stream_t stream) {
+  int rc = check_batch(b, "obte_batch_extend: invalid batch");
+  if (rc) return rc;
+  if (max_extend < 0 || !ext_max) return obte_fail(OBTE_EINVAL, "obte_batch_extend: max_extend < 0 or null ext_max");
+  hipStream_t st = (hipStream_t)stream;
+  Many src = make_many(b, 1);
+  hipLaunchKernelGGL((k_zero_active<Many>), dim3(b->n_problems), dim3(64), 0, st, src);
+  hipLaunchKernelGGL(k_zero_batch_extend, dim3(1), dim3(64), 0, st, b->queue_cnt, b->wave_cnt, ext_max);
+  if (max_extend > 0) {
+    // every task by a whole workgroup: the window holds the widest plan's level pairs
+    BatchQueues Q{b->queue_off, b->queue_cnt, window_cap(b->sweep_window, b->lvl_pair_max, b->mc), b->wave_off,
+                  b->wave_cnt, 0};
+    dispatch<ExtendMany>(b->mc, b->kin, src, b->groups, b->block, Q, (int)max_extend, ext_max, st);
+  }
+  return launch_status("obte_batch_extend: launch failed");
 }
 
 int obte_batch_gcr_reset(const obte_bte_batch_t* b, int32_t use_delta_t, obte_stream_t stream) {

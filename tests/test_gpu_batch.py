@@ -190,3 +190,35 @@ def test_batch_default_tuned_graphs_match_oracle():
         assert abs(out.kappa_eff / ref.kappa_eff - 1) < 1e-6
         assert np.abs(out.variables['Temperature_BTE']['data']
                       - ref.variables['Temperature_BTE']['data']).max() < 1e-8
+
+
+@pytest.mark.parametrize('mc', [8, 4])
+def test_batch_fused_extension_equals_rounds(mc):
+    """The batched extension loop in one launch (obte_batch_extend: each task
+    loops begin, transfer sweep and end on its own) gives the bits and the
+    extension counts of the round-by-round loop, eager and graph-replayed,
+    with workgroup and wave tasks in the batch."""
+    from openbte_amd.batch import BTEBatch
+    from openbte_amd.bte import SweepConfig
+    from openbte_amd.operator import build_host_operator
+    probs = [_problem(51, 8000), _problem(52, 300), _problem(53, 900)]
+    ops = [build_host_operator(p['geo'], p['mat'], p['bcs'], p['effective_thermal_conductivity']) for p in probs]
+    t0 = [np.zeros(p['geo'].n_elems) for p in probs]
+    runs = []
+    for fused, graphs in ((False, False), (True, False), (True, True)):
+        print(f'# batch mc={mc} fused={fused} graphs={graphs}', flush=True)
+        b = BTEBatch(ops, cfg=SweepConfig(mc=mc, block=256, fused_extend=fused))
+        if graphs:
+            b.enable_graphs(True)
+        b.reset(t0)
+        exts = [b.outer_iteration() for _ in range(4)]
+        runs.append((exts, [d.buf['dt'][0].cpu().numpy() for d in b.devs],
+                     [d.buf['kappa'].cpu().numpy() for d in b.devs], [d.buf['j'].cpu().numpy() for d in b.devs]))
+        del b
+    ref = runs[0]
+    assert ref[0][0] > 1
+    for r in runs[1:]:
+        assert r[0] == ref[0]
+        for k in (1, 2, 3):
+            for a, e in zip(r[k], ref[k]):
+                np.testing.assert_array_equal(a, e)
