@@ -121,8 +121,14 @@ def build_host_operator(geo, mat, bcs, effective_thermal_conductivity=None, heat
         s_diffuse = np.asarray(geo.side_physical_regions[bcs.diffuse], dtype=np.int64)
         i_diffuse = sem[s_diffuse, 0]
         dbs = na[s_diffuse] / vol[i_diffuse][:, None]
-        Gb = np.einsum('mqj,sj->mqs', np.asarray(mat.sigma)[:, :, :dim], dbs)
-        tot = np.clip(Gb, None, 0).sum(axis=(0, 1))
+        # tot_s = sum over the modes of min(sigma . dbs_s, 0), a chunk of
+        # sides at a time (the full modes x sides array runs to GBs on 3D
+        # meshes; the values per side are the same)
+        sig = np.ascontiguousarray(np.asarray(mat.sigma)[:, :, :dim])
+        tot = np.empty(len(s_diffuse))
+        for c0 in range(0, len(s_diffuse), 512):
+            Gb = np.einsum('mqj,sj->mqs', sig, dbs[c0:c0 + 512])
+            tot[c0:c0 + 512] = np.minimum(Gb, 0.0, out=Gb).sum(axis=(0, 1))
         rinv = np.divide(1.0, tot, out=np.zeros_like(tot), where=tot != 0)
         wc.append(i_diffuse)
         wside.append(np.arange(len(s_diffuse)))
