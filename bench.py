@@ -598,7 +598,7 @@ def north_star_legs(args, geo, mat, bcs, eff, c1, value):
         print(f'# 2k converged: kappa rel err {chk["kappa_rel_err"]:.2e}, T err '
               f'{chk["temperature_max_abs_err_K"]:.2e} K', file=sys.stderr, flush=True)
     ba = argparse.Namespace(config='batch64', steps=5, warmup=1, mc=None, block=None,
-                            no_window=args.no_window, no_graphs=False)
+                            no_window=args.no_window, no_graphs=False, no_fused=False)
     b = measure_batch(ba)
     north['batched_over_plain'] = b['config']['batched_over_plain']
     north['batched'] = {'value': b['value'], 'ms_per_step': b['ms_per_step'], 'steps': b['steps'],
