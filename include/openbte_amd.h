@@ -55,6 +55,29 @@ int obte_sched_export(void* handle, int32_t* order, int32_t* pos, int32_t* lvl_p
 int obte_sched_export_slots(void* handle, int32_t kin, int32_t* nb_src, int32_t* nb_face);
 int obte_sched_destroy(void* handle);
 
+/* ---- Fourier multigrid setup (host) ------------------------------------ */
+/* Smoothed-aggregation hierarchy of the symmetric Fourier operator (the rows
+ * of obte_fourier_plan_t: diag, off-diagonal weights w over row_ptr / col, in
+ * a cell order where every `agg` consecutive rows form an aggregate, e.g. a
+ * Morton order of the centroids).  P = (I - omega D^-1 A) Pt with Pt
+ * piecewise constant on levels with at most smooth_max_row off-diagonal
+ * entries per row on average (P = Pt on denser levels), coarse operator
+ * P^T A P, levels until one has at most coarse_max rows.  Replaces the SuperLU factorisation of openbte/fourier.py:
+ * 109-118 as the solver of the correction loop (no counterpart in the
+ * reference).  Host memory only; the handle is freed by obte_amg_destroy. */
+int obte_amg_create(int32_t n, const int32_t* row_ptr, const int32_t* col, const double* w, const double* diag,
+                    int32_t agg, int32_t coarse_max, double omega, int32_t smooth_max_row, void** handle);
+/* *n_levels = levels including the coarsest; sizes[4*l + {0,1,2,3}] = rows,
+ * off-diagonal entries, entries of P, entries of R of level l (sizes may be
+ * NULL; P and R are empty on the coarsest level). */
+int obte_amg_sizes(void* handle, int32_t* n_levels, int64_t* sizes);
+/* Copy level `level` out (any pointer may be NULL to skip that array). */
+int obte_amg_export(void* handle, int32_t level, int32_t* a_ptr, int32_t* a_col, double* a_w, double* a_diag,
+                    int32_t* p_ptr, int32_t* p_col, double* p_val, int32_t* r_ptr, int32_t* r_col, double* r_val);
+/* The coarsest level as a dense row-major matrix (rows x rows). */
+int obte_amg_export_dense(void* handle, double* dense);
+int obte_amg_destroy(void* handle);
+
 /* ---- device plan of one BTE solve ----------------------------------- */
 
 typedef struct obte_bte_plan {
@@ -313,6 +336,33 @@ int obte_batch_reduce(const obte_bte_batch_t* b, obte_stream_t stream);
 
 /* ---- Fourier (openbte/fourier.py:15-185) ------------------------------ */
 
+/* One level of the multigrid hierarchy on the device.  The operator rows are
+ * (A x)_i = a_diag_i x_i - sum_k a_w[k] x[a_col[k]] over a_ptr[i] .. a_ptr[i+1];
+ * P (n x n_next) prolongs from the next level, R = P^T restricts to it.  b, x,
+ * res and z are work vectors of n entries (b: right-hand side of the level,
+ * x: pre-smoothed iterate, res: its residual, z: the level's result).
+ * a_entries / r_entries: off-diagonal entries of the level's operator and
+ * entries of R (they set how many lanes share a row). */
+typedef struct obte_amg_level {
+  int32_t n;
+  int64_t a_entries;
+  int64_t r_entries;
+  const int32_t* a_ptr;
+  const int32_t* a_col;
+  const double* a_w;
+  const double* a_diag;
+  const int32_t* p_ptr;
+  const int32_t* p_col;
+  const double* p_val;
+  const int32_t* r_ptr;
+  const int32_t* r_col;
+  const double* r_val;
+  double* b;
+  double* x;
+  double* res;
+  double* z;
+} obte_amg_level_t;
+
 typedef struct obte_fourier_plan {
   int32_t n;          /* cells */
   int32_t dim;
@@ -345,17 +395,23 @@ typedef struct obte_fourier_plan {
   const double* c_coef;      /* v_orth (periodic) or conductance-limited v_orth (mixed) */
   double c_value;            /* periodic jump or wall temperature */
   double normalization;
-  /* two-level preconditioner of the multi-launch CG (optional): aggregate k is
-   * cells 256 k .. 256 k + 255 (cells numbered so that these are compact);
-   * coarse_inv is the inverse of the aggregated operator P^T A P (plus a
-   * constant when A is singular) */
-  int32_t n_coarse;          /* aggregates (= ceil(n / 256)), or 0: Jacobi only */
-  const double* coarse_inv;  /* [n_coarse][n_coarse] */
+  /* multigrid preconditioner of the multi-launch CG (optional): one V-cycle of
+   * the smoothed-aggregation hierarchy of obte_amg_create per CG iteration.
+   * levels[0] is the plan's operator (its A fields are not read); the last
+   * level restricts to the dense coarsest level, solved by coarse_inv. */
+  int32_t n_levels;          /* levels above the coarsest, or 0: Jacobi only */
+  const obte_amg_level_t* levels;  /* host array of n_levels descriptors */
+  int32_t n_coarse;          /* rows of the coarsest level */
+  const double* coarse_inv;  /* [n_coarse][n_coarse] inverse of the coarsest operator (plus a constant when
+                                it is singular) */
+  double* coarse_b;          /* [n_coarse] work vectors of the coarsest level */
+  double* coarse_x;
 } obte_fourier_plan_t;
 
+
 /* Solve F T = B + C (openbte/fourier.py:152-156) as the symmetric system
- * A T = V (B + C) by preconditioned CG (Jacobi, plus the additive coarse
- * correction P coarse_inv P^T r when n_coarse > 0), T holding the initial guess;
+ * A T = V (B + C) by preconditioned CG (Jacobi, or one multigrid V-cycle per
+ * iteration when n_levels > 0), T holding the initial guess;
  * with a pinned cell the pinned row is replaced by the compatibility
  * condition and T is centred by (max + min) / 2.  ws: 6n + 64 doubles; scal: 16
  * doubles.  iters_out / relres_out are host pointers (may be NULL). */

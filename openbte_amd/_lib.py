@@ -76,7 +76,18 @@ class FourierPlan(C.Structure):
         ('side_e', _p), ('side_w', _p), ('side_vn', _p), ('slot_side', _p), ('slot_sgn', _p),
         ('n_contact', _i32), ('contact_kind', _i32), ('ce1', _p), ('ce2', _p), ('c_coef', _p),
         ('c_value', _f64), ('normalization', _f64),
-        ('n_coarse', _i32), ('coarse_inv', _p),
+        ('n_levels', _i32), ('levels', _p), ('n_coarse', _i32), ('coarse_inv', _p), ('coarse_b', _p),
+        ('coarse_x', _p),
+    ]
+
+
+class AmgLevel(C.Structure):
+    """Mirror of obte_amg_level_t."""
+    _fields_ = [
+        ('n', _i32), ('a_entries', _i64), ('r_entries', _i64), ('a_ptr', _p), ('a_col', _p), ('a_w', _p),
+        ('a_diag', _p),
+        ('p_ptr', _p), ('p_col', _p), ('p_val', _p), ('r_ptr', _p), ('r_col', _p), ('r_val', _p),
+        ('b', _p), ('x', _p), ('res', _p), ('z', _p),
     ]
 
 
@@ -88,6 +99,11 @@ _SIGNATURES = {
     'obte_sched_export': [_p, _p, _p, _p, _p, _p, _p, _p, _p],
     'obte_sched_export_slots': [_p, _i32, _p, _p],
     'obte_sched_destroy': [_p],
+    'obte_amg_create': [_i32, _p, _p, _p, _p, _i32, _i32, _f64, _i32, C.POINTER(_p)],
+    'obte_amg_sizes': [_p, C.POINTER(_i32), _p],
+    'obte_amg_export': [_p, _i32, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p],
+    'obte_amg_export_dense': [_p, _p],
+    'obte_amg_destroy': [_p],
     'obte_bte_assemble': [C.POINTER(BtePlan), _p],
     'obte_bte_source': [C.POINTER(BtePlan), _p],
     'obte_bte_sweep': [C.POINTER(BtePlan), _i32, _p, _p, _p, _p, _p],

@@ -2,8 +2,8 @@
 
 The library holds the HIP kernels (bte_kernels.hip, fourier_kernels.hip, tim_kernels.hip,
 material_kernels.hip),
-the native host scheduler (schedule.cpp) and the C-ABI error state
-(capi.cpp).  All HIP code is compiled with packed-FP32 instruction
+the native host scheduler (schedule.cpp), the Fourier multigrid setup
+(amg.cpp) and the C-ABI error state (capi.cpp).  All HIP code is compiled with packed-FP32 instruction
 generation disabled.
 """
 from __future__ import annotations
@@ -21,7 +21,7 @@ LIB = os.path.join(HERE, 'libopenbte_amd.so')
 ARCH = os.environ.get('OBTE_OFFLOAD_ARCH', 'gfx950')
 
 HIP_SOURCES = ['bte_kernels.hip', 'fourier_kernels.hip', 'tim_kernels.hip', 'material_kernels.hip']
-CPP_SOURCES = ['schedule.cpp', 'capi.cpp']
+CPP_SOURCES = ['schedule.cpp', 'amg.cpp', 'capi.cpp']
 
 
 def _hipcc() -> str:

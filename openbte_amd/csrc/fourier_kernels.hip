@@ -88,64 +88,38 @@ __device__ __forceinline__ double sum_partials(const double* part, int nb, doubl
 }
 
 // Partial sums of the multi-launch CG live after the five work vectors:
-// [pq | rz | rr | bb | rc | ec], nb doubles each (6 nb <= n + 64 for every n).
-// rc[k] = sum of r over block k (the restriction P^T r: block k is aggregate
-// k), ec = coarse_inv rc (the coarse correction, one value per aggregate).
-// The preconditioned residual is z + ec[i / 256] with z = r / diag.
-constexpr int kAggShift = 8;  // 256 cells per aggregate = one block of the CG kernels
+// [pq | rz | rr | bb] x nb doubles (4 nb <= n + 64 for every n).
+// With the multigrid preconditioner the V-cycle's last kernel forms the
+// preconditioned residual z and the r.z partials; without it z = r / diag
+// is formed by the kernels that update r.
 
-// r = bh - A x, z = r / diag; partials of r.z, r.r, bh.bh and r
+// r = bh - A x; partials of r.r and bh.bh, and (Jacobi) z = r / diag with the
+// partials of r.z
 __global__ void k_cg_init(obte_fourier_plan_t P, const double* bh, const double* x, double* r, double* z,
                           double* part, int nb) {
   __shared__ double red[16];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool jac = P.n_levels == 0;
   double rz = 0.0, rr = 0.0, bb = 0.0;
   if (i < P.n) {
     const double ri = bh[i] - apply_row(P, x, i);
-    const double zi = ri / P.diag[i];
     r[i] = ri;
-    z[i] = zi;
-    rz = ri * zi;
+    if (jac) {
+      const double zi = ri / P.diag[i];
+      z[i] = zi;
+      rz = ri * zi;
+    }
     rr = ri * ri;
     bb = bh[i] * bh[i];
   }
-  double rsum = 0.0;
-  if (i < P.n) rsum = r[i];
-  const double rs = wg_sum(rsum, red);
-  rz = wg_sum(rz, red);
+  if (jac) rz = wg_sum(rz, red);
   rr = wg_sum(rr, red);
   bb = wg_sum(bb, red);
   if (threadIdx.x == 0) {
-    part[nb + blockIdx.x] = rz;
+    if (jac) part[nb + blockIdx.x] = rz;
     part[2 * nb + blockIdx.x] = rr;
     part[3 * nb + blockIdx.x] = bb;
-    part[4 * nb + blockIdx.x] = rs;
   }
-}
-
-// ec = coarse_inv rc: one wave per aggregate (four per block), each row
-// summed in a fixed order (lane-strided partial sums, then an xor tree).
-constexpr int kCoarseRows = 4;
-
-__global__ void k_cg_coarse(obte_fourier_plan_t P, double* part, int nb) {
-  const int nc = P.n_coarse;
-  const double* rc = part + 4 * nb;
-  double* ec = part + 5 * nb;
-  const int lane = threadIdx.x & 63;
-  const int I = blockIdx.x * kCoarseRows + (threadIdx.x >> 6);  // uniform over the wave
-  if (I >= nc) return;
-  const double* row = P.coarse_inv + (int64_t)I * nc;
-  double s = 0.0;
-  for (int J = lane; J < nc; J += 64) s += row[J] * rc[J];
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-  if (lane == 0) ec[I] = s;
-}
-
-// rc . ec (the coarse part of r.M^-1 r), computed the same way by every block
-__device__ __forceinline__ double coarse_dot(const obte_fourier_plan_t& P, const double* part, int nb, double* red) {
-  double s = 0.0;
-  for (int I = threadIdx.x; I < P.n_coarse; I += blockDim.x) s += part[4 * nb + I] * part[5 * nb + I];
-  return wg_sum(s, red);
 }
 
 // scal[S_RR], scal[S_BB] from the partials (host convergence test)
@@ -174,8 +148,6 @@ __global__ void k_cg_pq(obte_fourier_plan_t P, const double* z, const double* po
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool row = i < P.n;
   const bool lag = k > 0;
-  const bool two = P.n_coarse > 0;
-  const double* ec = part + 5 * nb;
   double di = 0.0, zi = 0.0, pi0 = 0.0;
   double we[kRowHoist], ze[kRowHoist], pe[kRowHoist];
   int e0 = 0, len = 0;
@@ -183,7 +155,7 @@ __global__ void k_cg_pq(obte_fourier_plan_t P, const double* z, const double* po
     e0 = P.row_ptr[i];
     len = P.row_ptr[i + 1] - e0;
     di = P.diag[i];
-    zi = two ? z[i] + ec[i >> kAggShift] : z[i];
+    zi = z[i];
     pi0 = lag ? pold[i] : 0.0;
   }
 #pragma unroll
@@ -194,12 +166,11 @@ __global__ void k_cg_pq(obte_fourier_plan_t P, const double* z, const double* po
     if (e < len) {
       const int j = P.col[e0 + e];
       we[e] = P.w[e0 + e];
-      ze[e] = two ? z[j] + ec[j >> kAggShift] : z[j];
+      ze[e] = z[j];
       pe[e] = lag ? pold[j] : 0.0;
     }
   }
-  double rz = sum_partials(part + nb, nb, red);
-  if (two) rz += coarse_dot(P, part, nb, red);
+  const double rz = sum_partials(part + nb, nb, red);
   double beta = 0.0;
   if (lag) {
     const double rzo = scal[S_RZ0 + ((k - 1) & 1)];
@@ -215,8 +186,7 @@ __global__ void k_cg_pq(obte_fourier_plan_t P, const double* z, const double* po
       if (e < len) v -= we[e] * (lag ? ze[e] + beta * pe[e] : ze[e]);
     for (int e = kRowHoist; e < len; ++e) {  // rows longer than the hoisted part
       const int j = P.col[e0 + e];
-      const double zj = two ? z[j] + ec[j >> kAggShift] : z[j];
-      v -= P.w[e0 + e] * (lag ? zj + beta * pold[j] : zj);
+      v -= P.w[e0 + e] * (lag ? z[j] + beta * pold[j] : z[j]);
     }
     pnew[i] = pi;
     q[i] = v;
@@ -227,14 +197,15 @@ __global__ void k_cg_pq(obte_fourier_plan_t P, const double* z, const double* po
 }
 
 // Iteration k, second half: alpha = rz_k / (p.q) (every block sums the p.q
-// partials); x += alpha p, r -= alpha q, z = r / diag; partials r.z, r.r
-// and (two-level) r.
-// The row's loads go out before the partial sums.
+// partials); x += alpha p, r -= alpha q; partials r.r and (Jacobi) z = r /
+// diag with the partials of r.z.  The row's loads go out before the partial
+// sums.
 __global__ void k_cg_xr(obte_fourier_plan_t P, double* x, double* r, double* z, const double* p, const double* q,
                         double* part, const double* scal, int nb, int k) {
   __shared__ double red[16];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool row = i < P.n;
+  const bool jac = P.n_levels == 0;
   double xi = 0.0, pi = 0.0, ri = 0.0, qi = 0.0, di = 1.0;
   if (row) {
     xi = x[i];
@@ -246,24 +217,165 @@ __global__ void k_cg_xr(obte_fourier_plan_t P, double* x, double* r, double* z, 
   const double pq = sum_partials(part, nb, red);
   const double rz = scal[S_RZ0 + (k & 1)];
   const double alpha = pq != 0.0 ? rz / pq : 0.0;
-  double rzn = 0.0, rr = 0.0, rn = 0.0;
+  double rzn = 0.0, rr = 0.0;
   if (row) {
     x[i] = xi + alpha * pi;
-    rn = ri - alpha * qi;
-    const double zn = rn / di;
+    const double rn = ri - alpha * qi;
     r[i] = rn;
-    z[i] = zn;
-    rzn = rn * zn;
+    if (jac) {
+      const double zn = rn / di;
+      z[i] = zn;
+      rzn = rn * zn;
+    }
     rr = rn * rn;
   }
-  const double rs = P.n_coarse > 0 ? wg_sum(rn, red) : 0.0;
-  rzn = wg_sum(rzn, red);
+  if (jac) rzn = wg_sum(rzn, red);
   rr = wg_sum(rr, red);
   if (threadIdx.x == 0) {
-    part[nb + blockIdx.x] = rzn;
+    if (jac) part[nb + blockIdx.x] = rzn;
     part[2 * nb + blockIdx.x] = rr;
-    part[4 * nb + blockIdx.x] = rs;
   }
+}
+
+// ---------------------------------------------------------------------------
+// Multigrid V-cycle z = M^-1 b (one per CG iteration, b = the CG residual on
+// the finest level).  Per level: pre-smoothing from zero x = w b / d and its
+// residual res = b - A x (k_mg_pre), restriction b_next = R res
+// (k_mg_restrict); the dense coarsest level x = coarse_inv b (k_mg_coarse);
+// on the way up y = x + P z_next and the post-smoothing z = y + w (b - A y) /
+// d (k_mg_post; y of the neighbours is formed on the fly: the hierarchy keeps
+// the prolongations of wide levels unsmoothed, so a row's neighbours and
+// their prolongation entries stay few).  Every row sum runs in a fixed order.
+// Per CG iteration: three launches per level and one for the coarsest.
+// ---------------------------------------------------------------------------
+constexpr double kMgOmega = 2.0 / 3.0;   // Jacobi weight of the smoother
+constexpr int kMgCoarseMax = 4096;       // rows of the dense coarsest level at most
+
+// one level's operator as kernel arguments (level 0: the plan's rows)
+struct MgOp {
+  int n;
+  const int32_t* ptr;
+  const int32_t* col;
+  const double* w;
+  const double* d;
+};
+
+__device__ __forceinline__ MgOp mg_op(const obte_amg_level_t& L) { return MgOp{L.n, L.a_ptr, L.a_col, L.a_w, L.a_diag}; }
+
+// A row of a level is handled by G consecutive lanes (G = 1, 4 or 16 by the
+// level's entries per row: the row loops are chains of dependent gathers, so
+// wide rows are split across lanes to keep more loads in flight).  Lane g of
+// the group takes entries g, g + G, ...; the partial sums are combined by an
+// xor tree over the group (fixed order); every lane of the group gets the
+// sum.  Out-of-range rows (i >= n) run with empty rows so that every lane
+// reaches the shuffles.
+template <int G>
+__device__ __forceinline__ double group_sum(double v) {
+#pragma unroll
+  for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// pre-smoothing of row i: x_i = w b_i / d_i, res_i = b_i - (A x)_i
+template <int G>
+__device__ __forceinline__ void mg_pre_row(const MgOp& A, const double* b, double* x, double* res, int i, int g) {
+  const bool row = i < A.n;
+  double acc = 0.0;
+  const int k1 = row ? A.ptr[i + 1] : 0;
+  for (int k = (row ? A.ptr[i] : 0) + g; k < k1; k += G) {
+    const int j = A.col[k];
+    acc += A.w[k] * (kMgOmega * b[j] / A.d[j]);
+  }
+  acc = group_sum<G>(acc);
+  if (row && g == 0) {
+    const double di = A.d[i];
+    const double xi = kMgOmega * b[i] / di;
+    x[i] = xi;
+    res[i] = b[i] - (di * xi - acc);
+  }
+}
+
+// restriction of row I: b_next_I = sum_k R_Ik res_k
+template <int G>
+__device__ __forceinline__ void mg_restrict_row(const obte_amg_level_t& L, int nn, const double* res, double* bn,
+                                                int I, int g) {
+  const bool row = I < nn;
+  double s = 0.0;
+  const int k1 = row ? L.r_ptr[I + 1] : 0;
+  for (int k = (row ? L.r_ptr[I] : 0) + g; k < k1; k += G) s += L.r_val[k] * res[L.r_col[k]];
+  s = group_sum<G>(s);
+  if (row && g == 0) bn[I] = s;
+}
+
+// y_i = x_i + (P z_next)_i
+__device__ __forceinline__ double mg_prolong(const obte_amg_level_t& L, const double* x, const double* zn, int i) {
+  double y = x[i];
+  for (int k = L.p_ptr[i]; k < L.p_ptr[i + 1]; ++k) y += L.p_val[k] * zn[L.p_col[k]];
+  return y;
+}
+
+// post-smoothing of row i: z_i = y_i + w (b_i - (A y)_i) / d_i; returns z_i
+// on lane 0 of the group (0 elsewhere and for rows past n)
+template <int G>
+__device__ __forceinline__ double mg_post_row(const MgOp& A, const obte_amg_level_t& L, const double* b,
+                                              const double* x, const double* zn, int i, int g) {
+  const bool row = i < A.n;
+  double acc = 0.0;
+  const int k1 = row ? A.ptr[i + 1] : 0;
+  for (int k = (row ? A.ptr[i] : 0) + g; k < k1; k += G) acc += A.w[k] * mg_prolong(L, x, zn, A.col[k]);
+  acc = group_sum<G>(acc);
+  if (!row || g != 0) return 0.0;
+  const double yi = mg_prolong(L, x, zn, i);
+  const double di = A.d[i];
+  return yi + kMgOmega * (b[i] - (di * yi - acc)) / di;
+}
+
+template <int G>
+__global__ void k_mg_pre(MgOp A, const double* b, double* x, double* res) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  mg_pre_row<G>(A, b, x, res, t / G, t % G);
+}
+
+template <int G>
+__global__ void k_mg_restrict(obte_amg_level_t L, int n_next, const double* res, double* bn) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  mg_restrict_row<G>(L, n_next, res, bn, t / G, t % G);
+}
+
+// rz: with `part` (the finest level, G = 1: b is the CG residual, one row
+// per thread), the r.z partial of each block goes to part[nb + block]
+template <int G>
+__global__ void k_mg_post(MgOp A, obte_amg_level_t L, const double* b, const double* x, const double* zn, double* z,
+                          double* part, int nb) {
+  __shared__ double red[16];
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = t / G, g = t % G;
+  const double zi = mg_post_row<G>(A, L, b, x, zn, i, g);
+  double rz = 0.0;
+  if (i < A.n && g == 0) {
+    z[i] = zi;
+    rz = b[i] * zi;
+  }
+  if (part != nullptr) {
+    rz = wg_sum(rz, red);
+    if (threadIdx.x == 0) part[nb + blockIdx.x] = rz;
+  }
+}
+
+// coarsest level: x = coarse_inv b, one wave per row (lanes stride over the
+// row: coalesced loads), each row summed in a fixed order (lane partial sums,
+// then an xor tree); four rows per block
+constexpr int kMgCoarseRows = 4;
+
+__global__ void k_mg_coarse(int nc, const double* cinv, const double* b, double* x) {
+  const int lane = threadIdx.x & 63;
+  const int I = blockIdx.x * kMgCoarseRows + (threadIdx.x >> 6);  // uniform over the wave
+  if (I >= nc) return;
+  const double* row = cinv + (int64_t)I * nc;
+  double s = 0.0;
+  for (int J = lane; J < nc; J += 64) s += row[J] * b[J];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if (lane == 0) x[I] = s;
 }
 
 // ---------------------------------------------------------------------------
@@ -539,7 +651,75 @@ int check(const obte_fourier_plan_t* P) {
   if (!P || P->n <= 0 || (P->dim != 2 && P->dim != 3) || P->nv <= 0 || P->nv > 4)
     return obte_fail(OBTE_EINVAL, "fourier plan: invalid sizes");
   if (P->pin >= P->n) return obte_fail(OBTE_EINVAL, "fourier plan: pin out of range");
+  if (P->n_levels < 0) return obte_fail(OBTE_EINVAL, "fourier plan: n_levels < 0");
   return OBTE_OK;
+}
+
+// The multigrid hierarchy fits the kernels: level 0 is the plan's operator,
+// every level has its operator (above level 0), transfer operators and work
+// vectors, and the coarsest level its inverse.
+int check_mg(const obte_fourier_plan_t& P) {
+  const obte_amg_level_t* L = P.levels;
+  if (!L || L[0].n != P.n) return obte_fail(OBTE_EINVAL, "fourier plan: levels[0].n must equal n");
+  if (P.n_coarse <= 0 || !P.coarse_inv || !P.coarse_b || !P.coarse_x)
+    return obte_fail(OBTE_EINVAL, "fourier plan: multigrid needs the coarsest level and its inverse");
+  if (P.n_coarse > kMgCoarseMax) return obte_fail(OBTE_ELIMIT, "fourier plan: coarsest level over 4096 rows");
+  for (int l = 0; l < P.n_levels; ++l) {
+    if (L[l].n <= 0 || !L[l].p_ptr || !L[l].p_col || !L[l].p_val || !L[l].r_ptr || !L[l].r_col || !L[l].r_val ||
+        !L[l].x || !L[l].res)
+      return obte_fail(OBTE_EINVAL, "fourier plan: incomplete multigrid level");
+    if (l > 0 && (!L[l].a_ptr || !L[l].a_col || !L[l].a_w || !L[l].a_diag || !L[l].b || !L[l].z))
+      return obte_fail(OBTE_EINVAL, "fourier plan: incomplete multigrid level");
+  }
+  return OBTE_OK;
+}
+
+// z = M^-1 r: one V-cycle; the r.z partials go to part[nb ..].
+void vcycle(const obte_fourier_plan_t& P, const double* r, double* z, double* part, int nb, hipStream_t st) {
+  const obte_amg_level_t* L = P.levels;
+  const int nl = P.n_levels;
+  auto op = [&](int l) {
+    return l == 0 ? MgOp{P.n, P.row_ptr, P.col, P.w, P.diag} : MgOp{L[l].n, L[l].a_ptr, L[l].a_col, L[l].a_w, L[l].a_diag};
+  };
+  // lanes per row from a level's entries per row (level 0 stays at one row
+  // per thread: its post kernel forms the CG's r.z partials per block of rows)
+  auto lanes = [](int64_t entries, int rows, bool finest) {
+    if (finest || entries <= 6 * (int64_t)rows) return 1;
+    return entries <= 24 * (int64_t)rows ? 4 : 16;
+  };
+  auto grid = [](int rows, int g) { return dim3((unsigned)(((int64_t)rows * g + kB - 1) / kB)); };
+  for (int l = 0; l < nl; ++l) {
+    const double* b = l == 0 ? r : L[l].b;
+    const int nn = l + 1 < nl ? L[l + 1].n : P.n_coarse;
+    double* bn = l + 1 < nl ? L[l + 1].b : P.coarse_b;
+    const MgOp A = op(l);
+    const int g_a = lanes(L[l].a_entries, A.n, l == 0);
+    const int g_r = lanes(L[l].r_entries, nn, false);
+    if (g_a == 1) hipLaunchKernelGGL(k_mg_pre<1>, grid(A.n, 1), dim3(kB), 0, st, A, b, L[l].x, L[l].res);
+    else if (g_a == 4) hipLaunchKernelGGL(k_mg_pre<4>, grid(A.n, 4), dim3(kB), 0, st, A, b, L[l].x, L[l].res);
+    else hipLaunchKernelGGL(k_mg_pre<16>, grid(A.n, 16), dim3(kB), 0, st, A, b, L[l].x, L[l].res);
+    const double* res = L[l].res;
+    if (g_r == 1) hipLaunchKernelGGL(k_mg_restrict<1>, grid(nn, 1), dim3(kB), 0, st, L[l], nn, res, bn);
+    else if (g_r == 4) hipLaunchKernelGGL(k_mg_restrict<4>, grid(nn, 4), dim3(kB), 0, st, L[l], nn, res, bn);
+    else hipLaunchKernelGGL(k_mg_restrict<16>, grid(nn, 16), dim3(kB), 0, st, L[l], nn, res, bn);
+  }
+  hipLaunchKernelGGL(k_mg_coarse, dim3((P.n_coarse + kMgCoarseRows - 1) / kMgCoarseRows), dim3(64 * kMgCoarseRows),
+                     0, st, P.n_coarse, P.coarse_inv, (const double*)P.coarse_b, P.coarse_x);
+  for (int l = nl - 1; l >= 0; --l) {
+    const double* b = l == 0 ? r : L[l].b;
+    const double* zn = l + 1 < nl ? L[l + 1].z : P.coarse_x;
+    double* zl = l == 0 ? z : L[l].z;
+    const MgOp A = op(l);
+    const double* x = L[l].x;
+    const int g_a = lanes(L[l].a_entries, A.n, l == 0);
+    if (g_a == 1)
+      hipLaunchKernelGGL(k_mg_post<1>, grid(A.n, 1), dim3(kB), 0, st, A, L[l], b, x, zn, zl, l == 0 ? part : nullptr,
+                         nb);
+    else if (g_a == 4)
+      hipLaunchKernelGGL(k_mg_post<4>, grid(A.n, 4), dim3(kB), 0, st, A, L[l], b, x, zn, zl, nullptr, nb);
+    else
+      hipLaunchKernelGGL(k_mg_post<16>, grid(A.n, 16), dim3(kB), 0, st, A, L[l], b, x, zn, zl, nullptr, nb);
+  }
 }
 
 // Device address of `scratch` on the current device, looked up once per
@@ -601,15 +781,16 @@ int obte_fourier_solve(const obte_fourier_plan_t* plan, const double* b, const d
     return OBTE_OK;
   }
 
+  if (P.n_levels > 0) {
+    rc = check_mg(P);
+    if (rc) return rc;
+  }
   hipLaunchKernelGGL(k_rhs, dim3(nb), dim3(kB), 0, st, P, b, c, bh, part);
   if (P.pin >= 0) hipLaunchKernelGGL(k_rhs_fix, dim3(1), dim3(1024), 0, st, P, bh, part, (int)nb);
   // bh is dead after the initial residual: it holds the second direction buffer
   double* pbuf[2] = {p, bh};
-  if (P.n_coarse != 0 && (P.n_coarse != (int)nb || !P.coarse_inv))
-    return obte_fail(OBTE_EINVAL, "obte_fourier_solve: n_coarse must be ceil(n / 256) with coarse_inv");
-  const unsigned nbc = (unsigned)((P.n_coarse + kCoarseRows - 1) / kCoarseRows);
   hipLaunchKernelGGL(k_cg_init, dim3(nb), dim3(kB), 0, st, P, bh, t, r, z, part, (int)nb);
-  if (P.n_coarse > 0) hipLaunchKernelGGL(k_cg_coarse, dim3(nbc), dim3(kB), 0, st, P, part, (int)nb);
+  if (P.n_levels > 0) vcycle(P, r, z, part, (int)nb, st);
   hipLaunchKernelGGL(k_cg_norms, dim3(1), dim3(1024), 0, st, part, (int)nb, scal, 1);
   rc = status("obte_fourier_solve: init launch failed");
   if (rc) return rc;
@@ -626,7 +807,7 @@ int obte_fourier_solve(const obte_fourier_plan_t* plan, const double* b, const d
       hipLaunchKernelGGL(k_cg_pq, dim3(nb), dim3(kB), 0, st, P, z, pbuf[(it + 1) & 1], pk, q, part, scal, (int)nb,
                          it);
       hipLaunchKernelGGL(k_cg_xr, dim3(nb), dim3(kB), 0, st, P, t, r, z, pk, q, part, scal, (int)nb, it);
-      if (P.n_coarse > 0) hipLaunchKernelGGL(k_cg_coarse, dim3(nbc), dim3(kB), 0, st, P, part, (int)nb);
+      if (P.n_levels > 0) vcycle(P, r, z, part, (int)nb, st);
     }
     hipLaunchKernelGGL(k_cg_norms, dim3(1), dim3(1024), 0, st, part, (int)nb, scal, 0);
     rc = status("obte_fourier_solve: iteration launch failed");

@@ -7,13 +7,13 @@ operator (structure plus weights, once); every solve, gradient and
 correction of the correction loop (openbte/fourier.py:149-178) runs as HIP
 kernels (obte_fourier_*), with T, C and the gradient resident on the device.
 
-Meshes of 16,384 to 262,144 cells get a two-level preconditioner in the
-multi-launch CG: the cells are renumbered along a Morton curve of
-their centroids, every 256 consecutive cells form an aggregate (one block of
-the CG kernels), and the aggregated operator P^T A P is inverted once on the
-host (a constant added when A is singular); the device applies
-M^-1 r = r / diag + P (P^T A P)^-1 P^T r.  Results are returned in the
-caller's cell order.
+Meshes of more than 3,072 cells (the one-workgroup solve takes the smaller
+ones) are solved by CG preconditioned with one multigrid V-cycle per
+iteration: the cells are renumbered along a Morton curve of their centroids,
+every 8 consecutive cells form an aggregate, and a smoothed-aggregation
+hierarchy (native obte_amg_create) goes down to at most 512 rows, whose dense
+operator is inverted on the host.  Results are returned in the caller's cell
+order.
 """
 from __future__ import annotations
 
@@ -166,11 +166,22 @@ def build_fourier_host(geo, kappa, bcs, effective_thermal_conductivity=None, hea
         xy=np.ascontiguousarray(np.asarray(geo.elem_centroids, np.float64)[:, :dim]))
 
 
-# The coarse level pays from ~16k cells on (iterations at 20k cells: 534 vs
-# 881 with Jacobi, at 200k: 672 vs 2,685; at 6k: 376 vs 470, which does not
-# cover the third launch per iteration); at most 1024 aggregates (dense inverse)
-COARSE_MIN_CELLS = 16384
-COARSE_MAX_CELLS = 256 * 1024
+# Meshes above the one-workgroup solve (3,072 cells) get the multigrid
+# preconditioner: cells renumbered along a Morton curve, aggregates of 8
+# consecutive cells per level, levels down to at most 512 rows (dense inverse).
+# Measured CG iterations of one solve from zero at 200k cells: 760 with the
+# former two-level preconditioner (aggregates of 256 cells, Jacobi smoothing
+# additive), 89 with the V-cycle.
+MG_MIN_CELLS = 3073
+MG_AGG = 8
+MG_COARSE_MAX = 512
+MG_OMEGA = 2.0 / 3.0   # prolongation smoothing weight (4/3 over the bound 2 of the spectrum of D^-1 A)
+# prolongations are smoothed on levels with at most this many off-diagonal
+# entries per row: 200k triangles smooth levels 0-1 (89 CG iterations from
+# zero), 150k tetrahedra level 0 only (94; smoothing level 1 too gives 69 but
+# a coarse level of ~520 entries per row)
+MG_SMOOTH_MAX_ROW = 16
+CG_CHECK_MG = 8
 
 
 def _spread_bits(v: np.ndarray, dim: int) -> np.ndarray:
@@ -223,25 +234,60 @@ def permute_host(h: FourierHost, perm: np.ndarray) -> FourierHost:
         xy=None if h.xy is None else h.xy[perm])
 
 
-def coarse_inverse(h: FourierHost) -> np.ndarray:
-    """(P^T A P)^-1 for aggregates of 256 consecutive cells; when A is
-    singular (a pinned cell: no fixed-temperature boundary) the constant
-    trace / nc^2 is added to every entry, which leaves the correction of
-    residuals with zero sum unchanged."""
-    n = h.n
-    nc = (n + 255) // 256
-    agg = np.arange(n) >> 8
-    rows = np.repeat(agg, np.diff(h.row_ptr))
-    cols = agg[np.asarray(h.col)]
-    Ac = np.bincount(rows * nc + cols, weights=-np.asarray(h.w), minlength=nc * nc)
-    Ac += np.bincount(agg * (nc + 1), weights=h.diag, minlength=nc * nc)
-    Ac = Ac.reshape(nc, nc)
+class Hierarchy(NamedTuple):
+    """The multigrid hierarchy of a (renumbered) Fourier system: per level
+    above the coarsest its rows, operator (levels >= 1), prolongation P and
+    restriction R = P^T as CSR arrays, and the inverse of the dense coarsest
+    operator (plus a constant when the system is singular)."""
+    levels: list
+    coarse_inv: np.ndarray
+
+
+def build_hierarchy(h: FourierHost) -> Hierarchy:
+    """Smoothed-aggregation hierarchy of ``h`` (native ``obte_amg_create``)."""
+    row_ptr = np.ascontiguousarray(h.row_ptr, np.int32)
+    col = np.ascontiguousarray(h.col, np.int32)
+    w = np.ascontiguousarray(h.w, np.float64)
+    diag = np.ascontiguousarray(h.diag, np.float64)
+    handle = C.c_void_p()
+    _lib.call('obte_amg_create', h.n, _lib.ptr(row_ptr), _lib.ptr(col), _lib.ptr(w), _lib.ptr(diag), MG_AGG,
+              MG_COARSE_MAX, MG_OMEGA, MG_SMOOTH_MAX_ROW, C.byref(handle))
+    try:
+        nl = C.c_int32(0)
+        _lib.call('obte_amg_sizes', handle, C.byref(nl), None)
+        sizes = np.zeros(4 * nl.value, np.int64)
+        _lib.call('obte_amg_sizes', handle, C.byref(nl), _lib.ptr(sizes))
+        sizes = sizes.reshape(-1, 4)
+        levels = []
+        for l in range(nl.value - 1):
+            n, na, npv, nr = (int(v) for v in sizes[l])
+            nn = int(sizes[l + 1, 0])
+            lv = {'n': n,
+                  'p_ptr': np.zeros(n + 1, np.int32), 'p_col': np.zeros(max(1, npv), np.int32),
+                  'p_val': np.zeros(max(1, npv)), 'r_ptr': np.zeros(nn + 1, np.int32),
+                  'r_col': np.zeros(max(1, nr), np.int32), 'r_val': np.zeros(max(1, nr))}
+            a = [None] * 4
+            if l > 0:
+                lv.update({'a_ptr': np.zeros(n + 1, np.int32), 'a_col': np.zeros(max(1, na), np.int32),
+                           'a_w': np.zeros(max(1, na)), 'a_diag': np.zeros(n)})
+                a = [_lib.ptr(lv[k]) for k in ('a_ptr', 'a_col', 'a_w', 'a_diag')]
+            _lib.call('obte_amg_export', handle, l, *a,
+                      *[_lib.ptr(lv[k]) for k in ('p_ptr', 'p_col', 'p_val', 'r_ptr', 'r_col', 'r_val')])
+            levels.append(lv)
+        nc = int(sizes[-1, 0])
+        dense = np.zeros((nc, nc))
+        _lib.call('obte_amg_export_dense', handle, _lib.ptr(dense))
+    finally:
+        _lib.call('obte_amg_destroy', handle)
     if h.pin >= 0:
-        Ac += np.trace(Ac) / (nc * nc)
+        # singular (no fixed temperature): a constant added to every entry
+        # leaves the correction of zero-sum residuals unchanged
+        dense += np.trace(dense) / (nc * nc)
     # a few BLAS threads only: preparation runs in several worker threads at
     # once (the serving loop), where all-core BLAS calls oversubscribe the host
     with _blas_threads(4):
-        return np.ascontiguousarray(np.linalg.inv(Ac))
+        cinv = np.ascontiguousarray(np.linalg.inv(dense))
+    return Hierarchy(levels=levels, coarse_inv=cinv)
 
 
 def _blas_threads(k: int):
@@ -255,12 +301,12 @@ def _blas_threads(k: int):
 
 def prepare_host(host: FourierHost):
     """The host system as the device solves it: renumbered along a Morton
-    curve with its coarse inverse when the two-level preconditioner applies
-    (returns (host, perm or None, coarse inverse or None))."""
-    if COARSE_MIN_CELLS <= host.n <= COARSE_MAX_CELLS and host.xy is not None:
+    curve with its multigrid hierarchy when the multigrid preconditioner
+    applies (returns (host, perm or None, Hierarchy or None))."""
+    if host.n >= MG_MIN_CELLS and host.xy is not None:
         perm = morton_order(host.xy)
         host = permute_host(host, perm)
-        return host, perm, coarse_inverse(host)
+        return host, perm, build_hierarchy(host)
     return host, None, None
 
 
@@ -278,7 +324,7 @@ def prepare(geometry, kappa, bcs, eff=None, heat_source=(), modified_fourier=Fal
                       np.asarray(g.elem_centroids), np.asarray(g.internal), *dict_arrays,
                       np.asarray(kappa, np.float64), repr(dict(bcs._asdict()) if hasattr(bcs, '_asdict') else bcs),
                       repr(eff), np.asarray(heat_source, np.float64), bool(modified_fourier),
-                      float(global_temperature), COARSE_MIN_CELLS, COARSE_MAX_CELLS)
+                      float(global_temperature), MG_MIN_CELLS, MG_AGG, MG_COARSE_MAX, MG_OMEGA, MG_SMOOTH_MAX_ROW)
 
     def build():
         host = build_fourier_host(geometry, kappa, bcs, eff, heat_source, modified_fourier)
@@ -294,7 +340,7 @@ class FourierDevice:
     def __init__(self, host: FourierHost, device=None, prepared=None):
         if not torch.cuda.is_available():
             raise RuntimeError('openbte_amd.Fourier needs a ROCm GPU; there is no CPU fallback')
-        host, self.perm, cinv = prepared if prepared is not None else prepare_host(host)
+        host, self.perm, hier = prepared if prepared is not None else prepare_host(host)
         self.h = host
         self.dev = dev = device or torch.device('cuda', torch.cuda.current_device())
         f64, i32 = torch.float64, torch.int32
@@ -327,10 +373,37 @@ class FourierDevice:
             setattr(P, name, b[name].data_ptr())
         P.n_contact, P.contact_kind = len(h.ce1), h.contact_kind
         P.c_value, P.normalization = h.c_value, h.normalization
-        if cinv is not None:
-            b['coarse_inv'] = T(cinv, f64)
-            P.n_coarse, P.coarse_inv = (n + 255) // 256, b['coarse_inv'].data_ptr()
+        self.levels_c = None
+        if hier is not None:
+            lv_c = (_lib.AmgLevel * len(hier.levels))()
+            for l, lv in enumerate(hier.levels):
+                L = lv_c[l]
+                L.n = lv['n']
+                L.a_entries = int(lv['a_ptr'][-1]) if 'a_ptr' in lv else int(h.row_ptr[-1])
+                L.r_entries = int(lv['r_ptr'][-1])
+                for k in ('a_ptr', 'a_col', 'p_ptr', 'p_col', 'r_ptr', 'r_col'):
+                    if k in lv:
+                        b[f'mg{l}_{k}'] = T(lv[k], i32)
+                        setattr(L, k, b[f'mg{l}_{k}'].data_ptr())
+                for k in ('a_w', 'a_diag', 'p_val', 'r_val'):
+                    if k in lv:
+                        b[f'mg{l}_{k}'] = T(lv[k], f64)
+                        setattr(L, k, b[f'mg{l}_{k}'].data_ptr())
+                for k in (('x', 'res') if l == 0 else ('b', 'x', 'res', 'z')):
+                    b[f'mg{l}_{k}'] = torch.zeros(lv['n'], dtype=f64, device=dev)
+                    setattr(L, k, b[f'mg{l}_{k}'].data_ptr())
+            nc = hier.coarse_inv.shape[0]
+            b['coarse_inv'] = T(hier.coarse_inv, f64)
+            b['coarse_b'] = torch.zeros(nc, dtype=f64, device=dev)
+            b['coarse_x'] = torch.zeros(nc, dtype=f64, device=dev)
+            self.levels_c = lv_c  # the plan points at this host array
+            P.n_levels, P.levels = len(hier.levels), C.addressof(lv_c)
+            P.n_coarse, P.coarse_inv = nc, b['coarse_inv'].data_ptr()
+            P.coarse_b, P.coarse_x = b['coarse_b'].data_ptr(), b['coarse_x'].data_ptr()
         self.stream = torch.cuda.current_stream(dev).cuda_stream
+        # convergence test every CG_CHECK_MG iterations with the multigrid (an
+        # iteration is ~12 launches, a test one host round trip), else every 16
+        self.check_every = CG_CHECK_MG if hier is not None else 16
         self.cg_iters = []
 
     def in_cell_order(self, a: np.ndarray) -> np.ndarray:
@@ -346,7 +419,8 @@ class FourierDevice:
         it = C.c_int32(0)
         rr = C.c_double(0.0)
         _lib.call('obte_fourier_solve', self.plan, b['B'].data_ptr(), b['C'].data_ptr(), b['T'].data_ptr(),
-                  b['ws'].data_ptr(), b['scal'].data_ptr(), rtol, maxit, 16, C.byref(it), C.byref(rr), self.stream)
+                  b['ws'].data_ptr(), b['scal'].data_ptr(), rtol, maxit, self.check_every, C.byref(it), C.byref(rr),
+                  self.stream)
         self.cg_iters.append((it.value, rr.value))
         if rr.value > rtol:
             warnings.warn(f'Fourier CG stopped at {it.value} iterations with relative residual {rr.value:.2e} '
@@ -397,4 +471,4 @@ def Fourier(geometry, thermal_conductivity, boundary_conditions, effective_therm
                  'Flux_Fourier': {'data': J * 1e9, 'units': 'W/m/m'}}
     return SolverResults(kappa_eff=0.0, variables=variables,
                          aux={'n_iter': n_iter, 'kappa_diffusive': kappa_diffusive, 'cg': dev.cg_iters,
-                              'two_level': dev.perm is not None})
+                              'multigrid': dev.perm is not None})

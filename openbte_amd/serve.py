@@ -33,7 +33,9 @@ counters), so its result does not depend on what else is in the batch.
 """
 from __future__ import annotations
 
+import contextlib
 import itertools
+import threading
 import time
 from concurrent.futures import CancelledError, ThreadPoolExecutor
 from dataclasses import dataclass, field
@@ -161,7 +163,7 @@ class BTEServer:
     """
 
     def __init__(self, cfg: Optional[SweepConfig] = None, hbm_budget: Optional[int] = None, max_slots: int = 64,
-                 prep_workers: int = 2, device=None, graphs: bool = True):
+                 prep_workers: int = 2, device=None, graphs: bool = True, fourier_concurrency: int = 0):
         if not torch.cuda.is_available():
             raise RuntimeError('openbte_amd.BTEServer needs a ROCm GPU; there is no CPU fallback')
         self.dev = device or torch.device('cuda', torch.cuda.current_device())
@@ -182,6 +184,8 @@ class BTEServer:
         self.stats = ServerStats()
         self._ids = itertools.count()
         self._pool = ThreadPoolExecutor(max_workers=max(1, prep_workers))
+        # Fourier guesses running on the device at once (0: no limit)
+        self._fourier_gate = threading.Semaphore(fourier_concurrency) if fourier_concurrency > 0 else None
         self._batch: Optional[BTEBatch] = None
         self._dirty = True
         self.graphs = bool(graphs)  # replay each round's launches from hipGraphs (per resident set)
@@ -202,11 +206,13 @@ class BTEServer:
             sched = build_schedule(op)
             f = req.fourier
             if isinstance(f, str) and f == 'auto':
-                stream = torch.cuda.Stream(device=self.dev)
-                with torch.cuda.stream(stream):
-                    f = Fourier(req.geo, req.mat.thermal_conductivity, req.bcs,
-                                req.effective_thermal_conductivity, verbose=False)
-                stream.synchronize()
+                gate = self._fourier_gate if self._fourier_gate is not None else contextlib.nullcontext()
+                with gate:
+                    stream = torch.cuda.Stream(device=self.dev)
+                    with torch.cuda.stream(stream):
+                        f = Fourier(req.geo, req.mat.thermal_conductivity, req.bcs,
+                                    req.effective_thermal_conductivity, verbose=False)
+                    stream.synchronize()
             t0 = (np.zeros(op.n) if f is None else
                   np.asarray(f.variables['Temperature_Fourier']['data'], np.float64))
             req.bytes = device_bytes(op, sched, self.cfg)

@@ -37,7 +37,7 @@ def test_struct_layout_matches_header():
     header = open(os.path.join(ROOT, 'include', 'openbte_amd.h')).read()
     for cname, py in (('obte_bte_plan', _lib.BtePlan), ('obte_gcr', _lib.Gcr), ('obte_fourier_plan', _lib.FourierPlan),
                       ('obte_sweep_io', _lib.SweepIO), ('obte_reduce_io', _lib.ReduceIO),
-                      ('obte_bte_batch', _lib.Batch)):
+                      ('obte_bte_batch', _lib.Batch), ('obte_amg_level', _lib.AmgLevel)):
         body = re.search(r'typedef struct ' + cname + r' \{(.*?)\}', header, re.S).group(1)
         body = re.sub(r'/\*.*?\*/', '', body, flags=re.S)
         fields = re.findall(r'^\s*(?:const\s+)?\w+\s*\**\s*(\w+)\s*;', body, re.M)
@@ -69,7 +69,7 @@ def test_struct_offsets_match_c_compiler(tmp_path):
         pytest.skip('no C compiler')
     structs = (('obte_bte_plan_t', _lib.BtePlan), ('obte_gcr_t', _lib.Gcr), ('obte_fourier_plan_t', _lib.FourierPlan),
                ('obte_sweep_io_t', _lib.SweepIO), ('obte_reduce_io_t', _lib.ReduceIO),
-               ('obte_bte_batch_t', _lib.Batch))
+               ('obte_bte_batch_t', _lib.Batch), ('obte_amg_level_t', _lib.AmgLevel))
     lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "openbte_amd.h"', 'int main(void) {']
     for cname, py in structs:
         lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')

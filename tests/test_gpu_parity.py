@@ -101,7 +101,8 @@ def test_fourier_cg_stop_above_tolerance_warns(small_case):
 
 
 def test_fourier_multi_launch_path_matches_oracle():
-    """More cells than one workgroup's solver takes: the multi-launch CG."""
+    """More cells than one workgroup's solver takes: the multi-launch CG with
+    the multigrid V-cycle (cells renumbered along a Morton curve)."""
     from oracle import fourier_oracle as fo
     from openbte_amd.fourier import Fourier
     geo, mat, bcs, eff = _case(20.0, 0.25, 10000, 4, 8, seed=9)
@@ -113,36 +114,40 @@ def test_fourier_multi_launch_path_matches_oracle():
     print('fourier 10k iters', out.aux['n_iter'], 'dT', dT)
     assert dT < T_ATOL
     assert abs(out.aux['kappa_diffusive'] / ref.aux['kappa_diffusive'] - 1) < 1e-9
-    assert not out.aux['two_level']
+    assert out.aux['multigrid']
 
 
-def test_fourier_two_level_path_matches_oracle():
-    """Meshes with a coarse level in the CG preconditioner (cells renumbered
-    along a Morton curve, 256-cell aggregates): same results in the caller's
-    cell order, and fewer CG iterations than the Jacobi-preconditioned CG."""
+def test_fourier_multigrid_path_matches_oracle():
+    """The multigrid-preconditioned CG at 30k cells: same results as the
+    oracle in the caller's cell order, and far fewer CG iterations than the
+    Jacobi-preconditioned multi-launch CG on the same system (which also
+    matches the oracle)."""
     from oracle import fourier_oracle as fo
     from openbte_amd import fourier as fm
     geo, mat, bcs, eff = _case(40.0, 0.25, 30000, 4, 8, seed=9)
-    assert fm.COARSE_MIN_CELLS <= geo.n_elems <= fm.COARSE_MAX_CELLS
+    assert geo.n_elems >= fm.MG_MIN_CELLS
     ref = fo.fourier(geo, mat.thermal_conductivity, bcs, eff)
     out = fm.Fourier(geo, mat.thermal_conductivity, bcs, eff, verbose=False)
-    assert out.aux['two_level']
+    assert out.aux['multigrid']
     assert out.aux['n_iter'] == ref.aux['n_iter']
     dT = np.abs(out.variables['Temperature_Fourier']['data'] - ref.variables['Temperature_Fourier']['data']).max()
     dJ = np.abs(out.variables['Flux_Fourier']['data'] - ref.variables['Flux_Fourier']['data']).max()
-    print('fourier two-level 30k iters', out.aux['n_iter'], 'dT', dT, 'cg', sum(c[0] for c in out.aux['cg']))
+    print('fourier multigrid 30k iters', out.aux['n_iter'], 'dT', dT, 'cg', sum(c[0] for c in out.aux['cg']))
     assert dT < T_ATOL
     assert dJ < 1e-6 * np.abs(ref.variables['Flux_Fourier']['data']).max()
     assert abs(out.aux['kappa_diffusive'] / ref.aux['kappa_diffusive'] - 1) < 1e-9
-    # the same system with the coarse level switched off
-    old = fm.COARSE_MIN_CELLS
-    fm.COARSE_MIN_CELLS = 1 << 30
+    # the same system with the multigrid switched off: Jacobi CG
+    old = fm.MG_MIN_CELLS
+    fm.MG_MIN_CELLS = 1 << 30
     try:
         jac = fm.Fourier(geo, mat.thermal_conductivity, bcs, eff, verbose=False)
     finally:
-        fm.COARSE_MIN_CELLS = old
-    assert not jac.aux['two_level']
-    assert sum(c[0] for c in out.aux['cg']) < 0.8 * sum(c[0] for c in jac.aux['cg'])
+        fm.MG_MIN_CELLS = old
+    assert not jac.aux['multigrid']
+    dTj = np.abs(jac.variables['Temperature_Fourier']['data'] - ref.variables['Temperature_Fourier']['data']).max()
+    print('fourier jacobi 30k cg', sum(c[0] for c in jac.aux['cg']), 'dT', dTj)
+    assert dTj < T_ATOL
+    assert sum(c[0] for c in out.aux['cg']) < 0.25 * sum(c[0] for c in jac.aux['cg'])
 
 
 @pytest.mark.parametrize("mc,block", [(4, 64), (8, 128), (16, 256), (32, 64), (64, 128)])

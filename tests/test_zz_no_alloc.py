@@ -3,7 +3,7 @@ workspace a file-scope device array) against obte_fourier_solve with a
 caller workspace, through the C ABI: the same temperatures bit for bit and
 the same CG iteration count, on two calls in a row whose second system is
 smaller than the first (the array keeps the first call's contents), over the
-three solve paths (two-level multi-launch CG, Jacobi multi-launch CG, one
+three solve paths (multigrid multi-launch CG, Jacobi multi-launch CG, one
 workgroup)."""
 import ctypes as C
 
@@ -13,7 +13,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def _device(n_cells, seed):
+def _device(n_cells, seed, multigrid=True):
     from openbte_amd.fourier import FourierDevice, build_fourier_host
     from openbte_amd.geometry import pore_cell_2d
     from openbte_amd.objects import BoundaryConditions, EffectiveThermalConductivity
@@ -21,7 +21,9 @@ def _device(n_cells, seed):
     bcs = BoundaryConditions(periodic={'Periodic_x': 1.0, 'Periodic_y': 0.0}, diffuse='Boundary')
     eff = EffectiveThermalConductivity(contact='Periodic_x', normalization=-1.0)
     host = build_fourier_host(geo, np.diag([150.0, 150.0]), bcs, eff)
-    return FourierDevice(host)
+    if multigrid:
+        return FourierDevice(host)
+    return FourierDevice(host, prepared=(host, None, None))
 
 
 def _solve(dev, noalloc):
@@ -40,11 +42,11 @@ def _solve(dev, noalloc):
     return b['T'].cpu().numpy().copy(), it.value, rr.value
 
 
-@pytest.mark.parametrize('sizes', [(20000, 2000), (30000, 6000)])
+@pytest.mark.parametrize('sizes', [((20000, True), (2000, True)), ((30000, True), (6000, False))])
 def test_noalloc_equals_workspace_solve(sizes):
-    for k, n in enumerate(sizes):
-        dev = _device(n, seed=5 + k)
-        print(f'# noalloc solve n={dev.h.n} two_level={dev.perm is not None}', flush=True)
+    for k, (n, mg) in enumerate(sizes):
+        dev = _device(n, seed=5 + k, multigrid=mg)
+        print(f'# noalloc solve n={dev.h.n} multigrid={dev.perm is not None}', flush=True)
         t_ref, it_ref, rr_ref = _solve(dev, noalloc=False)
         t_na, it_na, rr_na = _solve(dev, noalloc=True)
         print(f'  iterations {it_ref} / {it_na}, relres {rr_ref:.2e} / {rr_na:.2e}, '
