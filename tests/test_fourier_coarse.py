@@ -165,3 +165,26 @@ def test_vcycle_is_spd_and_cuts_cg_iterations():
     assert itm < 0.2 * itj, (itm, itj)
     d = (xm - xm.mean()) - (xj - xj.mean())
     assert np.abs(d).max() < 1e-9 * np.abs(xj - xj.mean()).max()
+
+
+def test_hierarchy_3d_keeps_dense_levels_unsmoothed():
+    """On tetrahedra the level-0 operator (~4 off-diagonals per row) gets a
+    smoothed prolongation and the Galerkin level 1 (~40 per row) a piecewise
+    constant one, so the coarse levels stay sparse; the hierarchy is the SciPy
+    construction with the same rule."""
+    from openbte_amd.geometry import pore_slab_3d
+    from openbte_amd.objects import BoundaryConditions, EffectiveThermalConductivity
+    geo = pore_slab_3d(10.0, 4.0, 0.2, 20000, seed=4)
+    bcs = BoundaryConditions(periodic={'Periodic_x': 1.0, 'Periodic_y': 0.0}, diffuse='Boundary')
+    eff = EffectiveThermalConductivity(contact='Periodic_x', normalization=-1.0)
+    h0 = build_fourier_host(geo, np.diag([150.0, 150.0, 150.0]), bcs, eff)
+    h = permute_host(h0, morton_order(h0.xy))
+    hier = F.build_hierarchy(h)
+    ref, _ = _sa_reference(h)
+    assert len(hier.levels) == len(ref) >= 2
+    per_row = [np.diff(lv['p_ptr']).mean() for lv in hier.levels]
+    assert per_row[0] > 2.0 and all(abs(p - 1.0) < 1e-12 for p in per_row[1:])
+    for lv, (A, P) in zip(hier.levels, ref):
+        Pn = _csr(lv['p_ptr'], lv['p_col'], lv['p_val'], P.shape)
+        assert abs(Pn - P).max() <= 1e-12 * abs(P).max()
+        assert abs(_level_A(lv, h) - A).max() <= 1e-11 * abs(A).max()

@@ -150,6 +150,32 @@ def test_fourier_multigrid_path_matches_oracle():
     assert sum(c[0] for c in out.aux['cg']) < 0.25 * sum(c[0] for c in jac.aux['cg'])
 
 
+def test_fourier_multigrid_3d_matches_oracle():
+    """The multigrid path on tetrahedra (a 6,000-tet porous slab, periodic in
+    x and y): same temperatures, flux and diffusive kappa as the oracle."""
+    from oracle import fourier_oracle as fo
+    from openbte_amd import fourier as fm
+    from openbte_amd.geometry import pore_slab_3d
+    from openbte_amd.material import synthetic_material_3d
+    from openbte_amd.objects import BoundaryConditions, EffectiveThermalConductivity
+    geo = pore_slab_3d(10.0, 3.0, 0.2, 6000, seed=24)
+    mat = synthetic_material_3d(8, 4, 6, 1.0, 1e4)
+    bcs = BoundaryConditions(periodic={'Periodic_x': 1.0, 'Periodic_y': 0.0}, diffuse='Boundary')
+    eff = EffectiveThermalConductivity(contact='Periodic_x', normalization=-1.0 / 3.0)
+    assert geo.n_elems >= fm.MG_MIN_CELLS
+    ref = fo.fourier(geo, mat.thermal_conductivity, bcs, eff)
+    out = fm.Fourier(geo, mat.thermal_conductivity, bcs, eff, verbose=False)
+    assert out.aux['multigrid']
+    assert out.aux['n_iter'] == ref.aux['n_iter']
+    dT = np.abs(out.variables['Temperature_Fourier']['data'] - ref.variables['Temperature_Fourier']['data']).max()
+    dJ = np.abs(out.variables['Flux_Fourier']['data'] - ref.variables['Flux_Fourier']['data']).max()
+    print('fourier multigrid 3d', geo.n_elems, 'iters', out.aux['n_iter'], 'dT', dT,
+          'cg', sum(c[0] for c in out.aux['cg']))
+    assert dT < T_ATOL
+    assert dJ < 1e-6 * np.abs(ref.variables['Flux_Fourier']['data']).max()
+    assert abs(out.aux['kappa_diffusive'] / ref.aux['kappa_diffusive'] - 1) < 1e-9
+
+
 @pytest.mark.parametrize("mc,block", [(4, 64), (8, 128), (16, 256), (32, 64), (64, 128)])
 def test_bte_matches_oracle_small(small_case, mc, block):
     from oracle import bte_oracle as bo, fourier_oracle as fo
