@@ -161,15 +161,16 @@ typedef struct obte_bte_plan {
   const double* tb;        /* [n_wall] wall temperatures of the previous iteration */
   double* t;               /* [L][n_chunks][n + ncut_max][mc] solution workspace; rows >= n
                               hold the interface inflows of the running sweep */
-  double* part_t;          /* [L][n] per-direction lattice temperature, sweep order */
-  double* part_j;          /* [L][n][dim] per-direction flux, sweep order */
+  double* part_tj;         /* [L][ps] per-direction sums in sweep order, ps = n (dim + 1)
+                              rounded up to even: lattice temperature and flux of position p
+                              at p (dim + 1) ... p (dim + 1) + dim */
   double* part_tb;         /* [L][n_chunks][n_wall] */
   double* kappa;           /* [M][L] */
   double* supp;            /* [M][L] */
   double* scratch;         /* >= 4096 doubles of reduction scratch */
 
   /* final-sweep mode sums (windowed sweeps) */
-  double* part_chunk;      /* [L][n_chunks][n][dim + 1] per-chunk sums sum_m t_ml T_ml and
+  double* part_chunk;      /* [L][n_chunks][ps] per-chunk sums sum_m t_ml T_ml and
                               sum_m sigma_ml T_ml of each position, written by windowed final
                               sweeps, which then store in t only the values read later (keep
                               flags); NULL: final sweeps store every value and

@@ -32,7 +32,7 @@ class BtePlan(C.Structure):
         ('mfps', _p),
         ('t_coeff', _p), ('sigma', _p), ('heat', _p), ('nano', _f64),
         ('pert', _p), ('rec', _p),
-        ('delta_t', _p), ('tb', _p), ('t', _p), ('part_t', _p), ('part_j', _p), ('part_tb', _p),
+        ('delta_t', _p), ('tb', _p), ('t', _p), ('part_tj', _p), ('part_tb', _p),
         ('kappa', _p), ('supp', _p), ('scratch', _p),
         ('part_chunk', _p), ('keep_t', _i32),
     ]

@@ -85,6 +85,13 @@ def _kmax(cfg: SweepConfig, ny: int) -> int:
     return int(max(8, min(cfg.kmax, cfg.basis_bytes // (16 * ny))))
 
 
+def part_stride(n: int, dim: int) -> int:
+    """Doubles per direction of the per-direction sums (and per chunk of the
+    per-chunk sums): n (dim + 1) rounded up to even, so rows are 16-byte
+    aligned."""
+    return (n * (dim + 1) + 1) // 2 * 2
+
+
 def _window_cap(cfg: SweepConfig, pair_max: int) -> int:
     """Positions of the LDS window of sweeps with this shape (0: none)."""
     return int(_lib.load().obte_sweep_window(int(bool(cfg.window)), int(pair_max), cfg.mc))
@@ -113,10 +120,11 @@ def device_bytes(op: HostOperator, sched: Schedule, cfg: Optional[SweepConfig] =
     f8 += L * n + L * n * 8                                  # pert, rec
     f8 += 2 * n + 2 * one(n_wall) + n * dim                  # dt, tb, j
     f8 += L * nch * (n + ncut_max) * mc                      # t
-    f8 += L * n * (1 + dim) + one(L * nch * n_wall)          # part_t, part_j, part_tb
+    ps = part_stride(n, dim)
+    f8 += L * ps + one(L * nch * n_wall)                     # part_tj, part_tb
     f8 += 2 * M * L + 4096 + 8                               # kappa, supp, scratch
     if _window_cap(cfg, level_pair_max(sched)) > 0:
-        f8 += L * nch * n * (1 + dim)                        # part_chunk
+        f8 += L * nch * ps                                   # part_chunk
     ny = max(1, nch * n_cut * mc)
     f8 += 4 * ny + 2 * _kmax(cfg, ny) * ny + M * L           # y r x w, u c, rnorm
     i4 += 2 * L * nch + L * nch * mc                         # k, active, lane_active
@@ -203,15 +211,15 @@ class BTEDevice:
         b['j'] = torch.zeros(n * dim, dtype=f64, device=dev)
         # zero-initialised: unused inflow slots read a finite value times 0
         b['t'] = torch.zeros(L * nch * (n + self.ncut_max) * mc, dtype=f64, device=dev)
-        b['part_t'] = torch.empty(L * n, dtype=f64, device=dev)
-        b['part_j'] = torch.empty(L * n * dim, dtype=f64, device=dev)
+        ps = part_stride(n, dim)
+        b['part_tj'] = torch.empty(L * ps, dtype=f64, device=dev)
         b['part_tb'] = torch.empty(max(1, L * nch * n_wall), dtype=f64, device=dev)
         b['kappa'] = torch.zeros(M * L, dtype=f64, device=dev)
         b['supp'] = torch.zeros(M * L, dtype=f64, device=dev)
         b['scratch'] = torch.zeros(4096 + 8, dtype=f64, device=dev)
         if _window_cap(cfg, self.lvl_pair_max) > 0:
             # per-chunk mode sums of the windowed final sweep
-            b['part_chunk'] = torch.empty(L * nch * n * (1 + dim), dtype=f64, device=dev)
+            b['part_chunk'] = torch.empty(L * nch * ps, dtype=f64, device=dev)
         # interface Krylov state
         ny = max(1, nch * self.n_cut * mc)
         cfg.kmax = _kmax(cfg, ny)
@@ -265,7 +273,7 @@ class BTEDevice:
         for name in ('order', 'pos', 'lvl_off', 'lvl_ptr', 'nb_src', 'nb_face', 'cut_off',
                      'cut_exit', 'dirs', 'k_face', 'jump_face', 'cf_ptr', 'cf_face', 'wall_ptr', 'wall_side',
                      'wall_db', 'wall_temp', 'wall_cell', 'wall_rinv', 'wall_dbs', 'ce1', 'ce2', 'c_na', 'mfps',
-                     't_coeff', 'sigma', 'heat', 'pert', 'rec', 't', 'part_t', 'part_j',
+                     't_coeff', 'sigma', 'heat', 'pert', 'rec', 't', 'part_tj',
                      'part_tb', 'kappa', 'supp', 'scratch'):
             setattr(P, name, b[name].data_ptr())
         P.nano = op.nano

@@ -28,6 +28,12 @@ constexpr int kMaxBlock = 1024;
 //   [6] (src0, src1)  [7] (src2, cell) as int32 pairs
 constexpr int kRec = 8;
 
+// doubles per direction of the per-direction sums part_tj and per (direction,
+// chunk) of part_chunk: n (dim + 1) rounded up to even (16-byte rows)
+__host__ __device__ __forceinline__ int64_t part_stride(int n, int dim) {
+  return ((int64_t)n * (dim + 1) + 1) & ~(int64_t)1;
+}
+
 __device__ __forceinline__ double pack_ints(int lo, int hi) { return __hiloint2double(hi, lo); }
 
 // s_waitcnt immediate (gfx9 encoding) that waits for vmcnt <= n only
@@ -569,8 +575,8 @@ __device__ __forceinline__ void sweep_task(const S& src, int pid, int lt, SH& sh
     // final sweeps store every value unless they form the mode sums here
     // (then only values read later, unless the caller asked for all of them)
     const bool store_all = MODE == 2 || (kFuse && P.keep_t != 0);
-    const __amdgpu_buffer_rsrc_t ps_rs =
-        make_rsrc(kFuse ? P.part_chunk + (int64_t)lt * n * (dim + 1) : T, kFuse ? (int64_t)n * (dim + 1) * 8 : 0);
+    const __amdgpu_buffer_rsrc_t ps_rs = make_rsrc(kFuse ? P.part_chunk + (int64_t)lt * part_stride(n, dim) : T,
+                                                   kFuse ? (int64_t)n * (dim + 1) * 8 : 0);
     // the upwind values of N (a level ending at h): window slot, or a load
     // from T when outside the window (lanes with nothing to load read past
     // the buffer: no memory access, a zero value)
@@ -1455,9 +1461,9 @@ __global__ void k_gcr_reset(S src, int use_dt) {
 }
 
 // Stage 1 (openbte/bte.py:222-230): mode sums of each direction in sweep
-// order,
-//   part_t[l][p] = sum_m t_ml T_ml,
-//   part_j[l][p] = sum_m sigma_ml T_ml - DeltaT_old sum_m sigma_ml,
+// order, interleaved per position (part_tj[l][p (dim + 1) + k]):
+//   k = 0:   sum_m t_ml T_ml,
+//   k = 1..: sum_m sigma_ml T_ml - DeltaT_old sum_m sigma_ml,
 // summed chunk by chunk in order, each chunk's sum formed by chunk_sums.
 // k_modesum reads T from the solution workspace; k_modesum_part sums the
 // per-chunk sums a windowed final sweep wrote to part_chunk.  Both give the
@@ -1483,10 +1489,10 @@ __device__ __forceinline__ void sigma_total(const PT& P, int l, double* s) {
 template <class PT>
 __device__ __forceinline__ void modesum_store(const PT& P, int l, int p, const double (&acc)[4],
                                               const double* stot) {
-  const int64_t gid = (int64_t)l * P.n + p;
-  const double dt = P.delta_t[P.order[gid]];
-  P.part_t[gid] = acc[0];
-  for (int d = 0; d < P.dim; ++d) P.part_j[gid * P.dim + d] = fma(-dt, stot[d], acc[1 + d]);
+  const double dt = P.delta_t[P.order[(int64_t)l * P.n + p]];
+  double* out = P.part_tj + (int64_t)l * part_stride(P.n, P.dim) + (int64_t)p * (P.dim + 1);
+  out[0] = acc[0];
+  for (int d = 0; d < P.dim; ++d) out[1 + d] = fma(-dt, stot[d], acc[1 + d]);
 }
 
 // From t: one thread per position, the bin pairs' weights staged in LDS.
@@ -1543,10 +1549,11 @@ __global__ __launch_bounds__(256) void k_modesum_part(S src) {
   if (l >= P.n_dirs || (int64_t)blockIdx.x * blockDim.x >= n) return;  // uniform over the block
   if (threadIdx.x == 0) sigma_total(P, l, stot);
   __syncthreads();
+  const int64_t ps = part_stride(n, dim);
   for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     for (int c = 0; c < nch; ++c) {
-      const double* e = P.part_chunk + (((int64_t)l * nch + c) * n + p) * (dim + 1);
+      const double* e = P.part_chunk + ((int64_t)l * nch + c) * ps + (int64_t)p * (dim + 1);
       acc[0] += e[0];
       acc[1] += e[1];
       acc[2] += e[2];
@@ -1571,12 +1578,13 @@ __global__ void k_reduce_cells(S src) {
   double s = 0.0;
   if (c < P.n) {
     double j0 = 0.0, j1 = 0.0, j2 = 0.0;
+    const int64_t ps = part_stride(P.n, dim);
     for (int l = 0; l < P.n_dirs; ++l) {
-      const int64_t o = (int64_t)l * P.n + P.pos[(int64_t)l * P.n + c];
-      s += P.part_t[o];
-      j0 += P.part_j[o * dim + 0];
-      j1 += P.part_j[o * dim + 1];
-      if (dim == 3) j2 += P.part_j[o * dim + 2];
+      const double* e = P.part_tj + l * ps + (int64_t)P.pos[(int64_t)l * P.n + c] * (dim + 1);
+      s += e[0];
+      j0 += e[1];
+      j1 += e[2];
+      if (dim == 3) j2 += e[3];
     }
     R.dt_new[c] = s;
     R.j_new[(int64_t)c * dim + 0] = j0;
