@@ -981,18 +981,20 @@ __device__ double block_lane_max(double v, double* red, double* bc) {
 
 // Block-wide per-lane sums of JB values at once (one reduction for JB dot
 // products: 2 barriers instead of 3 per value); v[j] is replaced by the sum
-// over the slots of the block, in a fixed order.  red: JB * blockDim doubles.
+// over the slots of the block, in a fixed order.  red: JB * kSweepBlock
+// doubles (value j of every thread at j * kSweepBlock: immediate LDS offsets,
+// no per-value address registers; blockDim <= kSweepBlock).
 template <int MC, int JB>
 __device__ __forceinline__ void block_lane_sums(double (&v)[JB], double* red, double* bc) {
   const int lane = threadIdx.x % MC;
   const int cpp = blockDim.x / MC;
 #pragma unroll
-  for (int j = 0; j < JB; ++j) red[j * blockDim.x + threadIdx.x] = v[j];
+  for (int j = 0; j < JB; ++j) red[j * kSweepBlock + threadIdx.x] = v[j];
   __syncthreads();
   for (int idx = threadIdx.x; idx < JB * MC; idx += blockDim.x) {
     const int j = idx / MC, ln = idx - j * MC;
     double acc = 0.0;
-    for (int q = 0; q < cpp; ++q) acc += red[j * blockDim.x + q * MC + ln];
+    for (int q = 0; q < cpp; ++q) acc += red[j * kSweepBlock + q * MC + ln];
     bc[idx] = acc;
   }
   __syncthreads();
@@ -1155,7 +1157,7 @@ __device__ void gcr_flags(const PT& P, const GT& G, int task, int l, int chunk,
 }
 
 // r = x - y; MGS projection of r on the recycled basis; y += U a.  One task
-// by the whole block; red, bc: kGsBlock * blockDim and kGsBlock * 64 doubles.
+// by the whole block; red, bc: kGsBlock * kSweepBlock and kGsBlock * 64 doubles.
 template <int MC, class PT, class GT>
 __device__ __forceinline__ void gcr_project_task(const PT& P, const GT& G, int task, double* red, double* bc) {
   const int l = task / P.n_chunks;
@@ -1338,7 +1340,7 @@ void k_extend(One src, int ntask, int cap, int max_extend, int32_t* ext_max) {
   const obte_gcr_t& G = src.G;
   if (G.active[t] == 0) return;  // uniform over the block
   double* const red = reinterpret_cast<double*>(win_lds);
-  double* const bc = red + kGsBlock * blockDim.x;
+  double* const bc = red + kGsBlock * kSweepBlock;
   int e = 0;
   while (e < max_extend) {
     // the task index passes through an empty asm statement, so nothing the
@@ -1375,7 +1377,7 @@ void k_extend_batch(Many src, const int32_t* __restrict__ qoff, int32_t* __restr
   __shared__ SweepShared<MC> sh;
   __shared__ int s_task;
   double* const red = reinterpret_cast<double*>(win_lds);
-  double* const bc = red + kGsBlock * blockDim.x;
+  double* const bc = red + kGsBlock * kSweepBlock;
   const int xcd = blockIdx.x % 8;
   for (int set = 0; set < 2; ++set) {
     const int32_t* off = set == 0 ? qoff : woff;
