@@ -220,7 +220,7 @@ def measure_batch(args):
     from openbte_amd.batch import BTEBatch
     from openbte_amd.bte import BTEDevice, SweepConfig
     from openbte_amd.fourier import Fourier
-    from openbte_amd.operator import build_host_operator, build_schedule
+    from openbte_amd.operator import BATCH_MAX_WIDTH, DEFAULT_MAX_WIDTH, build_host_operator, build_schedule
     geos, mat, bcs, eff = make_batch(args.config)
     M, L = mat.grid[0], mat.grid[1]
     ncells = np.array([g.n_elems for g in geos])
@@ -231,7 +231,10 @@ def measure_batch(args):
         f = Fourier(g, mat.thermal_conductivity, bcs, eff, verbose=False)
         t0s.append(f.variables['Temperature_Fourier']['data'])
     ops = [build_host_operator(g, mat, bcs, eff) for g in geos]
-    scheds = [build_schedule(o) for o in ops]
+    # the batched form's schedules (levels of at most one batch-kernel pass);
+    # the plain form is timed with these and with the single-solve default's
+    scheds = [build_schedule(o, max_width=BATCH_MAX_WIDTH) for o in ops]
+    scheds_by_width = {BATCH_MAX_WIDTH: scheds, DEFAULT_MAX_WIDTH: [build_schedule(o) for o in ops]}
     tuned = None
     if args.mc or args.block:
         cfg = SweepConfig(mc=args.mc or 8, block=args.block or 256, window=not args.no_window)
@@ -263,9 +266,9 @@ def measure_batch(args):
 
     # plain form: the same K outer iterations, one geometry at a time, at the
     # plain path's default shape and at the batch's shape; the faster counts
-    def plain(pcfg):
+    def plain(pcfg, width=DEFAULT_MAX_WIDTH):
         dt, ks = 0.0, []
-        for op, sc, t0 in zip(ops, scheds, t0s):
+        for op, sc, t0 in zip(ops, scheds_by_width[width], t0s):
             dev = BTEDevice(op, sched=sc, cfg=SweepConfig(**vars(pcfg)))
             dev.reset(t0)
             dev.outer_iteration()
@@ -281,14 +284,19 @@ def measure_batch(args):
         torch.cuda.empty_cache()
         return dt, ks
     # the plain form at the plain path's default shape, at mc 4 (faster for
-    # small meshes alone) and at the batch's shape; the fastest counts
-    plain_runs = {(8, 256): plain(SweepConfig(window=not args.no_window, fused_extend=not args.no_fused)),
-                  (4, 256): plain(SweepConfig(mc=4, window=not args.no_window, fused_extend=not args.no_fused))}
-    if (cfg.mc, cfg.block) not in plain_runs:
-        plain_runs[(cfg.mc, cfg.block)] = plain(cfg)
+    # small meshes alone) and at the batch's shape, each with the single-solve
+    # default schedules and with the batched form's; the fastest counts
+    plain_runs = {}
+    for width in (DEFAULT_MAX_WIDTH, BATCH_MAX_WIDTH):
+        plain_runs[(8, 256, width)] = plain(SweepConfig(window=not args.no_window, fused_extend=not args.no_fused),
+                                            width)
+        plain_runs[(4, 256, width)] = plain(SweepConfig(mc=4, window=not args.no_window,
+                                                        fused_extend=not args.no_fused), width)
+        if (cfg.mc, cfg.block, width) not in plain_runs:
+            plain_runs[(cfg.mc, cfg.block, width)] = plain(cfg, width)
     best_plain = min(plain_runs, key=lambda k: plain_runs[k][0])
     dt_plain = plain_runs[best_plain][0]
-    plain_kappa = plain_runs[(cfg.mc, cfg.block)][1]
+    plain_kappa = plain_runs[(cfg.mc, cfg.block, DEFAULT_MAX_WIDTH)][1]
     same = max(abs(a - b) / abs(b) for a, b in zip(kappas, plain_kappa))
     value = updates / dt_batch
     out = {
@@ -303,7 +311,9 @@ def measure_batch(args):
                    'autotune': tuned,
                    'plain_value': updates / dt_plain, 'plain_ms_per_step': dt_plain / K * 1e3,
                    'plain_shape': list(best_plain),
-                   'plain_values_by_shape': {f'mc{k[0]}_block{k[1]}': updates / v[0] for k, v in plain_runs.items()},
+                   'plain_values_by_shape': {f'mc{k[0]}_block{k[1]}_width{k[2]}': updates / v[0]
+                                             for k, v in plain_runs.items()},
+                   'schedule_max_width': BATCH_MAX_WIDTH,
                    'batched_over_plain': dt_plain / dt_batch,
                    'max_rel_kappa_diff_batched_vs_plain': same},
     }

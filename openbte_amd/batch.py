@@ -23,7 +23,7 @@ import torch
 from . import _lib
 from .bte import BTEDevice, SweepConfig, default_config, warn_unconverged
 from .objects import SolverResults
-from .operator import build_host_operator
+from .operator import BATCH_MAX_WIDTH, build_host_operator, build_schedule
 
 N_QUEUES = 8
 _CAPTURE_LOCK = threading.Lock()
@@ -42,7 +42,9 @@ class BTEBatch:
                  groups: Optional[int] = None, device=None, devices: Optional[Sequence[BTEDevice]] = None,
                  n_active: Optional[torch.Tensor] = None, norm2: Optional[torch.Tensor] = None,
                  slots: Optional[Sequence[int]] = None):
-        """Build the batch from host operators, or (``devices``) over existing
+        """Build the batch from host operators (schedules of at most
+        ``BATCH_MAX_WIDTH`` cells per level unless ``scheds`` are given), or
+        (``devices``) over existing
         ``BTEDevice`` objects whose counters are views ``n_active[slots[p]]``
         and ``norm2[slots[p]]`` of the given arrays (the serving loop keeps
         problems resident and rebuilds only this view when its set changes)."""
@@ -57,7 +59,8 @@ class BTEBatch:
             self.slots = np.arange(P)
             self.devs: List[BTEDevice] = []
             for i, op in enumerate(ops):
-                d = BTEDevice(op, sched=None if scheds is None else scheds[i], cfg=SweepConfig(**vars(self.cfg)),
+                sc = scheds[i] if scheds is not None else build_schedule(op, max_width=BATCH_MAX_WIDTH)
+                d = BTEDevice(op, sched=sc, cfg=SweepConfig(**vars(self.cfg)),
                               device=self.dev, n_active=self.n_active[i:i + 1], norm2=self.norm2[i:i + 1])
                 self.devs.append(d)
             assemble = True
@@ -388,7 +391,6 @@ def BTE_RTA_batch(problems, bte_max_iter: int = 100, bte_rtol: float = 1e-4,
     problem signature; ``autotune=False``: the single-solve default, which
     makes every result bitwise that of ``BTE_RTA`` alone); ``graphs`` replays
     the outer iterations from hipGraphs."""
-    from .operator import build_schedule
     ops, t0 = [], []
     for pr in problems:
         geo = pr['geo']
@@ -397,7 +399,7 @@ def BTE_RTA_batch(problems, bte_max_iter: int = 100, bte_rtol: float = 1e-4,
         f = pr.get('fourier')
         t0.append(np.zeros(geo.n_elems) if f is None else
                   np.asarray(f.variables['Temperature_Fourier']['data'], np.float64))
-    scheds = [build_schedule(o) for o in ops]
+    scheds = [build_schedule(o, max_width=BATCH_MAX_WIDTH) for o in ops]
     if config is None:
         if autotune:
             from . import tune

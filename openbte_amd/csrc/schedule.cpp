@@ -19,6 +19,9 @@
 //   in_face[e]      directed face id of the entry (for coefficient assembly)
 //   cut_exit[f]     position of the upstream cell of cut face f
 //   cut_face[f]     directed face id of cut face f
+// Cut faces are numbered in ascending directed-face id, which does not depend
+// on the layering: the interface vectors (and the order of every sum over
+// them) are the same for any level width.
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
@@ -156,25 +159,29 @@ void build_direction(int32_t n, int64_t n_faces, int dim, const int32_t* i_mat, 
   out.pos.assign(n, 0);
   for (int32_t p = 0; p < n; ++p) out.pos[out.order[p]] = p;
 
-  // inflow CSR in position order; cut faces numbered in position order
+  // cut faces numbered in ascending face id (is_cut is only set on inflow
+  // faces of this direction)
+  std::vector<int32_t> cut_index(n_faces, -1);
+  int32_t ncut = 0;
+  for (int64_t f = 0; f < n_faces; ++f)
+    if (is_cut[f]) cut_index[f] = ncut++;
+  out.cut_exit.resize(ncut);
+  out.cut_face.resize(ncut);
+  // inflow CSR in position order
   out.in_ptr.assign(n + 1, 0);
   out.in_src.resize(cnt[n]);
   out.in_face.resize(cnt[n]);
-  int32_t ncut = 0;
-  for (int32_t e = 0; e < cnt[n]; ++e) ncut += is_cut[row_faces[e]];
-  out.cut_exit.resize(ncut);
-  out.cut_face.resize(ncut);
-  int32_t w = 0, q = 0;
+  int32_t w = 0;
   for (int32_t p = 0; p < n; ++p) {
     const int32_t c = out.order[p];
     for (int32_t e = cnt[c]; e < cnt[c + 1]; ++e, ++w) {
       const int32_t f = row_faces[e];
       const int32_t up = out.pos[j_mat[f]];
       if (is_cut[f]) {
+        const int32_t q = cut_index[f];
         out.cut_face[q] = f;
         out.cut_exit[q] = up;
         out.in_src[w] = ~q;
-        ++q;
       } else {
         out.in_src[w] = up;
       }
