@@ -220,7 +220,7 @@ def measure_batch(args):
     from openbte_amd.batch import BTEBatch
     from openbte_amd.bte import BTEDevice, SweepConfig
     from openbte_amd.fourier import Fourier
-    from openbte_amd.operator import BATCH_MAX_WIDTH, DEFAULT_MAX_WIDTH, build_host_operator, build_schedule
+    from openbte_amd.operator import DEFAULT_MAX_WIDTH, batch_max_width, build_host_operator, build_schedule
     geos, mat, bcs, eff = make_batch(args.config)
     M, L = mat.grid[0], mat.grid[1]
     ncells = np.array([g.n_elems for g in geos])
@@ -231,19 +231,20 @@ def measure_batch(args):
         f = Fourier(g, mat.thermal_conductivity, bcs, eff, verbose=False)
         t0s.append(f.variables['Temperature_Fourier']['data'])
     ops = [build_host_operator(g, mat, bcs, eff) for g in geos]
-    # the batched form's schedules (levels of at most one batch-kernel pass);
-    # the plain form is timed with these and with the single-solve default's
-    scheds = [build_schedule(o, max_width=BATCH_MAX_WIDTH) for o in ops]
-    scheds_by_width = {BATCH_MAX_WIDTH: scheds, DEFAULT_MAX_WIDTH: [build_schedule(o) for o in ops]}
     tuned = None
     if args.mc or args.block:
         cfg = SweepConfig(mc=args.mc or 8, block=args.block or 256, window=not args.no_window)
     else:
         from openbte_amd import tune
-        cfg = tune.tune(ops, scheds)
+        cfg = tune.tune(ops)
         cfg.window = not args.no_window
         tuned = tune.last_results(ops)
         print(f'# autotuned mc={cfg.mc} block={cfg.block}', file=sys.stderr, flush=True)
+    # the batched form's schedules (levels of at most one batch-kernel pass);
+    # the plain form is timed with these and with the single-solve default's
+    width = batch_max_width(cfg.mc, cfg.block)
+    scheds = [build_schedule(o, max_width=width) for o in ops]
+    scheds_by_width = {width: scheds, DEFAULT_MAX_WIDTH: [build_schedule(o) for o in ops]}
     cfg.fused_extend = not args.no_fused
     K = args.steps
     updates = float(ncells.sum()) * M * L * K
@@ -287,13 +288,12 @@ def measure_batch(args):
     # small meshes alone) and at the batch's shape, each with the single-solve
     # default schedules and with the batched form's; the fastest counts
     plain_runs = {}
-    for width in (DEFAULT_MAX_WIDTH, BATCH_MAX_WIDTH):
-        plain_runs[(8, 256, width)] = plain(SweepConfig(window=not args.no_window, fused_extend=not args.no_fused),
-                                            width)
-        plain_runs[(4, 256, width)] = plain(SweepConfig(mc=4, window=not args.no_window,
-                                                        fused_extend=not args.no_fused), width)
-        if (cfg.mc, cfg.block, width) not in plain_runs:
-            plain_runs[(cfg.mc, cfg.block, width)] = plain(cfg, width)
+    for w in scheds_by_width:
+        plain_runs[(8, 256, w)] = plain(SweepConfig(window=not args.no_window, fused_extend=not args.no_fused), w)
+        plain_runs[(4, 256, w)] = plain(SweepConfig(mc=4, window=not args.no_window,
+                                                    fused_extend=not args.no_fused), w)
+        if (cfg.mc, cfg.block, w) not in plain_runs:
+            plain_runs[(cfg.mc, cfg.block, w)] = plain(cfg, w)
     best_plain = min(plain_runs, key=lambda k: plain_runs[k][0])
     dt_plain = plain_runs[best_plain][0]
     plain_kappa = plain_runs[(cfg.mc, cfg.block, DEFAULT_MAX_WIDTH)][1]
@@ -313,7 +313,7 @@ def measure_batch(args):
                    'plain_shape': list(best_plain),
                    'plain_values_by_shape': {f'mc{k[0]}_block{k[1]}_width{k[2]}': updates / v[0]
                                              for k, v in plain_runs.items()},
-                   'schedule_max_width': BATCH_MAX_WIDTH,
+                   'schedule_max_width': width,
                    'batched_over_plain': dt_plain / dt_batch,
                    'max_rel_kappa_diff_batched_vs_plain': same},
     }

@@ -23,7 +23,7 @@ import torch
 from . import _lib
 from .bte import BTEDevice, SweepConfig, default_config, warn_unconverged
 from .objects import SolverResults
-from .operator import BATCH_MAX_WIDTH, build_host_operator, build_schedule
+from .operator import batch_max_width, build_host_operator, build_schedule
 
 N_QUEUES = 8
 _CAPTURE_LOCK = threading.Lock()
@@ -43,7 +43,8 @@ class BTEBatch:
                  n_active: Optional[torch.Tensor] = None, norm2: Optional[torch.Tensor] = None,
                  slots: Optional[Sequence[int]] = None):
         """Build the batch from host operators (schedules of at most
-        ``BATCH_MAX_WIDTH`` cells per level unless ``scheds`` are given), or
+        ``batch_max_width(mc, block)`` cells per level unless ``scheds`` are
+        given), or
         (``devices``) over existing
         ``BTEDevice`` objects whose counters are views ``n_active[slots[p]]``
         and ``norm2[slots[p]]`` of the given arrays (the serving loop keeps
@@ -59,7 +60,8 @@ class BTEBatch:
             self.slots = np.arange(P)
             self.devs: List[BTEDevice] = []
             for i, op in enumerate(ops):
-                sc = scheds[i] if scheds is not None else build_schedule(op, max_width=BATCH_MAX_WIDTH)
+                sc = scheds[i] if scheds is not None else build_schedule(
+                    op, max_width=batch_max_width(self.cfg.mc, self.cfg.block))
                 d = BTEDevice(op, sched=sc, cfg=SweepConfig(**vars(self.cfg)),
                               device=self.dev, n_active=self.n_active[i:i + 1], norm2=self.norm2[i:i + 1])
                 self.devs.append(d)
@@ -399,14 +401,13 @@ def BTE_RTA_batch(problems, bte_max_iter: int = 100, bte_rtol: float = 1e-4,
         f = pr.get('fourier')
         t0.append(np.zeros(geo.n_elems) if f is None else
                   np.asarray(f.variables['Temperature_Fourier']['data'], np.float64))
-    scheds = [build_schedule(o, max_width=BATCH_MAX_WIDTH) for o in ops]
     if config is None:
         if autotune:
             from . import tune
-            config = tune.tune(ops, scheds)
+            config = tune.tune(ops)
         else:
             config = default_config(ops)
-    batch = BTEBatch(ops, cfg=config, scheds=scheds)
+    batch = BTEBatch(ops, cfg=config)
     if graphs:
         batch.enable_graphs(True)
     batch.solve(t0, bte_max_iter=bte_max_iter, bte_rtol=bte_rtol)

@@ -203,13 +203,16 @@ def build_host_operator(geo, mat, bcs, effective_thermal_conductivity=None, heat
 # fits one pass of a 256-thread windowed sweep workgroup at mc = 8 (3 cells
 # per thread) and two adjacent levels fit a 512-position LDS window.
 DEFAULT_MAX_WIDTH = 192
-# Cells per level of the schedules batched and served problems use: the
-# persistent batch kernel holds 2 cells per thread (128 cells per pass at mc =
-# 8), so wider levels would take a second pass that reloads its records
-# without prefetch.  The solution does not depend on the layering (cut faces
-# are numbered by face id), so these problems give the same bits as with the
-# default schedule.
-BATCH_MAX_WIDTH = 128
+
+
+def batch_max_width(mc: int, block: int) -> int:
+    """Cells per level of the schedules batched and served problems use: one
+    pass of the persistent batch kernel, 2 cells per thread and mc / 2 lanes
+    per cell (128 cells at mc = 8, block = 256).  Wider levels would take a
+    second pass that reloads its records without prefetch.  The solution does
+    not depend on the layering (cut faces are numbered by face id), so these
+    problems give the same bits as with the default schedule."""
+    return max(16, 4 * int(block) // int(mc))
 
 
 def build_schedule(op: HostOperator, dirs: Optional[np.ndarray] = None, cut_candidate: Optional[np.ndarray] = None,

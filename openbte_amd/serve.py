@@ -48,7 +48,7 @@ from .batch import BTEBatch
 from .bte import BTEDevice, SweepConfig, device_bytes
 from .fourier import Fourier
 from .objects import SolverResults
-from .operator import BATCH_MAX_WIDTH, build_host_operator, build_schedule
+from .operator import batch_max_width, build_host_operator, build_schedule
 
 
 @dataclass
@@ -203,7 +203,7 @@ class BTEServer:
         try:
             op = build_host_operator(req.geo, req.mat, req.bcs, req.effective_thermal_conductivity,
                                      req.heat_source)
-            sched = build_schedule(op, max_width=BATCH_MAX_WIDTH)
+            sched = build_schedule(op, max_width=batch_max_width(self.cfg.mc, self.cfg.block))
             f = req.fourier
             if isinstance(f, str) and f == 'auto':
                 gate = self._fourier_gate if self._fourier_gate is not None else contextlib.nullcontext()

@@ -23,6 +23,7 @@ import numpy as np
 import torch
 
 from .bte import BTEDevice, SweepConfig
+from .operator import batch_max_width, build_schedule
 
 CANDIDATES: Tuple[Tuple[int, int], ...] = ((4, 64), (4, 128), (8, 64), (8, 128), (8, 256), (16, 128), (16, 256))
 _memo: Dict[str, dict] = {}
@@ -57,7 +58,9 @@ def tune(ops: Sequence, scheds: Optional[Sequence] = None, candidates: Iterable[
          reps: int = 2, cache: Optional[str] = None, verbose: bool = False) -> SweepConfig:
     """Pick (mc, block) for solving ``ops`` together (one problem: the plain
     path; several: the batched path): the candidate with the shortest outer
-    iterations 1..reps from DeltaT = 0.  Returns a ``SweepConfig``."""
+    iterations 1..reps from DeltaT = 0.  Without ``scheds`` a batch candidate
+    runs on schedules whose levels fit one pass of its batch kernel
+    (``batch_max_width``).  Returns a ``SweepConfig``."""
     from .batch import BTEBatch
     key = signature(ops)
     rec = _memo.get(key)
@@ -73,7 +76,9 @@ def tune(ops: Sequence, scheds: Optional[Sequence] = None, candidates: Iterable[
                 obj = BTEDevice(ops[0], sched=None if scheds is None else scheds[0], cfg=cfg)
                 t0 = np.zeros(ops[0].n)
             else:
-                obj = BTEBatch(ops, cfg=cfg, scheds=scheds)
+                sc = scheds if scheds is not None else [
+                    build_schedule(o, max_width=batch_max_width(mc, block)) for o in ops]
+                obj = BTEBatch(ops, cfg=cfg, scheds=sc)
                 t0 = [np.zeros(o.n) for o in ops]
             t = _time_iterations(obj, t0, reps)
             results.append({'mc': mc, 'block': block, 'iter_s': t, 'updates_per_s': updates / t})
