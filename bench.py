@@ -414,7 +414,7 @@ def measure_serve(args):
                 srv.submit(reqs[order[k]])
                 k += 1
             got = srv.step()
-            if not srv.live and not got and k < N:
+            if not srv.live and not got and k < N and not srv.preparing:
                 time.sleep(min(0.002, max(0.0, arrivals[order[k]] - (time.perf_counter() - start))))
         makespan = time.perf_counter() - start
         srv.close()

@@ -54,6 +54,12 @@ int obte_sched_export(void* handle, int32_t* order, int32_t* pos, int32_t* lvl_p
  * one cell, else OBTE_EINVAL. */
 int obte_sched_export_slots(void* handle, int32_t kin, int32_t* nb_src, int32_t* nb_face);
 int obte_sched_destroy(void* handle);
+/* Threads of the host-side parallel loops (scheduling) that the calling
+ * thread starts from now on (n >= 1).  No counterpart in the reference (its
+ * preparation is serial): for callers that prepare several problems in
+ * concurrent threads, which would otherwise each start one host thread per
+ * core. */
+int obte_set_host_threads(int32_t n);
 
 /* ---- Fourier multigrid setup (host) ------------------------------------ */
 /* Smoothed-aggregation hierarchy of the symmetric Fourier operator (the rows

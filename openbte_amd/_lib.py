@@ -99,6 +99,7 @@ _SIGNATURES = {
     'obte_sched_export': [_p, _p, _p, _p, _p, _p, _p, _p, _p],
     'obte_sched_export_slots': [_p, _i32, _p, _p],
     'obte_sched_destroy': [_p],
+    'obte_set_host_threads': [_i32],
     'obte_amg_create': [_i32, _p, _p, _p, _p, _i32, _i32, _f64, _i32, C.POINTER(_p)],
     'obte_amg_sizes': [_p, C.POINTER(_i32), _p],
     'obte_amg_export': [_p, _i32, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p],

@@ -30,6 +30,8 @@
 #include <string>
 #include <vector>
 
+#include <omp.h>
+
 #include "capi_common.h"
 
 namespace {
@@ -322,6 +324,12 @@ int obte_sched_export_slots(void* handle, int32_t kin, int32_t* nb_src, int32_t*
 
 int obte_sched_destroy(void* handle) {
   delete static_cast<Schedule*>(handle);
+  return OBTE_OK;
+}
+
+int obte_set_host_threads(int32_t n) {
+  if (n < 1) return obte_fail(OBTE_EINVAL, "obte_set_host_threads: n must be at least 1");
+  omp_set_num_threads(n);  // the calling thread's parallel regions from now on
   return OBTE_OK;
 }
 

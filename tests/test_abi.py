@@ -102,3 +102,36 @@ def test_noalloc_fourier_rejects_oversized_plan():
                                         ctypes.byref(rr), None)
     assert rc == -5  # OBTE_ELIMIT
     assert b'OBTE_FOURIER_NOALLOC_MAX_CELLS' in lib.obte_last_error()
+
+
+def test_host_thread_budget():
+    """obte_set_host_threads sets the calling thread's budget for the native
+    scheduler's parallel loops: the schedule is the same with 1 and with 4
+    threads, and a budget below 1 is rejected."""
+    import threading
+    import numpy as np
+    from openbte_amd import _lib
+    from openbte_amd.operator import _build_schedule
+    from openbte_amd.geometry import pore_cell_2d
+    from openbte_amd.material import synthetic_material_2d
+    from openbte_amd.objects import BoundaryConditions, EffectiveThermalConductivity
+    from openbte_amd.operator import build_host_operator
+    geo = pore_cell_2d(10.0, 0.25, 2000, seed=5)
+    mat = synthetic_material_2d(4, 12, 1.0, 1e4)
+    bcs = BoundaryConditions(periodic={'Periodic_x': 1.0, 'Periodic_y': 0.0}, diffuse='Boundary')
+    eff = EffectiveThermalConductivity(contact='Periodic_x', normalization=-1.0)
+    op = build_host_operator(geo, mat, bcs, eff)
+    out = {}
+
+    def run(k):
+        _lib.call('obte_set_host_threads', k)
+        out[k] = _build_schedule(op, op.dirs, op.cut_candidate, 192)
+
+    for k in (1, 4):
+        t = threading.Thread(target=run, args=(k,))
+        t.start()
+        t.join()
+    for name in ('order', 'lvl_ptr', 'in_src', 'cut_exit', 'nb_src'):
+        assert np.array_equal(getattr(out[1], name), getattr(out[4], name))
+    with pytest.raises(_lib.ObteError):
+        _lib.call('obte_set_host_threads', 0)
