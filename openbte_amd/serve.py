@@ -34,6 +34,7 @@ counters), so its result does not depend on what else is in the batch.
 from __future__ import annotations
 
 import contextlib
+import heapq
 import itertools
 import os
 import threading
@@ -188,7 +189,10 @@ class BTEServer:
         self.free_slots = list(range(self.max_slots))
         self.live: List[_Live] = []
         self.ready: List[_Prepared] = []
-        self.preparing: List[tuple] = []  # (future, request)
+        self.preparing: List[Any] = []  # futures of (request, prepared or None)
+        # requests waiting for a preparation thread, largest first
+        self._prep_queue: List[tuple] = []
+        self._prep_lock = threading.Lock()
         self.done: List[Request] = []
         self.used_bytes = 0
         self.stats = ServerStats()
@@ -209,12 +213,23 @@ class BTEServer:
 
     # -- intake --------------------------------------------------------------
     def submit(self, req: Request) -> Request:
-        """Queue a request (non-blocking); its host preparation starts now."""
+        """Queue a request (non-blocking) for host preparation.  A free
+        preparation thread takes the largest waiting request first (cells x
+        modes): the requests that need the most rounds are admitted first,
+        which shortens the makespan of a burst of requests."""
         req.id = next(self._ids)
         req.t_submit = time.perf_counter()
         self.stats.submitted += 1
-        self.preparing.append((self._pool.submit(self._prepare, req), req))
+        cost = float(req.geo.n_elems) * float(np.asarray(req.mat.t_coeff).size)
+        with self._prep_lock:
+            heapq.heappush(self._prep_queue, (-cost, req.id, req))
+        self.preparing.append(self._pool.submit(self._prepare_next))
         return req
+
+    def _prepare_next(self):
+        with self._prep_lock:
+            _, _, req = heapq.heappop(self._prep_queue)
+        return req, self._prepare(req)
 
     def _prepare(self, req: Request) -> Optional[_Prepared]:
         try:
@@ -246,11 +261,11 @@ class BTEServer:
 
     def _collect_prepared(self) -> None:
         still = []
-        for fut, req in self.preparing:
+        for fut in self.preparing:
             if not fut.done():
-                still.append((fut, req))
+                still.append(fut)
                 continue
-            p = fut.result()
+            req, p = fut.result()
             if p is None:
                 self._finish_failed(req)
             else:
@@ -338,7 +353,7 @@ class BTEServer:
                 # nothing to run yet: wait for a preparation to finish rather
                 # than return at once (a caller looping on step would hold the
                 # interpreter lock the preparation threads need)
-                wait([f for f, _ in self.preparing], timeout=0.002, return_when=FIRST_COMPLETED)
+                wait(self.preparing, timeout=0.002, return_when=FIRST_COMPLETED)
             return []
         t = time.perf_counter()
         b = self._view()
