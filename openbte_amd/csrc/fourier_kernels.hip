@@ -25,7 +25,7 @@ namespace {
 
 constexpr int kB = 256;
 
-enum { S_RZ0 = 0, S_RZ1, S_ALPHA, S_BETA, S_RR, S_BB, S_TOT, S_MAX, S_MIN, S_ITER };
+enum { S_RZ0 = 0, S_RZ1, S_ALPHA, S_BETA, S_RR, S_BB, S_TOT, S_MAX, S_MIN, S_ITER, S_TGT, S_DONE };
 
 __device__ __forceinline__ double block_sum(double v, double* red) {
   red[threadIdx.x] = v;
@@ -87,6 +87,33 @@ __device__ __forceinline__ double sum_partials(const double* part, int nb, doubl
   return wg_sum(s, red);
 }
 
+// the sums of two partial arrays with one pair of barriers (each with the
+// bits sum_partials gives it); red: 32 doubles
+__device__ __forceinline__ void sum_partials2(const double* a, const double* b, int nb, double* red, double& sa,
+                                              double& sb) {
+  double u = 0.0, v = 0.0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+    u += a[i];
+    v += b[i];
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    u += __shfl_xor(u, o, 64);
+    v += __shfl_xor(v, o, 64);
+  }
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) {
+    red[threadIdx.x >> 6] = u;
+    red[16 + (threadIdx.x >> 6)] = v;
+  }
+  __syncthreads();
+  sa = 0.0;
+  sb = 0.0;
+  for (int j = 0; j < (int)(blockDim.x >> 6); ++j) {
+    sa += red[j];
+    sb += red[16 + j];
+  }
+}
+
 // Partial sums of the multi-launch CG live after the five work vectors:
 // [pq | rz | rr | bb] x nb doubles (4 nb <= n + 64 for every n).
 // With the multigrid preconditioner the V-cycle's last kernel forms the
@@ -122,14 +149,21 @@ __global__ void k_cg_init(obte_fourier_plan_t P, const double* bh, const double*
   }
 }
 
-// scal[S_RR], scal[S_BB] from the partials (host convergence test)
-__global__ void k_cg_norms(const double* part, int nb, double* scal, int with_bb) {
+// scal[S_RR], scal[S_BB] from the partials (host convergence test); with
+// with_bb (the initial residual) also the converged threshold
+// scal[S_TGT] = (rtol |b|)^2 and scal[S_DONE] = 0
+__global__ void k_cg_norms(const double* part, int nb, double* scal, int with_bb, double rtol) {
   __shared__ double red[16];
   const double rr = sum_partials(part + 2 * nb, nb, red);
   const double bb = with_bb ? sum_partials(part + 3 * nb, nb, red) : 0.0;
   if (threadIdx.x == 0) {
     scal[S_RR] = rr;
-    if (with_bb) scal[S_BB] = bb;
+    if (with_bb) {
+      scal[S_BB] = bb;
+      const double t = rtol * (bb > 0.0 ? sqrt(bb) : 1.0);
+      scal[S_TGT] = t * t;
+      scal[S_DONE] = 0.0;
+    }
   }
 }
 
@@ -144,7 +178,7 @@ constexpr int kRowHoist = 4;
 
 __global__ void k_cg_pq(obte_fourier_plan_t P, const double* z, const double* pold, double* pnew, double* q,
                         double* part, double* scal, int nb, int k) {
-  __shared__ double red[16];
+  __shared__ double red[32];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool row = i < P.n;
   const bool lag = k > 0;
@@ -170,13 +204,22 @@ __global__ void k_cg_pq(obte_fourier_plan_t P, const double* z, const double* po
       pe[e] = lag ? pold[j] : 0.0;
     }
   }
-  const double rz = sum_partials(part + nb, nb, red);
+  // converged (the residual of the previous update below the host's
+  // threshold): this and every later iteration leave x and r unchanged, so
+  // the iterations run between two host tests cannot disturb a converged
+  // solution.  Every block forms the same sums, so all decide alike.
+  double rz, rr;
+  sum_partials2(part + nb, part + 2 * nb, nb, red, rz, rr);
+  const bool done = scal[S_DONE] != 0.0 || rr <= scal[S_TGT];
   double beta = 0.0;
-  if (lag) {
+  if (lag && !done) {
     const double rzo = scal[S_RZ0 + ((k - 1) & 1)];
     beta = rzo != 0.0 ? rz / rzo : 0.0;
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) scal[S_RZ0 + (k & 1)] = rz;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    scal[S_RZ0 + (k & 1)] = rz;
+    if (done) scal[S_DONE] = 1.0;
+  }
   double pq = 0.0;
   if (row) {
     const double pi = lag ? zi + beta * pi0 : zi;
@@ -216,7 +259,7 @@ __global__ void k_cg_xr(obte_fourier_plan_t P, double* x, double* r, double* z, 
   }
   const double pq = sum_partials(part, nb, red);
   const double rz = scal[S_RZ0 + (k & 1)];
-  const double alpha = pq != 0.0 ? rz / pq : 0.0;
+  const double alpha = pq != 0.0 && scal[S_DONE] == 0.0 ? rz / pq : 0.0;
   double rzn = 0.0, rr = 0.0;
   if (row) {
     x[i] = xi + alpha * pi;
@@ -791,7 +834,7 @@ int obte_fourier_solve(const obte_fourier_plan_t* plan, const double* b, const d
   double* pbuf[2] = {p, bh};
   hipLaunchKernelGGL(k_cg_init, dim3(nb), dim3(kB), 0, st, P, bh, t, r, z, part, (int)nb);
   if (P.n_levels > 0) vcycle(P, r, z, part, (int)nb, st);
-  hipLaunchKernelGGL(k_cg_norms, dim3(1), dim3(1024), 0, st, part, (int)nb, scal, 1);
+  hipLaunchKernelGGL(k_cg_norms, dim3(1), dim3(1024), 0, st, part, (int)nb, scal, 1, rtol);
   rc = status("obte_fourier_solve: init launch failed");
   if (rc) return rc;
   if (hipMemcpyAsync(host, scal, 8 * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
@@ -809,7 +852,7 @@ int obte_fourier_solve(const obte_fourier_plan_t* plan, const double* b, const d
       hipLaunchKernelGGL(k_cg_xr, dim3(nb), dim3(kB), 0, st, P, t, r, z, pk, q, part, scal, (int)nb, it);
       if (P.n_levels > 0) vcycle(P, r, z, part, (int)nb, st);
     }
-    hipLaunchKernelGGL(k_cg_norms, dim3(1), dim3(1024), 0, st, part, (int)nb, scal, 0);
+    hipLaunchKernelGGL(k_cg_norms, dim3(1), dim3(1024), 0, st, part, (int)nb, scal, 0, rtol);
     rc = status("obte_fourier_solve: iteration launch failed");
     if (rc) return rc;
     if (hipMemcpyAsync(host, scal, 8 * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||

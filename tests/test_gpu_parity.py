@@ -117,6 +117,31 @@ def test_fourier_multi_launch_path_matches_oracle():
     assert out.aux['multigrid']
 
 
+def test_fourier_cg_iterations_past_convergence_change_nothing():
+    """The host tests the CG residual every CG_CHECK_MG iterations; the
+    iterations run after the residual fell below the threshold leave the
+    solution unchanged, so the result is the same bits with tests every 8 or
+    every 64 iterations (before, up to 31 iterations past convergence could
+    carry the residual of a served problem back up to 2e-5)."""
+    import warnings
+    from openbte_amd import fourier as fm
+    geo, mat, bcs, eff = _case(40.0, 0.25, 30000, 4, 8, seed=9)
+    old = fm.CG_CHECK_MG
+    out = {}
+    try:
+        for ce in (8, 64):
+            fm.CG_CHECK_MG = ce
+            with warnings.catch_warnings():
+                warnings.simplefilter('error')
+                out[ce] = fm.Fourier(geo, mat.thermal_conductivity, bcs, eff, verbose=False)
+    finally:
+        fm.CG_CHECK_MG = old
+    assert out[8].aux['multigrid']
+    for name in ('Temperature_Fourier', 'Flux_Fourier'):
+        assert np.array_equal(out[8].variables[name]['data'], out[64].variables[name]['data'])
+    assert out[8].aux['kappa_diffusive'] == out[64].aux['kappa_diffusive']
+
+
 def test_fourier_multigrid_path_matches_oracle():
     """The multigrid-preconditioned CG at 30k cells: same results as the
     oracle in the caller's cell order, and far fewer CG iterations than the
