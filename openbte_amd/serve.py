@@ -220,7 +220,10 @@ class BTEServer:
         req.id = next(self._ids)
         req.t_submit = time.perf_counter()
         self.stats.submitted += 1
-        cost = float(req.geo.n_elems) * float(np.asarray(req.mat.t_coeff).size)
+        try:
+            cost = float(req.geo.n_elems) * float(np.asarray(req.mat.t_coeff).size)
+        except Exception:  # a malformed request fails in its preparation, alone
+            cost = 0.0
         with self._prep_lock:
             heapq.heappush(self._prep_queue, (-cost, req.id, req))
         self.preparing.append(self._pool.submit(self._prepare_next))
