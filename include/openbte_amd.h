@@ -167,9 +167,10 @@ typedef struct obte_bte_plan {
   const double* tb;        /* [n_wall] wall temperatures of the previous iteration */
   double* t;               /* [L][n_chunks][n + ncut_max][mc] solution workspace; rows >= n
                               hold the interface inflows of the running sweep */
-  double* part_tj;         /* [L][ps] per-direction sums in sweep order, ps = n (dim + 1)
-                              rounded up to even: lattice temperature and flux of position p
-                              at p (dim + 1) ... p (dim + 1) + dim */
+  double* part_tj;         /* [L][ps] per-direction sums in sweep order, ps = n (nf + 1)
+                              rounded up to even, nf = 1 with sig_a, else dim: lattice
+                              temperature and flux sums of position p at p (nf + 1) ...
+                              p (nf + 1) + nf */
   double* part_tb;         /* [L][n_chunks][n_wall] */
   double* kappa;           /* [M][L] */
   double* supp;            /* [M][L] */
@@ -177,12 +178,22 @@ typedef struct obte_bte_plan {
 
   /* final-sweep mode sums (windowed sweeps) */
   double* part_chunk;      /* [L][n_chunks][ps] per-chunk sums sum_m t_ml T_ml and
-                              sum_m sigma_ml T_ml of each position, written by windowed final
+                              sum_m sigma_ml T_ml (with sig_a: sum_m sig_a_ml T_ml) of each
+                              position, written by windowed final
                               sweeps, which then store in t only the values read later (keep
                               flags); NULL: final sweeps store every value and
                               obte_bte_reduce forms the sums from t */
   int32_t keep_t;          /* 1: windowed final sweeps still store every value in t (for
                               callers that read t afterwards) */
+
+  /* separable group velocities (optional).  When every direction's velocities
+     are parallel, sigma_ml = sig_a[m][l] sig_dir[l], the flux of a position
+     needs one mode sum, sum_m sig_a_ml T_ml, instead of dim (the direction
+     multiplies it in the sum over directions): part_tj and part_chunk then
+     hold 2 doubles per position instead of dim + 1.  NULL: sigma's dim
+     components are summed. */
+  const double* sig_a;     /* [M][L] or NULL */
+  const double* sig_dir;   /* [L][dim] (with sig_a) */
 } obte_bte_plan_t;
 
 /* Input / output buffers of one sweep launch for one problem. */

@@ -34,7 +34,7 @@ class BtePlan(C.Structure):
         ('pert', _p), ('rec', _p),
         ('delta_t', _p), ('tb', _p), ('t', _p), ('part_tj', _p), ('part_tb', _p),
         ('kappa', _p), ('supp', _p), ('scratch', _p),
-        ('part_chunk', _p), ('keep_t', _i32),
+        ('part_chunk', _p), ('keep_t', _i32), ('sig_a', _p), ('sig_dir', _p),
     ]
 
 

@@ -30,7 +30,7 @@ import torch
 from . import _lib
 from .objects import SolverResults
 from .operator import (HostOperator, Schedule, build_host_operator, build_schedule, fixed_width_inflow,
-                       level_pair_max, max_inflow)
+                       level_pair_max, max_inflow, separable_velocities)
 
 
 @dataclass
@@ -45,6 +45,7 @@ class SweepConfig:
     window: bool = True      # keep recent levels of a sweep in LDS where they fit
     wave_tasks: bool = True  # batches: one wave per task for problems with narrow levels
     fused_extend: bool = True  # one launch runs every task's whole Krylov extension loop
+    separable_flux: bool = True  # one flux mode sum per position when the velocities allow it
 
 
 class InterfaceNotConverged(UserWarning):
@@ -85,11 +86,18 @@ def _kmax(cfg: SweepConfig, ny: int) -> int:
     return int(max(8, min(cfg.kmax, cfg.basis_bytes // (16 * ny))))
 
 
-def part_stride(n: int, dim: int) -> int:
+def part_stride(n: int, nf: int) -> int:
     """Doubles per direction of the per-direction sums (and per chunk of the
-    per-chunk sums): n (dim + 1) rounded up to even, so rows are 16-byte
-    aligned."""
-    return (n * (dim + 1) + 1) // 2 * 2
+    per-chunk sums): n (nf + 1) rounded up to even, so rows are 16-byte
+    aligned; nf = flux sums per position (``flux_sums``)."""
+    return (n * (nf + 1) + 1) // 2 * 2
+
+
+def flux_sums(op: HostOperator, cfg: SweepConfig):
+    """``(nf, sig_a, sig_dir)``: flux mode sums per position and the separable
+    factors of the group velocities (1 with them, else dim and None)."""
+    a, d = separable_velocities(op.sigma) if cfg.separable_flux else (None, None)
+    return (op.dim, None, None) if a is None else (1, a, d)
 
 
 def _window_cap(cfg: SweepConfig, pair_max: int) -> int:
@@ -120,7 +128,10 @@ def device_bytes(op: HostOperator, sched: Schedule, cfg: Optional[SweepConfig] =
     f8 += L * n + L * n * 8                                  # pert, rec
     f8 += 2 * n + 2 * one(n_wall) + n * dim                  # dt, tb, j
     f8 += L * nch * (n + ncut_max) * mc                      # t
-    ps = part_stride(n, dim)
+    nf, sig_a, _ = flux_sums(op, cfg)
+    if sig_a is not None:
+        f8 += sig_a.size + L * dim                           # sig_a, sig_dir
+    ps = part_stride(n, nf)
     f8 += L * ps + one(L * nch * n_wall)                     # part_tj, part_tb
     f8 += 2 * M * L + 4096 + 8                               # kappa, supp, scratch
     if _window_cap(cfg, level_pair_max(sched)) > 0:
@@ -198,6 +209,10 @@ class BTEDevice:
         b['mfps'] = _t(op.mfps, f64, dev)
         b['t_coeff'] = _t(op.t_coeff, f64, dev)
         b['sigma'] = _t(op.sigma, f64, dev)
+        self.n_flux, sig_a, sig_dir = flux_sums(op, cfg)
+        if sig_a is not None:
+            b['sig_a'] = _t(sig_a, f64, dev)
+            b['sig_dir'] = _t(sig_dir, f64, dev)
         b['heat'] = _t(op.heat, f64, dev)
         if op.t_face is not None:
             b['t_face'] = _t(op.t_face, f64, dev)
@@ -211,7 +226,7 @@ class BTEDevice:
         b['j'] = torch.zeros(n * dim, dtype=f64, device=dev)
         # zero-initialised: unused inflow slots read a finite value times 0
         b['t'] = torch.zeros(L * nch * (n + self.ncut_max) * mc, dtype=f64, device=dev)
-        ps = part_stride(n, dim)
+        ps = part_stride(n, self.n_flux)
         b['part_tj'] = torch.empty(L * ps, dtype=f64, device=dev)
         b['part_tb'] = torch.empty(max(1, L * nch * n_wall), dtype=f64, device=dev)
         b['kappa'] = torch.zeros(M * L, dtype=f64, device=dev)
@@ -283,6 +298,8 @@ class BTEDevice:
         P.tb = b['tb'][self.cur].data_ptr()
         P.part_chunk = b['part_chunk'].data_ptr() if 'part_chunk' in b else 0
         P.keep_t = 0
+        if 'sig_a' in b:
+            P.sig_a, P.sig_dir = b['sig_a'].data_ptr(), b['sig_dir'].data_ptr()
         return P
 
     def _make_gcr(self) -> _lib.Gcr:

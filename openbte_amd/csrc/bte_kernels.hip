@@ -29,9 +29,17 @@ constexpr int kMaxBlock = 1024;
 constexpr int kRec = 8;
 
 // doubles per direction of the per-direction sums part_tj and per (direction,
-// chunk) of part_chunk: n (dim + 1) rounded up to even (16-byte rows)
-__host__ __device__ __forceinline__ int64_t part_stride(int n, int dim) {
-  return ((int64_t)n * (dim + 1) + 1) & ~(int64_t)1;
+// chunk) of part_chunk: n (nf + 1) rounded up to even (16-byte rows), nf =
+// flux sums per position (n_flux)
+__host__ __device__ __forceinline__ int64_t part_stride(int n, int nf) {
+  return ((int64_t)n * (nf + 1) + 1) & ~(int64_t)1;
+}
+
+// flux mode sums per position: one when the group velocities are separable
+// (sigma_ml = sig_a_ml sig_dir_l), else one per component
+template <class PT>
+__host__ __device__ __forceinline__ int n_flux(const PT& P) {
+  return P.sig_a != nullptr ? 1 : P.dim;
 }
 
 __device__ __forceinline__ double pack_ints(int lo, int hi) { return __hiloint2double(hi, lo); }
@@ -528,6 +536,7 @@ __device__ __forceinline__ void sweep_task(const S& src, int pid, int lt, SH& sh
       sh.wk[0][2 * jl + h] = ok ? P.t_coeff[(int64_t)m * L + l] : 0.0;
 #pragma unroll
       for (int d = 0; d < 3; ++d) sh.wk[1 + d][2 * jl + h] = sh.sig[3 * (2 * jl + h) + d];
+      if (P.sig_a != nullptr) sh.wk[1][2 * jl + h] = ok ? P.sig_a[(int64_t)m * L + l] : 0.0;
     }
   }
   bool act = true;  // transfer sweeps skip converged bin pairs
@@ -575,8 +584,9 @@ __device__ __forceinline__ void sweep_task(const S& src, int pid, int lt, SH& sh
     // final sweeps store every value unless they form the mode sums here
     // (then only values read later, unless the caller asked for all of them)
     const bool store_all = MODE == 2 || (kFuse && P.keep_t != 0);
-    const __amdgpu_buffer_rsrc_t ps_rs = make_rsrc(kFuse ? P.part_chunk + (int64_t)lt * part_stride(n, dim) : T,
-                                                   kFuse ? (int64_t)n * (dim + 1) * 8 : 0);
+    const int nf = n_flux(P);
+    const __amdgpu_buffer_rsrc_t ps_rs = make_rsrc(kFuse ? P.part_chunk + (int64_t)lt * part_stride(n, nf) : T,
+                                                   kFuse ? (int64_t)n * (nf + 1) * 8 : 0);
     // the upwind values of N (a level ending at h): window slot, or a load
     // from T when outside the window (lanes with nothing to load read past
     // the buffer: no memory access, a zero value)
@@ -628,19 +638,19 @@ __device__ __forceinline__ void sweep_task(const S& src, int pid, int lt, SH& sh
     // mode sums of a finished level (positions q0 .. q1), read back from the
     // window while the next level is computed: the level's slots are not
     // overwritten before the level after next (cap holds two adjacent levels).
-    // One thread per cell reads its row and stores its dim + 1 sums (the
+    // One thread per cell reads its row and stores its nf + 1 sums (the
     // level's sums are one contiguous range: consecutive positions).
     auto reduce_level = [&](int q0, int q1) {
       for (int p = q0 + tid; p < q1; p += nthr) {
         const double2* row = win + (p & cmask) * G;
         double e[4] = {0.0, 0.0, 0.0, 0.0};
         chunk_sums<G, true>([&](int j) { return row[j]; },
-                            [&](int k, int j) { return *reinterpret_cast<const double2*>(&sh.wk[k][2 * j]); }, dim,
+                            [&](int k, int j) { return *reinterpret_cast<const double2*>(&sh.wk[k][2 * j]); }, nf,
                             e);
 #pragma unroll
         for (int k = 0; k < 4; ++k)
-          if (k <= dim)
-            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, e[k]), ps_rs, (p * (dim + 1) + k) * 8, 0,
+          if (k <= nf)
+            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, e[k]), ps_rs, (p * (nf + 1) + k) * 8, 0,
                                                   0);
       }
     };
@@ -1463,25 +1473,31 @@ __global__ void k_gcr_reset(S src, int use_dt) {
 }
 
 // Stage 1 (openbte/bte.py:222-230): mode sums of each direction in sweep
-// order, interleaved per position (part_tj[l][p (dim + 1) + k]):
+// order, interleaved per position (part_tj[l][p (nf + 1) + k]):
 //   k = 0:   sum_m t_ml T_ml,
-//   k = 1..: sum_m sigma_ml T_ml - DeltaT_old sum_m sigma_ml,
+//   k = 1..: sum_m sigma_ml T_ml - DeltaT_old sum_m sigma_ml
+//            (separable velocities: sig_a_ml in place of sigma_ml, one sum),
 // summed chunk by chunk in order, each chunk's sum formed by chunk_sums.
 // k_modesum reads T from the solution workspace; k_modesum_part sums the
 // per-chunk sums a windowed final sweep wrote to part_chunk.  Both give the
 // same bits.  Grid: y = direction, z = problem.
 constexpr int kModesumMaxM = 1024;
 
-// sum over the bins of sigma_ml (fixed order), by one thread
+// sum over the bins of sigma_ml (sig_a_ml when separable; fixed order), by
+// one thread
 template <class PT>
 __device__ __forceinline__ void sigma_total(const PT& P, int l, double* s) {
   const int L = P.n_dirs, dim = P.dim;
   double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-  for (int m = 0; m < P.n_mfp; ++m) {
-    const double* sg = P.sigma + ((int64_t)m * L + l) * dim;
-    a0 += sg[0];
-    a1 += sg[1];
-    if (dim == 3) a2 += sg[2];
+  if (P.sig_a != nullptr) {
+    for (int m = 0; m < P.n_mfp; ++m) a0 += P.sig_a[(int64_t)m * L + l];
+  } else {
+    for (int m = 0; m < P.n_mfp; ++m) {
+      const double* sg = P.sigma + ((int64_t)m * L + l) * dim;
+      a0 += sg[0];
+      a1 += sg[1];
+      if (dim == 3) a2 += sg[2];
+    }
   }
   s[0] = a0;
   s[1] = a1;
@@ -1492,9 +1508,10 @@ template <class PT>
 __device__ __forceinline__ void modesum_store(const PT& P, int l, int p, const double (&acc)[4],
                                               const double* stot) {
   const double dt = P.delta_t[P.order[(int64_t)l * P.n + p]];
-  double* out = P.part_tj + (int64_t)l * part_stride(P.n, P.dim) + (int64_t)p * (P.dim + 1);
+  const int nf = n_flux(P);
+  double* out = P.part_tj + (int64_t)l * part_stride(P.n, nf) + (int64_t)p * (nf + 1);
   out[0] = acc[0];
-  for (int d = 0; d < P.dim; ++d) out[1 + d] = fma(-dt, stot[d], acc[1 + d]);
+  for (int d = 0; d < nf; ++d) out[1 + d] = fma(-dt, stot[d], acc[1 + d]);
 }
 
 // From t: one thread per position, the bin pairs' weights staged in LDS.
@@ -1516,11 +1533,12 @@ __global__ __launch_bounds__(256) void k_modesum(S src) {
       const bool ok = m < M;
       const double* sg = P.sigma + ((int64_t)(ok ? m : 0) * L + l) * dim;
       w[8 * q + h] = ok ? P.t_coeff[(int64_t)m * L + l] : 0.0;
-      w[8 * q + 2 + h] = ok ? sg[0] : 0.0;
+      w[8 * q + 2 + h] = ok ? (P.sig_a != nullptr ? P.sig_a[(int64_t)m * L + l] : sg[0]) : 0.0;
       w[8 * q + 4 + h] = ok ? sg[1] : 0.0;
       w[8 * q + 6 + h] = ok && dim == 3 ? sg[2] : 0.0;
     }
   }
+  const int nf = n_flux(P);
   if (threadIdx.x == 0) sigma_total(P, l, stot);
   __syncthreads();
   const int64_t rows = (int64_t)n + P.ncut_max;
@@ -1530,7 +1548,7 @@ __global__ __launch_bounds__(256) void k_modesum(S src) {
       const double2* row = reinterpret_cast<const double2*>(P.t + (((int64_t)l * nch + c) * rows + p) * MC);
       const double2* wc = reinterpret_cast<const double2*>(w + 8 * c * G);
       double e[4] = {0.0, 0.0, 0.0, 0.0};
-      chunk_sums<G, false>([&](int j) { return row[j]; }, [&](int k, int j) { return wc[4 * j + k]; }, dim, e);
+      chunk_sums<G, false>([&](int j) { return row[j]; }, [&](int k, int j) { return wc[4 * j + k]; }, nf, e);
 #pragma unroll
       for (int k = 0; k < 4; ++k) acc[k] += e[k];
     }
@@ -1547,26 +1565,27 @@ __global__ __launch_bounds__(256) void k_modesum_part(S src) {
   if (!src.live(pid)) return;
   const auto& P = src.plan(pid);
   const int l = blockIdx.y;
-  const int n = P.n, dim = P.dim, nch = P.n_chunks;
+  const int n = P.n, nf = n_flux(P), nch = P.n_chunks;
   if (l >= P.n_dirs || (int64_t)blockIdx.x * blockDim.x >= n) return;  // uniform over the block
   if (threadIdx.x == 0) sigma_total(P, l, stot);
   __syncthreads();
-  const int64_t ps = part_stride(n, dim);
+  const int64_t ps = part_stride(n, nf);
   for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     for (int c = 0; c < nch; ++c) {
-      const double* e = P.part_chunk + ((int64_t)l * nch + c) * ps + (int64_t)p * (dim + 1);
+      const double* e = P.part_chunk + ((int64_t)l * nch + c) * ps + (int64_t)p * (nf + 1);
       acc[0] += e[0];
       acc[1] += e[1];
-      acc[2] += e[2];
-      if (dim == 3) acc[3] += e[3];
+      if (nf >= 2) acc[2] += e[2];
+      if (nf == 3) acc[3] += e[3];
     }
     modesum_store(P, l, p, acc, stot);
   }
 }
 
 // Stage 2: fixed-order sum over directions, gathering each direction's
-// partial at the cell's position; one thread per cell.
+// partial at the cell's position; one thread per cell.  With separable
+// velocities the flux of direction l is sig_dir_l times its one sum.
 template <class S>
 __global__ void k_reduce_cells(S src) {
   __shared__ double red[256];
@@ -1580,13 +1599,25 @@ __global__ void k_reduce_cells(S src) {
   double s = 0.0;
   if (c < P.n) {
     double j0 = 0.0, j1 = 0.0, j2 = 0.0;
-    const int64_t ps = part_stride(P.n, dim);
-    for (int l = 0; l < P.n_dirs; ++l) {
-      const double* e = P.part_tj + l * ps + (int64_t)P.pos[(int64_t)l * P.n + c] * (dim + 1);
-      s += e[0];
-      j0 += e[1];
-      j1 += e[2];
-      if (dim == 3) j2 += e[3];
+    const int nf = n_flux(P);
+    const int64_t ps = part_stride(P.n, nf);
+    if (P.sig_a != nullptr) {
+      for (int l = 0; l < P.n_dirs; ++l) {
+        const double* e = P.part_tj + l * ps + (int64_t)P.pos[(int64_t)l * P.n + c] * 2;
+        const double* dl = P.sig_dir + (int64_t)l * dim;
+        s += e[0];
+        j0 = fma(dl[0], e[1], j0);
+        j1 = fma(dl[1], e[1], j1);
+        if (dim == 3) j2 = fma(dl[2], e[1], j2);
+      }
+    } else {
+      for (int l = 0; l < P.n_dirs; ++l) {
+        const double* e = P.part_tj + l * ps + (int64_t)P.pos[(int64_t)l * P.n + c] * (dim + 1);
+        s += e[0];
+        j0 += e[1];
+        j1 += e[2];
+        if (dim == 3) j2 += e[3];
+      }
     }
     R.dt_new[c] = s;
     R.j_new[(int64_t)c * dim + 0] = j0;
@@ -1683,6 +1714,7 @@ int check_plan(const obte_bte_plan_t* P, const char* who) {
   if (P->kin != 2 && P->kin != 3) return obte_fail(OBTE_EINVAL, "plan: kin must be 2 or 3");
   if (P->n_mfp > kModesumMaxM) return obte_fail(OBTE_ELIMIT, "plan: more than 1024 MFP bins");
   if (P->ncut_max < 0) return obte_fail(OBTE_EINVAL, "plan: ncut_max < 0");
+  if (P->sig_a && !P->sig_dir) return obte_fail(OBTE_EINVAL, "plan: sig_a without sig_dir");
   if ((P->n + 255) / 256 > 4096) return obte_fail(OBTE_ELIMIT, "plan: more than 4096*256 cells (scratch size)");
   return OBTE_OK;
 }

@@ -205,6 +205,30 @@ def build_host_operator(geo, mat, bcs, effective_thermal_conductivity=None, heat
 DEFAULT_MAX_WIDTH = 192
 
 
+def separable_velocities(sigma: np.ndarray, rtol: float = 1e-13):
+    """``(sig_a [M][L], sig_dir [L][dim])`` with sigma_ml = sig_a_ml sig_dir_l
+    (sig_dir_l a unit vector) when every direction's group velocities are
+    parallel to within ``rtol`` of max |sigma|, else ``(None, None)``.  Gray
+    models give such velocities (openbte/material.py:192-305: sigma = const
+    s_l); the RTA models' velocities, interpolated from the bulk modes onto the
+    angle grid (openbte/material.py:7-189), generally do not.  With them the
+    flux of a position needs one mode sum instead of dim."""
+    sigma = np.asarray(sigma, np.float64)
+    M, L, dim = sigma.shape
+    scale = np.abs(sigma).max() if sigma.size else 0.0
+    if scale == 0.0:
+        return None, None
+    nrm = np.linalg.norm(sigma, axis=2)
+    top = nrm.argmax(axis=0)
+    ref = sigma[top, np.arange(L)]
+    rn = nrm[top, np.arange(L)][:, None]
+    d = np.divide(ref, rn, out=np.zeros_like(ref), where=rn > 0)
+    a = np.einsum('mld,ld->ml', sigma, d)
+    if np.abs(sigma - a[:, :, None] * d[None]).max() > rtol * scale:
+        return None, None
+    return np.ascontiguousarray(a), np.ascontiguousarray(d)
+
+
 def batch_max_width(mc: int, block: int) -> int:
     """Cells per level of the schedules batched and served problems use: one
     pass of the persistent batch kernel, 2 cells per thread and mc / 2 lanes

@@ -380,6 +380,37 @@ def test_bte_3d_slab_matches_oracle():
         assert np.abs(out.variables['Flux_BTE']['data'] - Jr).max() < 1e-7 * np.abs(Jr).max()
 
 
+@pytest.mark.parametrize("dim,window", [(2, True), (2, False), (3, True)])
+def test_separable_flux_sums_equal_component_sums(small_case, dim, window):
+    """Separable velocities (one flux sum per position, the direction applied
+    in the sum over directions) against the per-component sums: temperatures,
+    kappa and iteration counts identical bit for bit (their sums do not
+    change), fluxes equal to rounding."""
+    from openbte_amd.bte import BTE_RTA, SweepConfig
+    from openbte_amd.fourier import Fourier
+    if dim == 2:
+        geo, mat, bcs, eff = small_case
+    else:
+        from openbte_amd.geometry import pore_slab_3d
+        from openbte_amd.material import synthetic_material_3d
+        from openbte_amd.objects import BoundaryConditions, EffectiveThermalConductivity
+        geo = pore_slab_3d(10.0, 3.0, 0.2, 900, seed=2, nz=3)
+        mat = synthetic_material_3d(5, 4, 4, 1.0, 1e3)
+        bcs = BoundaryConditions(periodic={'Periodic_x': 1.0, 'Periodic_y': 0.0}, diffuse='Boundary')
+        eff = EffectiveThermalConductivity(contact='Periodic_x', normalization=-1.0 / 3.0)
+    f = Fourier(geo, mat.thermal_conductivity, bcs, eff, verbose=False)
+    outs = [BTE_RTA(geo, mat, bcs, fourier=f, effective_thermal_conductivity=eff, bte_max_iter=6, verbose=False,
+                    config=SweepConfig(mc=4, block=64, window=window, separable_flux=sep)) for sep in (True, False)]
+    a, b = outs
+    assert a.aux['n_iter'] == b.aux['n_iter']
+    assert a.kappa_eff == b.kappa_eff
+    np.testing.assert_array_equal(a.variables['Temperature_BTE']['data'], b.variables['Temperature_BTE']['data'])
+    np.testing.assert_array_equal(a.aux['suppression'], b.aux['suppression'])
+    Ja, Jb = a.variables['Flux_BTE']['data'], b.variables['Flux_BTE']['data']
+    print('separable flux', dim, window, np.abs(Ja - Jb).max() / np.abs(Jb).max())
+    assert np.abs(Ja - Jb).max() <= 1e-13 * np.abs(Jb).max()
+
+
 def test_bte_3d_nanowire_drive_matches_oracle():
     """Nanowire drive: the perturbation gains -s_z in every cell
     (openbte/bte.py:112-113); no kappa contact (the reference's nanowire
