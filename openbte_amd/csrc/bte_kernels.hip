@@ -799,55 +799,92 @@ __device__ __forceinline__ void sweep_task(const S& src, int pid, int lt, SH& sh
 
   if (kFinal) {
     const int32_t* __restrict__ pos = P.pos + lo;
-    // wall temperatures: TB_s = -sum_ml T_ml[i_s] GG_ml,s  (openbte/bte.py:73-78,227)
+    // wall temperatures: TB_s = -sum_ml T_ml[i_s] GG_ml,s  (openbte/bte.py:73-78,227);
+    // a side's row is read KB values at a time, all loads in flight together
+    constexpr int KB = 4;
     for (int sd = tid; sd < P.n_wall; sd += nthr) {
       const double* db = P.wall_dbs + (int64_t)sd * dim;
       const double rinv = P.wall_rinv[sd];
-      const double* row = T + (int64_t)pos[P.wall_cell[sd]] * MC;
+      const double2* row = reinterpret_cast<const double2*>(T + (int64_t)pos[P.wall_cell[sd]] * MC);
       double acc = 0.0;
 #pragma unroll 1
-      for (int m = 0; m < MC; ++m) {
-        double gb = sh.sig[3 * m] * db[0] + sh.sig[3 * m + 1] * db[1];
-        if (dim == 3) gb += sh.sig[3 * m + 2] * db[2];
-        acc += -row[m] * ((gb > 0.0 ? gb : 0.0) * rinv);
+      for (int m0 = 0; m0 < MC; m0 += KB) {
+        double v[KB];
+#pragma unroll
+        for (int j = 0; j < KB / 2; ++j) {
+          const double2 t = row[m0 / 2 + j];
+          v[2 * j] = t.x;
+          v[2 * j + 1] = t.y;
+        }
+#pragma unroll
+        for (int j = 0; j < KB; ++j) {
+          const int m = m0 + j;
+          double gb = sh.sig[3 * m] * db[0] + sh.sig[3 * m + 1] * db[1];
+          if (dim == 3) gb += sh.sig[3 * m + 2] * db[2];
+          acc += -v[j] * ((gb > 0.0 ? gb : 0.0) * rinv);
+        }
       }
       P.part_tb[(int64_t)lt * P.n_wall + sd] = acc;
     }
     // kappa and suppression of each mode (openbte/bte.py:232-243), summed
     // by the task's first wave in a fixed order (lane = contact side mod 64,
     // then an xor tree), so a task run by one wave or by a workgroup gives
-    // the same bits
+    // the same bits.  KB bins at a time: a contact side's indices and normal
+    // are read once per KB bins and its rows KB values at a time.
     if (P.contact_kind != 0 && tid < 64) {
 #pragma unroll 1
-      for (int m = 0; m < MC; ++m) {
-        double kap = 0.0, sup = 0.0;
-        const double s0 = sh.sig[3 * m], s1 = sh.sig[3 * m + 1], s2 = sh.sig[3 * m + 2];
+      for (int m0 = 0; m0 < MC; m0 += KB) {
+        double kap[KB], sup[KB];
+#pragma unroll
+        for (int j = 0; j < KB; ++j) kap[j] = sup[j] = 0.0;
         for (int sc = tid; sc < P.n_contact; sc += 64) {
           const double* na = P.c_na + (int64_t)sc * dim;
-          double sn = s0 * na[0] + s1 * na[1];
-          double area2 = na[0] * na[0] + na[1] * na[1];
-          if (dim == 3) {
-            sn += s2 * na[2];
-            area2 += na[2] * na[2];
-          }
-          const double t1 = T[(int64_t)pos[P.ce1[sc]] * MC + m];
-          const double snp = sn > 0.0 ? sn : 0.0;
-          const double snm = sn < 0.0 ? sn : 0.0;
-          double kc = t1 * snp + P.contact_jump * snm;
-          if (P.contact_kind == 1) kc += T[(int64_t)pos[P.ce2[sc]] * MC + m] * snm;
-          kap += P.c_t != nullptr ? P.c_t[sc] * kc : kc;
-          sup += t1 * sqrt(area2);
-        }
-        double ksum = kap, ssum = sup;
+          const double2* r1 = reinterpret_cast<const double2*>(T + (int64_t)pos[P.ce1[sc]] * MC + m0);
+          const double2* r2 =
+              reinterpret_cast<const double2*>(T + (int64_t)pos[P.contact_kind == 1 ? P.ce2[sc] : P.ce1[sc]] * MC + m0);
+          double v1[KB], v2[KB];
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-          ksum += __shfl_xor(ksum, o, 64);
-          ssum += __shfl_xor(ssum, o, 64);
+          for (int j = 0; j < KB / 2; ++j) {
+            const double2 a = r1[j];
+            v1[2 * j] = a.x;
+            v1[2 * j + 1] = a.y;
+            const double2 b = P.contact_kind == 1 ? r2[j] : make_double2(0.0, 0.0);
+            v2[2 * j] = b.x;
+            v2[2 * j + 1] = b.y;
+          }
+          const double ct = P.c_t != nullptr ? P.c_t[sc] : 1.0;
+#pragma unroll
+          for (int j = 0; j < KB; ++j) {
+            const int m = m0 + j;
+            const double s0 = sh.sig[3 * m], s1 = sh.sig[3 * m + 1], s2 = sh.sig[3 * m + 2];
+            double sn = s0 * na[0] + s1 * na[1];
+            double area2 = na[0] * na[0] + na[1] * na[1];
+            if (dim == 3) {
+              sn += s2 * na[2];
+              area2 += na[2] * na[2];
+            }
+            const double t1 = v1[j];
+            const double snp = sn > 0.0 ? sn : 0.0;
+            const double snm = sn < 0.0 ? sn : 0.0;
+            double kc = t1 * snp + P.contact_jump * snm;
+            if (P.contact_kind == 1) kc += v2[j] * snm;
+            kap[j] += P.c_t != nullptr ? ct * kc : kc;
+            sup[j] += t1 * sqrt(area2);
+          }
         }
-        const int mg = chunk * MC + m;
-        if (tid == 0 && mg < P.n_mfp) {
-          P.kappa[(int64_t)mg * L + l] = ksum * P.normalization;
-          P.supp[(int64_t)mg * L + l] = ssum * P.normalization;
+#pragma unroll
+        for (int j = 0; j < KB; ++j) {
+          double ksum = kap[j], ssum = sup[j];
+#pragma unroll
+          for (int o = 32; o > 0; o >>= 1) {
+            ksum += __shfl_xor(ksum, o, 64);
+            ssum += __shfl_xor(ssum, o, 64);
+          }
+          const int mg = chunk * MC + m0 + j;
+          if (tid == 0 && mg < P.n_mfp) {
+            P.kappa[(int64_t)mg * L + l] = ksum * P.normalization;
+            P.supp[(int64_t)mg * L + l] = ssum * P.normalization;
+          }
         }
       }
     }
