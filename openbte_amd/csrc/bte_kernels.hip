@@ -27,6 +27,9 @@ constexpr int kMaxBlock = 1024;
 //   [0] D_l  [1] b0 = DeltaT + heat  [2] q  [3..5] inflow coefficients (unused = 0)
 //   [6] (src0, src1)  [7] (src2, cell) as int32 pairs
 constexpr int kRec = 8;
+// faces (or values) per thread and step of the latency-bound per-face loops:
+// their loads go out together
+constexpr int kFaces = 4;
 
 // doubles per direction of the per-direction sums part_tj and per (direction,
 // chunk) of part_chunk: n (nf + 1) rounded up to even (16-byte rows), nf =
@@ -459,6 +462,9 @@ __device__ __forceinline__ int64_t uniform_i64(int64_t v) {
 __device__ __forceinline__ double buf_load1(__amdgpu_buffer_rsrc_t rs, int off) {
   return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, 0));
 }
+__device__ __forceinline__ void buf_store1(__amdgpu_buffer_rsrc_t rs, int off, double v) {
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), rs, off, 0, 0);
+}
 __device__ __forceinline__ void buf_store2(__amdgpu_buffer_rsrc_t rs, int off, double2 t) {
   const f64x2 v = {t.x, t.y};
   __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, v), rs, off, 0, 0);
@@ -788,12 +794,26 @@ __device__ __forceinline__ void sweep_task(const S& src, int pid, int lt, SH& sh
   }
 
   // exit values of the cut faces (upstream cells across the cut)
+  // (kFaces values per thread and step, every index and value load issued
+  // before the stores)
   if (io.x != nullptr) {
     double* __restrict__ xv = io.x + ((int64_t)chunk * P.n_cut + cbase) * MC;
     const int32_t* __restrict__ ex = P.cut_exit + cbase;
-    for (int i = tid; i < ncut * MC; i += nthr) {
-      const int f = i / MC;
-      xv[i] = T[(int64_t)ex[f] * MC + (i - f * MC)];
+    const int total = ncut * MC;
+    for (int i0 = tid; i0 < total; i0 += kFaces * nthr) {
+      int src[kFaces];
+#pragma unroll
+      for (int q = 0; q < kFaces; ++q) {
+        const int i = min(i0 + q * nthr, total - 1);
+        const int f = i / MC;
+        src[q] = ex[f] * MC + (i - f * MC);
+      }
+      double t[kFaces];
+#pragma unroll
+      for (int q = 0; q < kFaces; ++q) t[q] = T[(int64_t)src[q]];
+#pragma unroll
+      for (int q = 0; q < kFaces; ++q)
+        if (i0 + q * nthr < total) xv[i0 + q * nthr] = t[q];
     }
   }
 
@@ -1053,13 +1073,16 @@ __device__ __forceinline__ void block_lane_sums(double (&v)[JB], double* red, do
 constexpr int kGsBlock = 8;  // basis vectors per block of the Gram-Schmidt passes
 
 // cn -= sum_j (c_j . cn) c_j and un -= the same combination of u_j over the
-// first kb basis vectors, JB vectors at a time (dot products of a block
-// together, blocks in sequence).  Each thread walks its faces f = slot,
-// slot + cpp, ... in order; the loads of R consecutive faces are issued
-// together before their arithmetic (the loop is bound by memory latency: a
-// task's interface is a few hundred faces per thread group), which keeps the
-// per-thread operation order and so the results of a face-at-a-time loop.
-template <int MC, class GT>
+// first kb basis vectors (kAddU: un += it), JB vectors at a time (dot products
+// of a block together, blocks in sequence).  Each thread walks its faces
+// f = slot, slot + cpp, ... in order; the loads of R consecutive faces are
+// issued together before their arithmetic (the loop is bound by memory
+// latency: a task's interface is a few hundred faces per thread group), which
+// keeps the per-thread operation order and so the results of a face-at-a-time
+// loop.  Every vector is read through a buffer resource of the task's slice:
+// faces past the end read zero (adding +0 to the sums) and their stores are
+// dropped, so the last group of faces needs no separate, load-by-load loop.
+template <int MC, bool kAddU = false, class GT>
 __device__ void gs_pass(const GT& G, int64_t stride_, int64_t base, int ncut_, int kb_, double* cn,
                         double* un, double* red, double* bc) {
   // uniform over the block (a batch reads them from memory per task)
@@ -1075,6 +1098,8 @@ __device__ void gs_pass(const GT& G, int64_t stride_, int64_t base, int ncut_, i
   const int64_t vbytes = (int64_t)ncut * MC * 8;  // this task's slice of one basis vector
   const double* const gcb = uniform_ptr(G.c + base);
   const double* const gub = uniform_ptr(G.u + base);
+  const __amdgpu_buffer_rsrc_t cns = make_rsrc(uniform_ptr(cn), vbytes);
+  const __amdgpu_buffer_rsrc_t uns = make_rsrc(uniform_ptr(un), vbytes);
   for (int j0 = 0; j0 < kb; j0 += JB) {
     // one buffer per basis vector of the block (vectors past kb: empty
     // buffers, which read as zero and are not used)
@@ -1088,13 +1113,12 @@ __device__ void gs_pass(const GT& G, int64_t stride_, int64_t base, int ncut_, i
     double d[JB];
 #pragma unroll
     for (int jj = 0; jj < JB; ++jj) d[jj] = 0.0;
-    int f = slot;
-    for (; f + (RD - 1) * cpp < ncut; f += RD * cpp) {
+    for (int f = slot; f < ncut; f += RD * cpp) {
       double c0[RD], cc[RD][JB];
 #pragma unroll
       for (int r = 0; r < RD; ++r) {
         const int e = (f + r * cpp) * MC + lane;
-        c0[r] = cn[e];
+        c0[r] = buf_load1(cns, e * 8);
 #pragma unroll
         for (int jj = 0; jj < JB; ++jj) cc[r][jj] = buf_load1(crs[jj], e * 8);
       }
@@ -1104,22 +1128,14 @@ __device__ void gs_pass(const GT& G, int64_t stride_, int64_t base, int ncut_, i
         for (int jj = 0; jj < JB; ++jj)
           if (j0 + jj < kb) d[jj] += cc[r][jj] * c0[r];
     }
-    for (; f < ncut; f += cpp) {
-      const int e = f * MC + lane;
-      const double c0 = cn[e];
-#pragma unroll
-      for (int jj = 0; jj < JB; ++jj)
-        if (j0 + jj < kb) d[jj] += buf_load1(crs[jj], e * 8) * c0;
-    }
     block_lane_sums<MC, JB>(d, red, bc);
-    f = slot;
-    for (; f + (RU - 1) * cpp < ncut; f += RU * cpp) {
+    for (int f = slot; f < ncut; f += RU * cpp) {
       double cv[RU], uv[RU], cc[RU][JB], uu[RU][JB];
 #pragma unroll
       for (int r = 0; r < RU; ++r) {
         const int e = (f + r * cpp) * MC + lane;
-        cv[r] = cn[e];
-        uv[r] = un[e];
+        cv[r] = buf_load1(cns, e * 8);
+        uv[r] = buf_load1(uns, e * 8);
 #pragma unroll
         for (int jj = 0; jj < JB; ++jj) {
           cc[r][jj] = buf_load1(crs[jj], e * 8);
@@ -1132,24 +1148,13 @@ __device__ void gs_pass(const GT& G, int64_t stride_, int64_t base, int ncut_, i
         for (int jj = 0; jj < JB; ++jj)
           if (j0 + jj < kb) {
             cv[r] -= d[jj] * cc[r][jj];
-            uv[r] -= d[jj] * uu[r][jj];
+            if constexpr (kAddU) uv[r] += d[jj] * uu[r][jj];
+            else uv[r] -= d[jj] * uu[r][jj];
           }
         const int e = (f + r * cpp) * MC + lane;
-        cn[e] = cv[r];
-        un[e] = uv[r];
+        buf_store1(cns, e * 8, cv[r]);
+        buf_store1(uns, e * 8, uv[r]);
       }
-    }
-    for (; f < ncut; f += cpp) {
-      const int e = f * MC + lane;
-      double cv = cn[e], uv = un[e];
-#pragma unroll
-      for (int jj = 0; jj < JB; ++jj)
-        if (j0 + jj < kb) {
-          cv -= d[jj] * buf_load1(crs[jj], e * 8);
-          uv -= d[jj] * buf_load1(urs[jj], e * 8);
-        }
-      cn[e] = cv;
-      un[e] = uv;
     }
   }
 }
@@ -1203,39 +1208,62 @@ __device__ void gcr_flags(const PT& P, const GT& G, int task, int l, int chunk,
   }
 }
 
+// Per-face loops over a task's interface slice: each thread takes its faces
+// f = slot, slot + cpp, ... in order, kFaces of them per step with every load
+// issued before the arithmetic; vectors go through buffer resources of the
+// slice, so faces past the end read zero and drop their stores.
+
+struct Slice {
+  int ncut, slot, cpp, lane, mc;
+  __device__ __forceinline__ int off(int f, int q) const { return ((f + q * cpp) * mc + lane) * 8; }
+};
+
+template <int MC>
+__device__ __forceinline__ Slice slice_of(const GcrView& v) {
+  return Slice{v.ncut, (int)threadIdx.x / MC, (int)blockDim.x / MC, (int)threadIdx.x % MC, MC};
+}
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t slice_rsrc(const double* p, const Slice& S) {
+  return make_rsrc(uniform_ptr(p), (int64_t)S.ncut * S.mc * 8);
+}
+
 // r = x - y; MGS projection of r on the recycled basis; y += U a.  One task
 // by the whole block; red, bc: kGsBlock * kSweepBlock and kGsBlock * 64 doubles.
 template <int MC, class PT, class GT>
 __device__ __forceinline__ void gcr_project_task(const PT& P, const GT& G, int task, double* red, double* bc) {
   const int l = task / P.n_chunks;
   const int chunk = task - l * P.n_chunks;
-  const int lane = threadIdx.x % MC;
-  const int slot = threadIdx.x / MC;
-  const int cpp = blockDim.x / MC;
   const GcrView v = gcr_view(P, l, chunk, MC);
+  const Slice S = slice_of<MC>(v);
   double* y = G.y + v.base;
   double* r = G.r + v.base;
-  const double* x = G.x + v.base;
-  for (int f = slot; f < v.ncut; f += cpp) {
-    const int64_t o = (int64_t)f * MC + lane;
-    r[o] = x[o] - y[o];
+  const __amdgpu_buffer_rsrc_t ys = slice_rsrc(y, S), rs = slice_rsrc(r, S), xs = slice_rsrc(G.x + v.base, S);
+  for (int f = S.slot; f < S.ncut; f += kFaces * S.cpp) {
+    double xv[kFaces], yv[kFaces];
+#pragma unroll
+    for (int q = 0; q < kFaces; ++q) {
+      xv[q] = buf_load1(xs, S.off(f, q));
+      yv[q] = buf_load1(ys, S.off(f, q));
+    }
+#pragma unroll
+    for (int q = 0; q < kFaces; ++q) buf_store1(rs, S.off(f, q), xv[q] - yv[q]);
   }
   const int kb = G.k[task];
-  // r -= C a, y -= U (-a) with a = C^T r: gs_pass with (cn, un) = (r, -y)
-  for (int f = slot; f < v.ncut; f += cpp) {
-    const int64_t o = (int64_t)f * MC + lane;
-    y[o] = -y[o];
-  }
-  gs_pass<MC>(G, v.stride, v.base, v.ncut, kb, r, y, red, bc);
-  for (int f = slot; f < v.ncut; f += cpp) {
-    const int64_t o = (int64_t)f * MC + lane;
-    y[o] = -y[o];
-  }
+  // r -= C a, y += U a with a = C^T r
+  gs_pass<MC, true>(G, v.stride, v.base, v.ncut, kb, r, y, red, bc);
   double rp = 0.0, ym = 0.0;
-  for (int f = slot; f < v.ncut; f += cpp) {
-    const int64_t o = (int64_t)f * MC + lane;
-    rp += r[o] * r[o];
-    ym = fmax(ym, fabs(y[o]));
+  for (int f = S.slot; f < S.ncut; f += kFaces * S.cpp) {
+    double rv[kFaces], yv[kFaces];
+#pragma unroll
+    for (int q = 0; q < kFaces; ++q) {
+      rv[q] = buf_load1(rs, S.off(f, q));
+      yv[q] = buf_load1(ys, S.off(f, q));
+    }
+#pragma unroll
+    for (int q = 0; q < kFaces; ++q) {
+      rp += rv[q] * rv[q];
+      ym = fmax(ym, fabs(yv[q]));
+    }
   }
   const double rn2 = block_lane_sum<MC>(rp, red, bc);
   const double ymax = block_lane_max<MC>(ym, red, bc);
@@ -1256,16 +1284,16 @@ template <int MC, class PT, class GT>
 __device__ __forceinline__ void gcr_begin_task(const PT& P, const GT& G, int task) {
   const int l = task / P.n_chunks;
   const int chunk = task - l * P.n_chunks;
-  const int lane = threadIdx.x % MC;
-  const int slot = threadIdx.x / MC;
-  const int cpp = blockDim.x / MC;
   const GcrView v = gcr_view(P, l, chunk, MC);
-  const int act = G.lane_active[(int64_t)task * MC + lane];
-  const double* r = G.r + v.base;
-  double* w = G.w + v.base;
-  for (int f = slot; f < v.ncut; f += cpp) {
-    const int64_t o = (int64_t)f * MC + lane;
-    w[o] = act ? r[o] : 0.0;
+  const Slice S = slice_of<MC>(v);
+  const int act = G.lane_active[(int64_t)task * MC + S.lane];
+  const __amdgpu_buffer_rsrc_t rs = slice_rsrc(G.r + v.base, S), ws = slice_rsrc(G.w + v.base, S);
+  for (int f = S.slot; f < S.ncut; f += kFaces * S.cpp) {
+    double rv[kFaces];
+#pragma unroll
+    for (int q = 0; q < kFaces; ++q) rv[q] = buf_load1(rs, S.off(f, q));
+#pragma unroll
+    for (int q = 0; q < kFaces; ++q) buf_store1(ws, S.off(f, q), act ? rv[q] : 0.0);
   }
 }
 
@@ -1279,6 +1307,19 @@ __global__ __launch_bounds__(kSweepBlock) void k_gcr_begin(S src) {
   gcr_begin_task<MC>(src.plan(pid), G, task);
 }
 
+// sum over this thread's faces of v^2 (faces in order)
+__device__ __forceinline__ double slice_norm2(__amdgpu_buffer_rsrc_t vs, const Slice& S) {
+  double part = 0.0;
+  for (int f = S.slot; f < S.ncut; f += kFaces * S.cpp) {
+    double t[kFaces];
+#pragma unroll
+    for (int q = 0; q < kFaces; ++q) t[q] = buf_load1(vs, S.off(f, q));
+#pragma unroll
+    for (int q = 0; q < kFaces; ++q) part += t[q] * t[q];
+  }
+  return part;
+}
+
 // c = w - x (= (I - T_op) w); orthogonalise against the basis, normalise,
 // append, then y += a u, r -= a c with a = c . r.  One active task by the
 // whole block (red, bc as for gcr_project_task).
@@ -1287,67 +1328,83 @@ __device__ __forceinline__ void gcr_end_task(const PT& P, const GT& G, int task,
                              bool count) {
   const int l = task / P.n_chunks;
   const int chunk = task - l * P.n_chunks;
-  const int lane = threadIdx.x % MC;
-  const int slot = threadIdx.x / MC;
-  const int cpp = blockDim.x / MC;
   const GcrView v = gcr_view(P, l, chunk, MC);
+  const Slice S = slice_of<MC>(v);
   int kb = G.k[task];
   if (kb >= G.kmax) kb = 0;  // restart: drop the recycled basis
   double* un = G.u + kb * v.stride + v.base;
   double* cn = G.c + kb * v.stride + v.base;
-  const double* w = G.w + v.base;
-  const double* x = G.x + v.base;
+  const __amdgpu_buffer_rsrc_t us = slice_rsrc(un, S), cs = slice_rsrc(cn, S);
+  const __amdgpu_buffer_rsrc_t ws = slice_rsrc(G.w + v.base, S), xs = slice_rsrc(G.x + v.base, S);
   // converged lanes carry (u, c) = (0, 0): the transfer sweep may have skipped
   // their bins, so their exit values are not T_op(0)
-  const bool act = G.lane_active[(int64_t)task * MC + lane] != 0;
-  for (int f = slot; f < v.ncut; f += cpp) {
-    const int64_t o = (int64_t)f * MC + lane;
-    un[o] = act ? w[o] : 0.0;
-    cn[o] = act ? w[o] - x[o] : 0.0;
+  const bool act = G.lane_active[(int64_t)task * MC + S.lane] != 0;
+  double part = 0.0;
+  for (int f = S.slot; f < S.ncut; f += kFaces * S.cpp) {
+    double wv[kFaces], xv[kFaces];
+#pragma unroll
+    for (int q = 0; q < kFaces; ++q) {
+      wv[q] = buf_load1(ws, S.off(f, q));
+      xv[q] = buf_load1(xs, S.off(f, q));
+    }
+#pragma unroll
+    for (int q = 0; q < kFaces; ++q) {
+      const double c = act ? wv[q] - xv[q] : 0.0;
+      buf_store1(us, S.off(f, q), act ? wv[q] : 0.0);
+      buf_store1(cs, S.off(f, q), c);
+      part += c * c;
+    }
   }
   // block Gram-Schmidt; a second pass only where the first one cancelled
   // more than half of the vector's norm ("twice is enough", Kahan-Parlett)
-  double part = 0.0;
-  for (int f = slot; f < v.ncut; f += cpp) {
-    const int64_t o = (int64_t)f * MC + lane;
-    part += cn[o] * cn[o];
-  }
   const double n_before = block_lane_sum<MC>(part, red, bc);
   gs_pass<MC>(G, v.stride, v.base, v.ncut, kb, cn, un, red, bc);
-  part = 0.0;
-  for (int f = slot; f < v.ncut; f += cpp) {
-    const int64_t o = (int64_t)f * MC + lane;
-    part += cn[o] * cn[o];
-  }
-  double n_after = block_lane_sum<MC>(part, red, bc);
+  double n_after = block_lane_sum<MC>(slice_norm2(cs, S), red, bc);
   if (block_any(n_after < 0.25 * n_before)) {  // uniform over the block (it holds barriers)
     gs_pass<MC>(G, v.stride, v.base, v.ncut, kb, cn, un, red, bc);
-    part = 0.0;
-    for (int f = slot; f < v.ncut; f += cpp) {
-      const int64_t o = (int64_t)f * MC + lane;
-      part += cn[o] * cn[o];
-    }
-    n_after = block_lane_sum<MC>(part, red, bc);
+    n_after = block_lane_sum<MC>(slice_norm2(cs, S), red, bc);
   }
   const double nu = sqrt(n_after);
   const double inv = nu > 0.0 ? 1.0 / nu : 0.0;
-  double* y = G.y + v.base;
-  double* r = G.r + v.base;
+  const __amdgpu_buffer_rsrc_t ys = slice_rsrc(G.y + v.base, S), rs = slice_rsrc(G.r + v.base, S);
   part = 0.0;
-  for (int f = slot; f < v.ncut; f += cpp) {
-    const int64_t o = (int64_t)f * MC + lane;
-    cn[o] *= inv;
-    un[o] *= inv;
-    part += cn[o] * r[o];
+  for (int f = S.slot; f < S.ncut; f += kFaces * S.cpp) {
+    double cv[kFaces], uv[kFaces], rv[kFaces];
+#pragma unroll
+    for (int q = 0; q < kFaces; ++q) {
+      cv[q] = buf_load1(cs, S.off(f, q));
+      uv[q] = buf_load1(us, S.off(f, q));
+      rv[q] = buf_load1(rs, S.off(f, q));
+    }
+#pragma unroll
+    for (int q = 0; q < kFaces; ++q) {
+      cv[q] *= inv;
+      uv[q] *= inv;
+      buf_store1(cs, S.off(f, q), cv[q]);
+      buf_store1(us, S.off(f, q), uv[q]);
+      part += cv[q] * rv[q];
+    }
   }
   const double a = block_lane_sum<MC>(part, red, bc);
   double rp = 0.0, ym = 0.0;
-  for (int f = slot; f < v.ncut; f += cpp) {
-    const int64_t o = (int64_t)f * MC + lane;
-    y[o] += a * un[o];
-    r[o] -= a * cn[o];
-    rp += r[o] * r[o];
-    ym = fmax(ym, fabs(y[o]));
+  for (int f = S.slot; f < S.ncut; f += kFaces * S.cpp) {
+    double cv[kFaces], uv[kFaces], rv[kFaces], yv[kFaces];
+#pragma unroll
+    for (int q = 0; q < kFaces; ++q) {
+      cv[q] = buf_load1(cs, S.off(f, q));
+      uv[q] = buf_load1(us, S.off(f, q));
+      rv[q] = buf_load1(rs, S.off(f, q));
+      yv[q] = buf_load1(ys, S.off(f, q));
+    }
+#pragma unroll
+    for (int q = 0; q < kFaces; ++q) {
+      yv[q] += a * uv[q];
+      rv[q] -= a * cv[q];
+      buf_store1(ys, S.off(f, q), yv[q]);
+      buf_store1(rs, S.off(f, q), rv[q]);
+      rp += rv[q] * rv[q];
+      ym = fmax(ym, fabs(yv[q]));
+    }
   }
   const double rn2 = block_lane_sum<MC>(rp, red, bc);
   const double ymax = block_lane_max<MC>(ym, red, bc);
