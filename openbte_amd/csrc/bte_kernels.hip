@@ -1666,12 +1666,24 @@ __global__ __launch_bounds__(256) void k_modesum_part(S src) {
   const int64_t ps = part_stride(n, nf);
   for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int c = 0; c < nch; ++c) {
-      const double* e = P.part_chunk + ((int64_t)l * nch + c) * ps + (int64_t)p * (nf + 1);
-      acc[0] += e[0];
-      acc[1] += e[1];
-      if (nf >= 2) acc[2] += e[2];
-      if (nf == 3) acc[3] += e[3];
+    const double* e = P.part_chunk + (int64_t)l * nch * ps + (int64_t)p * (nf + 1);
+    // kFaces chunks' sums loaded together, added in chunk order
+    for (int c0 = 0; c0 < nch; c0 += kFaces) {
+      double v[kFaces][4];
+#pragma unroll
+      for (int q = 0; q < kFaces; ++q) {
+        const double* eq = e + (int64_t)(c0 + q < nch ? c0 + q : c0) * ps;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) v[q][k] = k <= nf ? eq[k] : 0.0;
+      }
+#pragma unroll
+      for (int q = 0; q < kFaces; ++q)
+        if (c0 + q < nch) {
+          acc[0] += v[q][0];
+          acc[1] += v[q][1];
+          if (nf >= 2) acc[2] += v[q][2];
+          if (nf == 3) acc[3] += v[q][3];
+        }
     }
     modesum_store(P, l, p, acc, stot);
   }
@@ -1695,8 +1707,9 @@ __global__ void k_reduce_cells(S src) {
     double j0 = 0.0, j1 = 0.0, j2 = 0.0;
     const int nf = n_flux(P);
     const int64_t ps = part_stride(P.n, nf);
+    const int L = P.n_dirs;
     if (P.sig_a != nullptr) {
-      for (int l = 0; l < P.n_dirs; ++l) {
+      for (int l = 0; l < L; ++l) {
         const double* e = P.part_tj + l * ps + (int64_t)P.pos[(int64_t)l * P.n + c] * 2;
         const double* dl = P.sig_dir + (int64_t)l * dim;
         s += e[0];
@@ -1705,7 +1718,7 @@ __global__ void k_reduce_cells(S src) {
         if (dim == 3) j2 = fma(dl[2], e[1], j2);
       }
     } else {
-      for (int l = 0; l < P.n_dirs; ++l) {
+      for (int l = 0; l < L; ++l) {
         const double* e = P.part_tj + l * ps + (int64_t)P.pos[(int64_t)l * P.n + c] * (dim + 1);
         s += e[0];
         j0 += e[1];
